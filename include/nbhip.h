@@ -1,0 +1,174 @@
+/*
+ * nbhip -- MI355X (gfx950) Barnes-Hut force step and KDK leapfrog for the
+ * Milkyway@Home nbody client, exposed as a plain C ABI.
+ *
+ * The library replaces the OpenCL hot path of the reference
+ * (nbody/src/nbody_cl.c + nbody/kernels/nbody_kernels.cl) behind the same
+ * call sites.  It follows the CPU semantics of the reference (tree rooted at
+ * the origin with a persistent, doubling root size; per-body opening
+ * decisions; KDK leapfrog), not the OpenCL variants.
+ *
+ * All entry points return an NBodyStatus bit set (values identical to
+ * NBodyStatus, nbody/include/nbody_types.h:541-577); 0 is success.
+ * Arrays are plain host pointers; positions, velocities and accelerations
+ * are n x 3 row-major (x, y, z per body).  The library never retains a
+ * caller pointer past the call.
+ */
+#ifndef NBHIP_H
+#define NBHIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status bits: NBodyStatus (nbody/include/nbody_types.h:541-577) ---- */
+#define NBHIP_SUCCESS                0u
+#define NBHIP_ERROR                  (1u << 1)
+#define NBHIP_TREE_STRUCTURE_ERROR   (1u << 3)
+#define NBHIP_TREE_INCEST_FATAL      (1u << 4)
+#define NBHIP_GPU_ERROR              (1u << 7)   /* NBODY_CL_ERROR */
+#define NBHIP_CAPABILITY_ERROR       (1u << 8)
+#define NBHIP_UNSUPPORTED            (1u << 11)
+#define NBHIP_USER_ERROR             (1u << 12)
+#define NBHIP_MAX_DEPTH_ERROR        (1u << 16)
+#define NBHIP_CELL_OVERFLOW_ERROR    (1u << 17)
+#define NBHIP_TREE_INCEST_NONFATAL   (1u << 24)
+#define NBHIP_STATUS_ALL_WARNINGS    0xFF000000u
+#define NBHIP_STATUS_IS_FATAL(x)     (((x) & ~NBHIP_STATUS_ALL_WARNINGS) != 0u)
+
+/* ---- opening criteria: criterion_t (nbody/include/nbody_types.h:166-173) ---- */
+#define NBHIP_TREECODE 0
+#define NBHIP_SW93     1
+#define NBHIP_BH86     2
+#define NBHIP_EXACT    3
+
+/* ---- potential component codes (nbody/include/nbody_potential_types.h) ---- */
+#define NBHIP_SPHERICAL_NONE      0
+#define NBHIP_SPHERICAL_HERNQUIST 1
+#define NBHIP_SPHERICAL_PLUMMER   2
+#define NBHIP_DISK_NONE           0
+#define NBHIP_DISK_MIYAMOTO_NAGAI 1
+#define NBHIP_HALO_NONE           0
+#define NBHIP_HALO_LOGARITHMIC    1
+#define NBHIP_HALO_NFW            2
+
+/* ---- walk arithmetic ---- */
+#define NBHIP_PRECISION_FP64  0   /* plain form: FP64 end to end */
+#define NBHIP_PRECISION_MIXED 1   /* FP32 walk, FP64 cross-batch sums, FP64 state */
+
+/* The parts of Potential (nbody/include/nbody_potential_types.h) that the
+ * supported components read. */
+typedef struct nbhip_potential {
+    int sphere_type;  double sphere_mass, sphere_scale;
+    int disk_type;    double disk_mass, disk_scale_length, disk_scale_height;
+    int disk2_type;   double disk2_mass, disk2_scale_length, disk2_scale_height;
+    int halo_type;    double halo_vhalo, halo_scale_length, halo_flatten_z;
+} nbhip_potential;
+
+/* The NBodyCtx fields on the force/integrate path
+ * (nbody/include/nbody_types.h:450-521). */
+typedef struct nbhip_ctx {
+    double eps2;            /* ctx->eps2 */
+    double theta;           /* ctx->theta */
+    double timestep;        /* ctx->timestep */
+    double tree_rsize;      /* ctx->treeRSize: initial root side, power of two */
+    int criterion;          /* ctx->criterion */
+    int use_quad;           /* ctx->useQuad */
+    int allow_incest;       /* ctx->allowIncest */
+    int use_potential;      /* ctx->potentialType: 0 NONE, 1 DEFAULT */
+    int precision;          /* NBHIP_PRECISION_* */
+    int reserved;
+    nbhip_potential pot;    /* ctx->pot */
+} nbhip_ctx;
+
+typedef struct nbhip_state nbhip_state;
+
+/* Tree statistics of the last force evaluation. */
+typedef struct nbhip_tree_info {
+    double rsize;           /* st->tree.rsize */
+    int ncell;              /* st->tree.cellUsed */
+    int max_depth;          /* st->tree.maxDepth */
+    int nnode;              /* cells + massive bodies */
+    int nmassive;
+    unsigned step;          /* st->step */
+    unsigned status;        /* sticky status bits */
+} nbhip_tree_info;
+
+/* Replaces nbInitCL + nbInitNBodyStateCL (nbody/src/nbody_types.c:258-341):
+ * validates the context (custom/unsupported potentials -> NBHIP_UNSUPPORTED,
+ * as nbody_types.c:297-301), selects the device and reserves buffers for
+ * `nbody` bodies. */
+unsigned nbhip_create(const nbhip_ctx* ctx, int device, int nbody, nbhip_state** out);
+
+/* Replaces nbMarshalBodies(st, TRUE) (nbody/src/nbody_cl.c:2211-2269):
+ * host bodies -> device.  mass may contain zeros (test particles, never
+ * inserted in the tree: nbody/src/nbody_tree.c:438-442). */
+unsigned nbhip_upload(nbhip_state* st, const double* pos, const double* vel,
+                      const double* mass);
+
+/* Replaces nbMarshalBodies(st, FALSE): device -> host.  Any pointer may be
+ * NULL.  acc receives st->acctab (the last force evaluation). */
+unsigned nbhip_download(nbhip_state* st, double* pos, double* vel, double* acc);
+
+/* Replaces nbGravMap (nbody/src/nbody_grav.c:305-328): tree build + walk +
+ * external potential into the acceleration table, no integration. */
+unsigned nbhip_gravity(nbhip_state* st);
+
+/* Replaces nbStepSystem / nbStepSystemPlain (nbody/src/nbody.c:182-200,
+ * nbody/src/nbody_plain.c:329-364): one KDK step, step counter += 1. */
+unsigned nbhip_step(nbhip_state* st);
+
+/* Replaces nbRunSystem / nbRunSystemPlain (nbody/src/nbody.c:202-212,
+ * nbody/src/nbody_plain.c:366-461) without the output hooks: one force
+ * evaluation, then `nsteps` KDK steps, stopping at the first fatal status. */
+unsigned nbhip_run(nbhip_state* st, unsigned nsteps);
+
+/* Runs `nsteps` KDK steps without the leading force evaluation (for callers
+ * that already called nbhip_gravity, e.g. after upload). */
+unsigned nbhip_steps(nbhip_state* st, unsigned nsteps);
+
+unsigned nbhip_tree_info_get(nbhip_state* st, nbhip_tree_info* info);
+
+/* Pre-order node table of the last tree (cells and massive bodies in
+ * More/Next threading order, nbody/src/nbody_tree.c:127-150).  Arrays hold
+ * info.nnode entries; kind 0 cell / 1 body; ref = body index (original order)
+ * or -1; skip = pre-order index of Next(); level = cell level or -1;
+ * pos/mass = centre of mass (cells) or body; quad 6 per node (xx xy xz yy yz zz). */
+unsigned nbhip_tree_download(nbhip_state* st, int* kind, int* ref, int* skip, int* level,
+                             double* pos, double* mass, double* rcrit2, double* quad);
+
+/* FP64 direct sum (nbGravity_Exact, nbody/src/nbody_grav.c:205-227) of the
+ * self-gravity on the `nsample` bodies listed in idx, against all bodies. */
+unsigned nbhip_direct_accel(nbhip_state* st, int nsample, const int* idx, double* acc);
+
+/* Self-gravity only (no external potential) from the tree walk of the
+ * current tree for all bodies, in the precision of the context. */
+unsigned nbhip_walk_accel(nbhip_state* st, double* acc);
+
+/* Walk counters of the last force evaluation: body-body interactions,
+ * body-cell interactions, opened cells (flop accounting). */
+unsigned nbhip_walk_counters(nbhip_state* st, unsigned long long* counts3);
+
+/* Per-phase device time of the last step in milliseconds, measured with HIP
+ * events on the library's stream: [0] drift+bbox, [1] keys+sort, [2] build,
+ * [3] summarize, [4] walk (+potential, kick).  Requires profiling enabled. */
+unsigned nbhip_set_profiling(nbhip_state* st, int enable);
+unsigned nbhip_phase_times(nbhip_state* st, double* ms5);
+
+/* Waits for all work queued by the library. */
+unsigned nbhip_sync(nbhip_state* st);
+
+/* Replaces the OpenCL teardown inside destroyNBodyState
+ * (nbody/src/nbody_types.c:142-175). */
+void nbhip_release(nbhip_state* st);
+
+/* Text of the last error of this thread (never NULL). */
+const char* nbhip_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NBHIP_H */

@@ -1,0 +1,198 @@
+/*
+ * Device helpers shared by the nbhip kernels: fixed-point Morton keys,
+ * the external Milky Way potential and small wave64 utilities.
+ *
+ * FP64 expressions keep the reference's operation order (the library is
+ * compiled with -ffp-contract=off) so that the plain form reproduces the CPU
+ * path bit for bit wherever the operations themselves are correctly rounded.
+ */
+#ifndef NBHIP_DEVICE_H
+#define NBHIP_DEVICE_H
+
+#include <hip/hip_runtime.h>
+#include "nbhip_internal.h"
+
+/* ---------------------------------------------------------------------- */
+/* Fixed-point coordinates and Morton prefixes                            */
+/* ---------------------------------------------------------------------- */
+/* u = floor((x + R/2) * 2^63 / R), clamped to [0, 2^63 - 1].  With R a
+ * power of two, x * 2^63/R is an exact scaling, so bit (62 - l) of u equals
+ * the reference's comparison "midpoint <= x" at level l + 1 (nbSubIndex,
+ * nbody/src/nbody_tree.c:36-52): every midpoint is an integer in this
+ * fixed-point frame and the comparison is exact. */
+__device__ __forceinline__ unsigned long long nbh_fixed(double x, double key_scale)
+{
+    double t = floor(x * key_scale);
+    long long it = (long long) t + (1LL << 62);
+    if (it < 0) it = 0;
+    if (it > 0x7FFFFFFFFFFFFFFFLL) it = 0x7FFFFFFFFFFFFFFFLL;
+    return (unsigned long long) it;
+}
+
+/* Spread the low 21 bits of v so that bit i lands on bit 3i. */
+__device__ __forceinline__ unsigned long long nbh_spread21(unsigned long long v)
+{
+    v &= 0x1FFFFFull;
+    v = (v | (v << 32)) & 0x001F00000000FFFFull;
+    v = (v | (v << 16)) & 0x001F0000FF0000FFull;
+    v = (v | (v << 8)) & 0x100F00F00F00F00Full;
+    v = (v | (v << 4)) & 0x10C30C30C30C30C3ull;
+    v = (v | (v << 2)) & 0x1249249249249249ull;
+    return v;
+}
+
+/* Morton word of 21 levels; x is the most significant bit of each level
+ * (octant index = 4*bx + 2*by + bz, nbody/src/nbody_tree.c:41-49). */
+__device__ __forceinline__ unsigned long long nbh_morton21(unsigned long long bx,
+                                                           unsigned long long by,
+                                                           unsigned long long bz)
+{
+    return (nbh_spread21(bx) << 2) | (nbh_spread21(by) << 1) | nbh_spread21(bz);
+}
+
+/* Common Morton prefix, in bits (0..189), of two bodies given their
+ * 63-bit fixed-point coordinates. */
+__device__ __forceinline__ int nbh_prefix_bits(unsigned long long ax, unsigned long long ay,
+                                               unsigned long long az, unsigned long long bx,
+                                               unsigned long long by, unsigned long long bz)
+{
+    int ex = __clzll((long long) (ax ^ bx)) - 1;   /* bit 63 is always clear */
+    int ey = __clzll((long long) (ay ^ by)) - 1;
+    int ez = __clzll((long long) (az ^ bz)) - 1;
+    int m = min(ex, min(ey, ez));
+    if (m >= 63) return 189;
+    return 3 * m + (ex == m ? 0 : (ey == m ? 1 : 2));
+}
+
+/* Number of octree levels shared by a prefix of `bits` bits; -1 for none. */
+__device__ __forceinline__ int nbh_levels(int bits)
+{
+    return bits < 0 ? -1 : min(bits, 189) / 3;
+}
+
+/* ---------------------------------------------------------------------- */
+/* Correctly rounded powers used by the potentials (mw_pow -> crlibm        */
+/* pow_rn, milkyway/include/milkyway_math_double.h:59): x^1.5 and t^3 are  */
+/* formed in double-double and rounded once.                               */
+/* ---------------------------------------------------------------------- */
+__device__ __forceinline__ double nbh_pow1p5(double x)
+{
+    double s = sqrt(x);
+    double e = fma(-s, s, x);
+    double ds = e / (2.0 * s);
+    double p = x * s;
+    double pe = fma(x, s, -p);
+    return p + (pe + x * ds);
+}
+
+__device__ __forceinline__ double nbh_cube(double t)
+{
+    double h = t * t;
+    double l = fma(t, t, -h);
+    double p = h * t;
+    double pe = fma(h, t, -p);
+    return p + (pe + l * t);
+}
+
+/* nbExtAcceleration (nbody/src/nbody_potential.c:627-762) restricted to the
+ * supported components: Miyamoto-Nagai disk (:173-194), logarithmic halo
+ * (:488-504), NFW halo (:506-515), Hernquist (:157-162) and Plummer
+ * (:164-169) spheres. */
+__device__ __forceinline__ void nbh_ext_accel(const nbhip_potential& pot, double x, double y,
+                                              double z, double& oax, double& oay, double& oaz)
+{
+    const double limit = 0.00390625;
+    double ab = sqrt(z * z + (y * y + x * x));
+    double r = (double) (ab <= limit) * limit + (double) (ab > limit) * ab;
+    double a0, a1, a2, t0, t1, t2;
+
+    if (pot.disk_type == NBHIP_DISK_MIYAMOTO_NAGAI) {
+        double M = pot.disk_mass, da = pot.disk_scale_length, db = pot.disk_scale_height;
+        double zp = sqrt(z * z + db * db);
+        double azp = da + zp;
+        double rp = x * x + y * y + azp * azp;
+        double rth = nbh_pow1p5(rp);
+        a0 = -M * x / rth;
+        a1 = -M * y / rth;
+        a2 = -M * z * azp / (zp * rth);
+    } else {
+        a0 = a1 = a2 = 0.0;
+    }
+    if (pot.disk2_type == NBHIP_DISK_MIYAMOTO_NAGAI) {
+        double M = pot.disk2_mass, da = pot.disk2_scale_length, db = pot.disk2_scale_height;
+        double zp = sqrt(z * z + db * db);
+        double azp = da + zp;
+        double rp = x * x + y * y + azp * azp;
+        double rth = nbh_pow1p5(rp);
+        t0 = -M * x / rth;
+        t1 = -M * y / rth;
+        t2 = -M * z * azp / (zp * rth);
+    } else {
+        t0 = t1 = t2 = 0.0;
+    }
+    a0 += t0; a1 += t1; a2 += t2;
+
+    if (pot.halo_type == NBHIP_HALO_LOGARITHMIC) {
+        double v0 = pot.halo_vhalo, q = pot.halo_flatten_z, d = pot.halo_scale_length;
+        double zq = z / q;
+        double denom = d * d + x * x + y * y + zq * zq;
+        double k = -2.0 * v0 * v0 / denom;
+        t0 = k * x;
+        t1 = k * y;
+        t2 = k * z / (q * q);
+    } else if (pot.halo_type == NBHIP_HALO_NFW) {
+        double ha = pot.halo_scale_length;
+        double M = pot.halo_vhalo * pot.halo_vhalo * ha / 0.2162165954;
+        double ar = ha + r;
+        double c = (-M / (r * r)) * (log(ar / ha) / r - 1 / ar);
+        t0 = c * x; t1 = c * y; t2 = c * z;
+    } else {
+        t0 = t1 = t2 = 0.0;
+    }
+    a0 += t0; a1 += t1; a2 += t2;
+
+    if (pot.sphere_type == NBHIP_SPHERICAL_HERNQUIST) {
+        double tmp = pot.sphere_scale + r;
+        double s = -pot.sphere_mass / (r * (tmp * tmp));
+        t0 = s * x; t1 = s * y; t2 = s * z;
+    } else if (pot.sphere_type == NBHIP_SPHERICAL_PLUMMER) {
+        double sc = pot.sphere_scale;
+        double tmp = sqrt(sc * sc + r * r);
+        double s = -pot.sphere_mass / nbh_cube(tmp);
+        t0 = s * x; t1 = s * y; t2 = s * z;
+    } else {
+        t0 = t1 = t2 = 0.0;
+    }
+    oax = a0 + t0;
+    oay = a1 + t1;
+    oaz = a2 + t2;
+}
+
+/* ---------------------------------------------------------------------- */
+/* wave64 helpers                                                          */
+/* ---------------------------------------------------------------------- */
+__device__ __forceinline__ unsigned long long nbh_lanemask_lt()
+{
+    unsigned lane = __lane_id();
+    return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+__device__ __forceinline__ double nbh_wave_max(double v)
+{
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+__device__ __forceinline__ unsigned long long nbh_wave_sum_u64(unsigned long long v)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__device__ __forceinline__ unsigned nbh_wave_sum_u32(unsigned v)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+#endif

@@ -1,0 +1,864 @@
+/*
+ * Tree construction on gfx950: root-size update, Morton keys, LSD radix sort,
+ * Karras binary radix tree, octree emission in pre-order and bottom-up
+ * summarisation (centre of mass, critical radius, quadrupole).
+ *
+ * The octree is exactly the reference's (nbody/src/nbody_tree.c): root at the
+ * origin with side st->tree.rsize, midpoints at exact dyadic positions, and a
+ * cell wherever two or more massive bodies share a prefix -- including the
+ * single-child chain cells that serial insertion creates (nbLoadBody,
+ * :242-282).  Cells are the prefixes (level l, first body a) of the sorted
+ * 189-bit keys; with delta_a the common prefix of sorted bodies a and a+1,
+ * the cells whose first body is a are levels L(delta_{a-1})+1 .. L(delta_a),
+ * L = floor(bits/3).  Pre-order (More/Next threading, :127-150) lists, for
+ * each sorted body a, its new cells by level, then the body itself; so
+ * pre(body a) = a + #cells with first body <= a and Next(cell [lo,hi]) =
+ * pre(body hi) + 1.
+ */
+#include "nbhip_device.h"
+
+#include <cfloat>
+
+/* ---------------------------------------------------------------------- */
+/* bounding box (+ optional opening half-kick and drift)                  */
+/* ---------------------------------------------------------------------- */
+/* advancePosVel (nbody/src/nbody_plain.c:268-284) fused with expandBox's
+ * max-|x| scan (nbody/src/nbody_tree.c:156-176). */
+__global__ __launch_bounds__(256) void k_drift_bbox(int n, int drift, double dt, double dtHalf,
+                                                    double* __restrict__ px, double* __restrict__ py,
+                                                    double* __restrict__ pz, double* __restrict__ vx,
+                                                    double* __restrict__ vy, double* __restrict__ vz,
+                                                    const double* __restrict__ ax,
+                                                    const double* __restrict__ ay,
+                                                    const double* __restrict__ az,
+                                                    DevScalars* __restrict__ scal)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    double mx = 0.0;
+    if (i < n) {
+        double x = px[i], y = py[i], z = pz[i];
+        if (drift) {
+            double ux = vx[i] + dtHalf * (ax[i] + 0.0);
+            double uy = vy[i] + dtHalf * (ay[i] + 0.0);
+            double uz = vz[i] + dtHalf * (az[i] + 0.0);
+            x = x + dt * ux;
+            y = y + dt * uy;
+            z = z + dt * uz;
+            vx[i] = ux; vy[i] = uy; vz[i] = uz;
+            px[i] = x; py[i] = y; pz[i] = z;
+        }
+        mx = fmax(mx, fabs(x));
+        mx = fmax(mx, fabs(y));
+        mx = fmax(mx, fabs(z));
+    }
+    mx = nbh_wave_max(mx);
+    __shared__ double wmax[4];
+    if (__lane_id() == 0) wmax[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = fmax(fmax(wmax[0], wmax[1]), fmax(wmax[2], wmax[3]));
+        atomicMax(&scal->xyzmax_bits, (unsigned long long) __double_as_longlong(m));
+    }
+}
+
+/* Root doubling rule (expandBox, nbody/src/nbody_tree.c:173-175) and per-build
+ * resets. */
+__global__ void k_rsize(DevScalars* scal)
+{
+    double xm = __longlong_as_double((long long) scal->xyzmax_bits);
+    double r = scal->rsize;
+    if (!(xm <= DBL_MAX)) {
+        scal->status |= NBHIP_ERROR;
+    } else {
+        while (r < 2.0 * xm) r *= 2.0;
+    }
+    scal->rsize = r;
+    scal->key_scale = 9223372036854775808.0 / r;
+    scal->xyzmax_bits = 0ull;
+    scal->nmassive = 0;
+    scal->ncell = 0;
+    scal->nnode = 0;
+    scal->maxdepth = 0;
+    scal->counters[0] = scal->counters[1] = scal->counters[2] = 0ull;
+}
+
+/* ---------------------------------------------------------------------- */
+/* keys                                                                   */
+/* ---------------------------------------------------------------------- */
+/* Massless bodies are not inserted (nbMakeTree, nbody/src/nbody_tree.c:438-442):
+ * their key sorts after every massive key. */
+__global__ __launch_bounds__(256) void k_keys(int n, const double* __restrict__ px,
+                                              const double* __restrict__ py,
+                                              const double* __restrict__ pz,
+                                              const double* __restrict__ m,
+                                              unsigned long long* __restrict__ key,
+                                              unsigned* __restrict__ perm,
+                                              DevScalars* __restrict__ scal)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    unsigned massive = 0;
+    if (i < n) {
+        const double ks = scal->key_scale;
+        unsigned long long k = ~0ull;
+        if (m[i] != 0.0) {
+            unsigned long long ux = nbh_fixed(px[i], ks);
+            unsigned long long uy = nbh_fixed(py[i], ks);
+            unsigned long long uz = nbh_fixed(pz[i], ks);
+            k = nbh_morton21(ux >> 42, uy >> 42, uz >> 42);
+            massive = 1;
+        }
+        key[i] = k;
+        perm[i] = (unsigned) i;
+    }
+    massive = nbh_wave_sum_u32(massive);
+    if (__lane_id() == 0 && massive) atomicAdd(&scal->nmassive, (int) massive);
+}
+
+/* ---------------------------------------------------------------------- */
+/* LSD radix sort: 8-bit digits, 256 threads x 16 keys per block, stable   */
+/* wave64 multisplit ranking.                                              */
+/* ---------------------------------------------------------------------- */
+__global__ __launch_bounds__(256) void k_radix_hist(int n, int shift,
+                                                    const unsigned long long* __restrict__ key,
+                                                    unsigned* __restrict__ hist, int nb)
+{
+    __shared__ unsigned cnt[256];
+    cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const int base = blockIdx.x * NBH_SORT_TILE;
+    #pragma unroll 4
+    for (int r = 0; r < NBH_SORT_TILE / 256; ++r) {
+        int idx = base + r * 256 + threadIdx.x;
+        if (idx < n) atomicAdd(&cnt[(unsigned) (key[idx] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[threadIdx.x * nb + blockIdx.x] = cnt[threadIdx.x];
+}
+
+/* Exclusive scan of the digit-major histogram (256 * nb entries), one block. */
+__global__ __launch_bounds__(1024) void k_radix_scan(unsigned* __restrict__ hist, int nb)
+{
+    __shared__ unsigned wsum[16];
+    const int total = 256 * nb;
+    const int chunk = (total + 1023) / 1024;
+    const int t = threadIdx.x;
+    const int b0 = t * chunk;
+    unsigned s = 0;
+    for (int k = 0; k < chunk; ++k) {
+        int idx = b0 + k;
+        if (idx < total) s += hist[idx];
+    }
+    /* block exclusive scan of s */
+    unsigned v = s;
+    for (int off = 1; off < 64; off <<= 1) {
+        unsigned u = __shfl_up(v, off, 64);
+        if ((int) __lane_id() >= off) v += u;
+    }
+    if (__lane_id() == 63) wsum[t >> 6] = v;
+    __syncthreads();
+    if (t < 16) {
+        unsigned w = wsum[t];
+        for (int off = 1; off < 16; off <<= 1) {
+            unsigned u = __shfl_up(w, off, 16);
+            if (t >= off) w += u;
+        }
+        wsum[t] = w;
+    }
+    __syncthreads();
+    unsigned excl = v - s + ((t >> 6) ? wsum[(t >> 6) - 1] : 0u);
+    for (int k = 0; k < chunk; ++k) {
+        int idx = b0 + k;
+        if (idx < total) {
+            unsigned h = hist[idx];
+            hist[idx] = excl;
+            excl += h;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_radix_scatter(int n, int shift,
+                                                       const unsigned long long* __restrict__ kin,
+                                                       const unsigned* __restrict__ vin,
+                                                       unsigned long long* __restrict__ kout,
+                                                       unsigned* __restrict__ vout,
+                                                       const unsigned* __restrict__ hist, int nb)
+{
+    __shared__ unsigned goff[256];
+    __shared__ unsigned running[256];
+    __shared__ unsigned wcnt[4][256];
+    const int t = threadIdx.x;
+    const int w = t >> 6;
+    goff[t] = hist[t * nb + blockIdx.x];
+    running[t] = 0;
+    wcnt[0][t] = wcnt[1][t] = wcnt[2][t] = wcnt[3][t] = 0;
+    __syncthreads();
+    const unsigned long long lt = nbh_lanemask_lt();
+    const int base = blockIdx.x * NBH_SORT_TILE;
+    for (int r = 0; r < NBH_SORT_TILE / 256; ++r) {
+        const int idx = base + r * 256 + t;
+        const bool valid = idx < n;
+        unsigned long long k = valid ? kin[idx] : 0ull;
+        unsigned v = valid ? vin[idx] : 0u;
+        unsigned d = (unsigned) (k >> shift) & 255u;
+        unsigned long long peers = __ballot(valid);
+        #pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            unsigned long long bb = __ballot(valid && ((d >> b) & 1u));
+            peers &= ((d >> b) & 1u) ? bb : ~bb;
+        }
+        unsigned rank = (unsigned) __popcll(peers & lt);
+        if (valid && rank == 0) wcnt[w][d] = (unsigned) __popcll(peers);
+        __syncthreads();
+        if (valid) {
+            unsigned before = running[d];
+            for (int ww = 0; ww < w; ++ww) before += wcnt[ww][d];
+            unsigned pos = goff[d] + before + rank;
+            kout[pos] = k;
+            vout[pos] = v;
+        }
+        __syncthreads();
+        running[t] += wcnt[0][t] + wcnt[1][t] + wcnt[2][t] + wcnt[3][t];
+        wcnt[0][t] = wcnt[1][t] = wcnt[2][t] = wcnt[3][t] = 0;
+        __syncthreads();
+    }
+}
+
+/* ---------------------------------------------------------------------- */
+/* ties in the first 21 levels: order by levels 22-63                      */
+/* ---------------------------------------------------------------------- */
+__device__ __forceinline__ void nbh_deep_words(const double* px, const double* py,
+                                               const double* pz, double ks, unsigned j,
+                                               unsigned long long& w1, unsigned long long& w2)
+{
+    unsigned long long ux = nbh_fixed(px[j], ks), uy = nbh_fixed(py[j], ks),
+                       uz = nbh_fixed(pz[j], ks);
+    const unsigned long long m21 = 0x1FFFFFull;
+    w1 = nbh_morton21((ux >> 21) & m21, (uy >> 21) & m21, (uz >> 21) & m21);
+    w2 = nbh_morton21(ux & m21, uy & m21, uz & m21);
+}
+
+__global__ __launch_bounds__(256) void k_tiefix(const unsigned long long* __restrict__ key,
+                                                unsigned* __restrict__ perm,
+                                                const double* __restrict__ px,
+                                                const double* __restrict__ py,
+                                                const double* __restrict__ pz,
+                                                const DevScalars* __restrict__ scal)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int M = scal->nmassive;
+    if (i >= M) return;
+    const unsigned long long k = key[i];
+    if (i > 0 && key[i - 1] == k) return;
+    if (i + 1 >= M || key[i + 1] != k) return;
+    int e = i + 1;
+    while (e < M && key[e] == k) ++e;
+    const double ks = scal->key_scale;
+    for (int a = i + 1; a < e; ++a) {
+        unsigned v = perm[a];
+        unsigned long long v1, v2;
+        nbh_deep_words(px, py, pz, ks, v, v1, v2);
+        int b = a - 1;
+        while (b >= i) {
+            unsigned long long b1, b2;
+            nbh_deep_words(px, py, pz, ks, perm[b], b1, b2);
+            if (b1 > v1 || (b1 == v1 && b2 > v2)) {
+                perm[b + 1] = perm[b];
+                --b;
+            } else {
+                break;
+            }
+        }
+        perm[b + 1] = v;
+    }
+}
+
+/* ---------------------------------------------------------------------- */
+/* sorted gather, adjacent prefixes, cells-per-first-body                  */
+/* ---------------------------------------------------------------------- */
+__global__ __launch_bounds__(256) void k_gather(int n, const unsigned* __restrict__ perm,
+                                                const double* __restrict__ px,
+                                                const double* __restrict__ py,
+                                                const double* __restrict__ pz,
+                                                const double* __restrict__ m,
+                                                double4* __restrict__ spos,
+                                                unsigned long long* __restrict__ su,
+                                                const DevScalars* __restrict__ scal)
+{
+    const int s = blockIdx.x * 256 + threadIdx.x;
+    if (s >= n) return;
+    const unsigned j = perm[s];
+    double x = px[j], y = py[j], z = pz[j];
+    spos[s] = make_double4(x, y, z, m[j]);
+    if (s < scal->nmassive) {
+        const double ks = scal->key_scale;
+        su[s] = nbh_fixed(x, ks);
+        su[n + s] = nbh_fixed(y, ks);
+        su[2 * n + s] = nbh_fixed(z, ks);
+    }
+}
+
+/* Prefix length between sorted massive bodies i and j, with equal keys
+ * ordered by index (Karras 2012, section 4). */
+__device__ __forceinline__ int nbh_kdelta(const unsigned long long* __restrict__ su, int n, int M,
+                                          int i, int j)
+{
+    if (j < 0 || j >= M) return -1;
+    int d = nbh_prefix_bits(su[i], su[n + i], su[2 * n + i], su[j], su[n + j], su[2 * n + j]);
+    if (d == 189) d = 189 + __clz((unsigned) (i ^ j));
+    return d;
+}
+
+__global__ __launch_bounds__(256) void k_delta_count(int n, const unsigned long long* __restrict__ su,
+                                                     int* __restrict__ dadj,
+                                                     unsigned* __restrict__ ccount,
+                                                     const DevScalars* __restrict__ scal)
+{
+    const int a = blockIdx.x * 256 + threadIdx.x;
+    const int M = scal->nmassive;
+    if (a >= M) return;
+    int d = nbh_kdelta(su, n, M, a, a + 1);
+    dadj[a] = d;
+    int c;
+    if (a == 0) {
+        c = max(nbh_levels(d), 0) + 1;
+    } else {
+        int dp = nbh_kdelta(su, n, M, a - 1, a);
+        c = max(0, nbh_levels(d) - nbh_levels(dp));
+    }
+    ccount[a] = (unsigned) c;
+}
+
+/* ---------------------------------------------------------------------- */
+/* inclusive scan of cells-per-first-body (length nmassive)               */
+/* ---------------------------------------------------------------------- */
+__device__ __forceinline__ unsigned nbh_block1024_excl(unsigned s, unsigned* wsum,
+                                                       unsigned* total)
+{
+    const int t = threadIdx.x;
+    unsigned v = s;
+    for (int off = 1; off < 64; off <<= 1) {
+        unsigned u = __shfl_up(v, off, 64);
+        if ((int) __lane_id() >= off) v += u;
+    }
+    if (__lane_id() == 63) wsum[t >> 6] = v;
+    __syncthreads();
+    if (t < 16) {
+        unsigned w = wsum[t];
+        for (int off = 1; off < 16; off <<= 1) {
+            unsigned u = __shfl_up(w, off, 16);
+            if (t >= off) w += u;
+        }
+        wsum[t] = w;
+    }
+    __syncthreads();
+    *total = wsum[15];
+    return v - s + ((t >> 6) ? wsum[(t >> 6) - 1] : 0u);
+}
+
+__global__ __launch_bounds__(1024) void k_scan_tiles(unsigned* __restrict__ data,
+                                                     unsigned* __restrict__ part,
+                                                     const DevScalars* __restrict__ scal)
+{
+    __shared__ unsigned wsum[16];
+    const int M = scal->nmassive;
+    const int base = blockIdx.x * NBH_SCAN_TILE + threadIdx.x * 4;
+    if (blockIdx.x * NBH_SCAN_TILE >= M) return;
+    unsigned v[4];
+    #pragma unroll
+    for (int k = 0; k < 4; ++k) v[k] = (base + k < M) ? data[base + k] : 0u;
+    unsigned s = v[0] + v[1] + v[2] + v[3];
+    unsigned tot;
+    unsigned ex = nbh_block1024_excl(s, wsum, &tot);
+    #pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        ex += v[k];
+        if (base + k < M) data[base + k] = ex;
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_part(unsigned* __restrict__ part,
+                                                    const DevScalars* __restrict__ scal)
+{
+    __shared__ unsigned wsum[16];
+    const int M = scal->nmassive;
+    const int nb = (M + NBH_SCAN_TILE - 1) / NBH_SCAN_TILE;
+    const int base = threadIdx.x * 4;
+    unsigned v[4];
+    #pragma unroll
+    for (int k = 0; k < 4; ++k) v[k] = (base + k < nb) ? part[base + k] : 0u;
+    unsigned s = v[0] + v[1] + v[2] + v[3];
+    unsigned tot;
+    unsigned ex = nbh_block1024_excl(s, wsum, &tot);
+    #pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        ex += v[k];
+        if (base + k < nb) part[base + k] = ex;
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_scan_add(unsigned* __restrict__ data,
+                                                   const unsigned* __restrict__ part,
+                                                   const DevScalars* __restrict__ scal)
+{
+    const int M = scal->nmassive;
+    if (blockIdx.x == 0) return;
+    const int base = blockIdx.x * NBH_SCAN_TILE + threadIdx.x * 4;
+    const unsigned add = part[blockIdx.x - 1];
+    #pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (base + k < M) data[base + k] += add;
+}
+
+/* ---------------------------------------------------------------------- */
+/* node counts, trivial trees                                              */
+/* ---------------------------------------------------------------------- */
+__device__ __forceinline__ double nbh_rcrit2(const nbhip_ctx& c, double mx, double my, double mz,
+                                             double rsize, double cx, double cy, double cz,
+                                             double psize)
+{
+    /* findRCrit (nbody/src/nbody_tree.c:308-343) */
+    double rc;
+    if (c.theta == 0.0) {
+        rc = 2.0 * rsize;
+        return rc * rc;
+    }
+    if (c.criterion == NBHIP_TREECODE) {
+        double dx = cx - mx, dy = cy - my, dz = cz - mz;
+        rc = psize / c.theta + sqrt(dx * dx + dy * dy + dz * dz);
+        return rc * rc;
+    }
+    if (c.criterion == NBHIP_SW93) {
+        double d0 = cx - (mx - 0.5 * psize);
+        double e0 = fmax(d0, psize - d0);
+        double bmax2 = e0 * e0;
+        double d1 = cy - (my - 0.5 * psize);
+        double e1 = fmax(d1, psize - d1);
+        bmax2 += e1 * e1;
+        double d2 = cz - (mz - 0.5 * psize);
+        double e2 = fmax(d2, psize - d2);
+        bmax2 += e2 * e2;
+        return bmax2 / (c.theta * c.theta);
+    }
+    rc = psize / c.theta;
+    return rc * rc;
+}
+
+__global__ void k_node_counts(const unsigned* __restrict__ cincl, NodeW* __restrict__ node,
+                              double* __restrict__ nquad, double* __restrict__ nmid,
+                              int* __restrict__ nlevel, int* __restrict__ nparent,
+                              unsigned* __restrict__ nchild, unsigned* __restrict__ narrive,
+                              int node_cap, nbhip_ctx ctx, DevScalars* __restrict__ scal)
+{
+    const int M = scal->nmassive;
+    const int C = M > 0 ? (int) cincl[M - 1] : 1;
+    const int nn = M + C;
+    scal->ncell = C;
+    scal->nnode = nn;
+    if (nn > node_cap) {
+        scal->status |= NBHIP_CELL_OVERFLOW_ERROR;
+        return;
+    }
+    if (M <= 1) {
+        /* root only (M == 0) or root + one body (M == 1) */
+        NodeW r;
+        r.x = r.y = r.z = 0.0;
+        r.m = 0.0;
+        r.rcrit2 = nbh_rcrit2(ctx, 0.0, 0.0, 0.0, scal->rsize, 0.0, 0.0, 0.0, scal->rsize);
+        r.next = M == 0 ? 1 : 2;
+        r.body = -1;
+        node[0] = r;
+        for (int k = 0; k < 6; ++k) nquad[k] = 0.0;
+        nmid[0] = nmid[1] = nmid[2] = 0.0;
+        nlevel[0] = 0;
+        nparent[0] = -1;
+        nchild[0] = 0;
+        narrive[0] = 0;
+    }
+}
+
+/* ---------------------------------------------------------------------- */
+/* Karras binary radix tree                                                */
+/* ---------------------------------------------------------------------- */
+__global__ __launch_bounds__(256) void k_karras(int n, const unsigned long long* __restrict__ su,
+                                                int* __restrict__ blo, int* __restrict__ bhi,
+                                                int* __restrict__ bdelta, int* __restrict__ bparent,
+                                                const DevScalars* __restrict__ scal)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int M = scal->nmassive;
+    if (i >= M - 1) return;
+    if (scal->status & NBHIP_CELL_OVERFLOW_ERROR) return;
+    const int dr = nbh_kdelta(su, n, M, i, i + 1);
+    const int dl = nbh_kdelta(su, n, M, i, i - 1);
+    const int d = dr > dl ? 1 : -1;
+    const int dmin = d > 0 ? dl : dr;
+    int lmax = 2;
+    while (nbh_kdelta(su, n, M, i, i + lmax * d) > dmin) lmax <<= 1;
+    int l = 0;
+    for (int t = lmax >> 1; t >= 1; t >>= 1)
+        if (nbh_kdelta(su, n, M, i, i + (l + t) * d) > dmin) l += t;
+    const int j = i + l * d;
+    const int dnode = nbh_kdelta(su, n, M, i, j);
+    int s = 0;
+    for (int div = 2;; div <<= 1) {
+        int t = (l + div - 1) / div;
+        if (nbh_kdelta(su, n, M, i, i + (s + t) * d) > dnode) s += t;
+        if (t <= 1) break;
+    }
+    const int gamma = i + s * d + min(d, 0);
+    const int lo = min(i, j), hi = max(i, j);
+    blo[i] = lo;
+    bhi[i] = hi;
+    bdelta[i] = dnode;
+    const int left = (gamma == lo) ? (M - 1) + gamma : gamma;
+    const int right = (gamma + 1 == hi) ? (M - 1) + gamma + 1 : gamma + 1;
+    bparent[left] = i;
+    bparent[right] = i;
+    if (i == 0) bparent[0] = -1;
+}
+
+/* ---------------------------------------------------------------------- */
+/* octree emission in pre-order                                            */
+/* ---------------------------------------------------------------------- */
+struct EmitView {
+    const int* blo;
+    const int* bdelta;
+    const int* bparent;
+    const int* dadj;
+    const unsigned* cincl;
+};
+
+__device__ __forceinline__ int nbh_first_level(const EmitView& v, int lo)
+{
+    return lo == 0 ? 0 : nbh_levels(v.dadj[lo - 1]) + 1;
+}
+
+__device__ __forceinline__ int nbh_excl(const EmitView& v, int lo)
+{
+    return lo == 0 ? 0 : (int) v.cincl[lo - 1];
+}
+
+__device__ __forceinline__ int nbh_parent_levels(const EmitView& v, int k)
+{
+    return k == 0 ? -1 : nbh_levels(v.bdelta[v.bparent[k]]);
+}
+
+/* Pre-order index of the deepest octree cell owned by the nearest binary
+ * ancestor-or-self of k that owns any cell. */
+__device__ __forceinline__ int nbh_owner_pre(const EmitView& v, int k, int M)
+{
+    for (int guard = 0; k > 0 && nbh_levels(v.bdelta[k]) <= nbh_parent_levels(v, k); ++guard) {
+        k = v.bparent[k];
+        if (guard > 4 * 192 || k < 0 || k >= M - 1) return -1;
+    }
+    if (k < 0 || k >= M - 1) return -1;
+    const int lo = v.blo[k];
+    return lo + nbh_excl(v, lo) + (nbh_levels(v.bdelta[k]) - nbh_first_level(v, lo));
+}
+
+__global__ __launch_bounds__(256) void k_emit_cells(const int* __restrict__ blo,
+                                                    const int* __restrict__ bhi,
+                                                    const int* __restrict__ bdelta,
+                                                    const int* __restrict__ bparent,
+                                                    const int* __restrict__ dadj,
+                                                    const unsigned* __restrict__ cincl,
+                                                    const unsigned long long* __restrict__ su,
+                                                    int n, NodeW* __restrict__ node,
+                                                    double* __restrict__ nmid,
+                                                    int* __restrict__ nlevel,
+                                                    int* __restrict__ nparent,
+                                                    unsigned* __restrict__ nchild,
+                                                    unsigned* __restrict__ narrive,
+                                                    DevScalars* __restrict__ scal)
+{
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    const int M = scal->nmassive;
+    if (k >= M - 1) return;
+    if (scal->status & NBHIP_CELL_OVERFLOW_ERROR) return;
+    EmitView v = { blo, bdelta, bparent, dadj, cincl };
+    const int Lk = nbh_levels(bdelta[k]);
+    const int Lp = nbh_parent_levels(v, k);
+    if (Lk <= Lp) return;
+    const int lo = blo[k], hi = bhi[k];
+    const int base = lo + nbh_excl(v, lo) - nbh_first_level(v, lo);
+    const int next = hi + 1 + (int) cincl[hi];
+    const double R = scal->rsize;
+    const unsigned long long ux = su[lo], uy = su[n + lo], uz = su[2 * n + lo];
+    const int nn = scal->nnode;
+    int top_parent = -1;
+    if (k != 0) {
+        top_parent = nbh_owner_pre(v, bparent[k], M);
+        if (top_parent < 0) {
+            atomicOr(&scal->status, NBHIP_ERROR);
+            return;
+        }
+    }
+    for (int l = Lp + 1; l <= Lk; ++l) {
+        const int pre = base + l;
+        if (pre < 0 || pre >= nn) {
+            atomicOr(&scal->status, NBHIP_ERROR);
+            return;
+        }
+        NodeW c;
+        c.x = c.y = c.z = c.m = 0.0;
+        c.rcrit2 = 0.0;
+        c.next = next;
+        c.body = -1;
+        node[pre] = c;
+        nlevel[pre] = l;
+        nparent[pre] = (l == Lp + 1) ? top_parent : pre - 1;
+        nchild[pre] = 0;
+        narrive[pre] = 0;
+        double mx = 0.0, my = 0.0, mz = 0.0;
+        if (l > 0) {
+            const double h = ldexp(R, -(l + 1));
+            const int sh = 63 - l;
+            mx = (double) (2ull * (ux >> sh) + 1ull) * h - 0.5 * R;
+            my = (double) (2ull * (uy >> sh) + 1ull) * h - 0.5 * R;
+            mz = (double) (2ull * (uz >> sh) + 1ull) * h - 0.5 * R;
+        }
+        nmid[3 * pre + 0] = mx;
+        nmid[3 * pre + 1] = my;
+        nmid[3 * pre + 2] = mz;
+    }
+    atomicMax(&scal->maxdepth, Lk);
+}
+
+__global__ __launch_bounds__(256) void k_emit_leaves(const int* __restrict__ blo,
+                                                     const int* __restrict__ bdelta,
+                                                     const int* __restrict__ bparent,
+                                                     const int* __restrict__ dadj,
+                                                     const unsigned* __restrict__ cincl,
+                                                     const double4* __restrict__ spos,
+                                                     NodeW* __restrict__ node,
+                                                     int* __restrict__ nlevel,
+                                                     int* __restrict__ nparent,
+                                                     DevScalars* __restrict__ scal)
+{
+    const int s = blockIdx.x * 256 + threadIdx.x;
+    const int M = scal->nmassive;
+    if (s >= M) return;
+    if (scal->status & NBHIP_CELL_OVERFLOW_ERROR) return;
+    EmitView v = { blo, bdelta, bparent, dadj, cincl };
+    const int pre = s + (int) cincl[s];
+    const int parent = (M == 1) ? 0 : nbh_owner_pre(v, bparent[(M - 1) + s], M);
+    if (pre < 0 || pre >= scal->nnode || parent < 0 || parent >= pre) {
+        atomicOr(&scal->status, NBHIP_ERROR);
+        return;
+    }
+    const double4 p = spos[s];
+    NodeW b;
+    b.x = p.x; b.y = p.y; b.z = p.z; b.m = p.w;
+    b.rcrit2 = 0.0;
+    b.next = pre + 1;
+    b.body = s;
+    node[pre] = b;
+    nlevel[pre] = -1;
+    nparent[pre] = parent;
+}
+
+__global__ __launch_bounds__(256) void k_count_children(const int* __restrict__ nparent,
+                                                        unsigned* __restrict__ nchild,
+                                                        DevScalars* __restrict__ scal)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (scal->status & NBHIP_CELL_OVERFLOW_ERROR) return;
+    if (i >= scal->nnode) return;
+    const int p = nparent[i];
+    if (p >= i) {
+        atomicOr(&scal->status, NBHIP_ERROR);
+        return;
+    }
+    if (p >= 0) atomicAdd(&nchild[p], 1u);
+}
+
+/* ---------------------------------------------------------------------- */
+/* bottom-up summarisation: the last child to arrive finishes its parent   */
+/* ---------------------------------------------------------------------- */
+/* hackCofM + findRCrit + nbCheckTreeStructure (nbody/src/nbody_tree.c:
+ * 345-424) and hackQuad (:70-121), children in octant (pre-)order. */
+__global__ __launch_bounds__(256) void k_summarize(const unsigned* __restrict__ cincl,
+                                                   NodeW* node, double* nquad,
+                                                   const double* __restrict__ nmid,
+                                                   const int* __restrict__ nlevel,
+                                                   const int* __restrict__ nparent,
+                                                   const unsigned* __restrict__ nchild,
+                                                   unsigned* narrive, nbhip_ctx ctx,
+                                                   DevScalars* scal)
+{
+    const int s = blockIdx.x * 256 + threadIdx.x;
+    const int M = scal->nmassive;
+    if (s >= M) return;
+    if (scal->status & NBHIP_CELL_OVERFLOW_ERROR) return;
+    if (scal->status & NBHIP_ERROR) return;
+    const double rsize = scal->rsize;
+    const int nn = scal->nnode;
+    int self = s + (int) cincl[s];
+    if (self >= nn) return;
+    int p = nparent[self];
+    while (p >= 0) {
+        if (p >= self) {
+            atomicOr(&scal->status, NBHIP_ERROR);
+            return;
+        }
+        self = p;
+        __threadfence();
+        const unsigned old = atomicAdd(&narrive[p], 1u);
+        if (old + 1u != nchild[p]) return;
+        __threadfence();
+
+        const int end = node[p].next;
+        if (end <= p || end > nn) {
+            atomicOr(&scal->status, NBHIP_ERROR);
+            return;
+        }
+        double mass = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
+        for (int c = p + 1; c < end;) {
+            const NodeW ch = node[c];
+            mass += ch.m;
+            cx += ch.m * ch.x;
+            cy += ch.m * ch.y;
+            cz += ch.m * ch.z;
+            if (ch.next <= c || ch.next > end) {
+                atomicOr(&scal->status, NBHIP_ERROR);
+                return;
+            }
+            c = ch.next;
+        }
+        const double mx = nmid[3 * p + 0], my = nmid[3 * p + 1], mz = nmid[3 * p + 2];
+        if (mass > 0.0) {
+            cx /= mass;
+            cy /= mass;
+            cz /= mass;
+        } else {
+            cx = mx; cy = my; cz = mz;
+        }
+        const double psize = ldexp(rsize, -nlevel[p]);
+        const double half = 0.5 * psize;
+        if (cx < mx - half || cx > mx + half || cy < my - half || cy > my + half ||
+            cz < mz - half || cz > mz + half)
+            atomicOr(&scal->status, NBHIP_TREE_STRUCTURE_ERROR);
+        const double rc2 = nbh_rcrit2(ctx, mx, my, mz, rsize, cx, cy, cz, psize);
+
+        if (ctx.use_quad) {
+            double q0 = 0.0, q1 = 0.0, q2 = 0.0, q3 = 0.0, q4 = 0.0, q5 = 0.0;
+            for (int c = p + 1; c < end;) {
+                const NodeW ch = node[c];
+                const double dx = ch.x - cx, dy = ch.y - cy, dz = ch.z - cz;
+                const double drsq = dz * dz + (dy * dy + dx * dx);
+                const double m = ch.m;
+                double e0 = m * (3.0 * (dx * dx) - drsq);
+                double e1 = m * (3.0 * (dx * dy));
+                double e2 = m * (3.0 * (dx * dz));
+                double e3 = m * (3.0 * (dy * dy) - drsq);
+                double e4 = m * (3.0 * (dy * dz));
+                double e5 = m * (3.0 * (dz * dz) - drsq);
+                if (ch.body < 0) {
+                    const double* cq = &nquad[6 * c];
+                    e0 += cq[0]; e1 += cq[1]; e2 += cq[2];
+                    e3 += cq[3]; e4 += cq[4]; e5 += cq[5];
+                }
+                q0 += e0; q1 += e1; q2 += e2; q3 += e3; q4 += e4; q5 += e5;
+                c = ch.next;
+            }
+            double* pq = &nquad[6 * p];
+            pq[0] = q0; pq[1] = q1; pq[2] = q2; pq[3] = q3; pq[4] = q4; pq[5] = q5;
+        }
+        node[p].x = cx;
+        node[p].y = cy;
+        node[p].z = cz;
+        node[p].m = mass;
+        node[p].rcrit2 = rc2;
+        p = nparent[p];
+    }
+}
+
+/* Structure-error rule of nbLoadBody (nbody/src/nbody_tree.c:256-264):
+ * descending through a cell of side <= DBL_EPSILON with an occupied slot,
+ * i.e. a cell exists at level D with rsize * 2^-(D-1) <= DBL_EPSILON. */
+__global__ void k_depth_check(DevScalars* scal)
+{
+    const int D = scal->maxdepth;
+    if (D >= 1 && ldexp(scal->rsize, -(D - 1)) <= DBL_EPSILON)
+        scal->status |= NBHIP_TREE_STRUCTURE_ERROR;
+    else if (D >= 63)
+        scal->status |= NBHIP_MAX_DEPTH_ERROR;
+}
+
+/* ---------------------------------------------------------------------- */
+/* host launchers                                                          */
+/* ---------------------------------------------------------------------- */
+static inline int nbh_blocks(int n, int per) { return n > 0 ? (n + per - 1) / per : 1; }
+
+void nbh_bbox(nbhip_state* st, int drift)
+{
+    const double dt = st->ctx.timestep;
+    hipLaunchKernelGGL(k_drift_bbox, dim3(nbh_blocks(st->n, 256)), dim3(256), 0, st->stream,
+                       st->n, drift, dt, 0.5 * dt, st->px, st->py, st->pz, st->vx, st->vy, st->vz,
+                       st->ax, st->ay, st->az, st->scal);
+    hipLaunchKernelGGL(k_rsize, dim3(1), dim3(1), 0, st->stream, st->scal);
+}
+
+static void nbh_mark(nbhip_state* st, int k)
+{
+    if (st->profiling) (void) hipEventRecord(st->ev[k], st->stream);
+}
+
+void nbh_build(nbhip_state* st)
+{
+    const int n = st->n;
+    const int g256 = nbh_blocks(n, 256);
+    hipStream_t s = st->stream;
+
+    hipLaunchKernelGGL(k_keys, dim3(g256), dim3(256), 0, s, n, st->px, st->py, st->pz, st->m,
+                       st->key, st->perm, st->scal);
+    unsigned long long* kin = st->key;
+    unsigned long long* kout = st->key_alt;
+    unsigned* vin = st->perm;
+    unsigned* vout = st->perm_alt;
+    const int nb = st->sort_blocks;
+    for (int pass = 0; pass < 8; ++pass) {
+        const int shift = 8 * pass;
+        hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(256), 0, s, n, shift, kin, st->hist, nb);
+        hipLaunchKernelGGL(k_radix_scan, dim3(1), dim3(1024), 0, s, st->hist, nb);
+        hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(256), 0, s, n, shift, kin, vin, kout,
+                           vout, st->hist, nb);
+        unsigned long long* tk = kin; kin = kout; kout = tk;
+        unsigned* tv = vin; vin = vout; vout = tv;
+    }
+    /* 8 passes: sorted data is back in st->key / st->perm */
+    hipLaunchKernelGGL(k_tiefix, dim3(g256), dim3(256), 0, s, st->key, st->perm, st->px, st->py,
+                       st->pz, st->scal);
+    nbh_mark(st, 2);
+
+    hipLaunchKernelGGL(k_gather, dim3(g256), dim3(256), 0, s, n, st->perm, st->px, st->py, st->pz,
+                       st->m, st->spos, st->su, st->scal);
+    hipLaunchKernelGGL(k_delta_count, dim3(g256), dim3(256), 0, s, n, st->su, st->dadj, st->cincl,
+                       st->scal);
+    const int nscan = nbh_blocks(n, NBH_SCAN_TILE);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(nscan), dim3(1024), 0, s, st->cincl, st->scan_part,
+                       st->scal);
+    hipLaunchKernelGGL(k_scan_part, dim3(1), dim3(1024), 0, s, st->scan_part, st->scal);
+    hipLaunchKernelGGL(k_scan_add, dim3(nscan), dim3(1024), 0, s, st->cincl, st->scan_part,
+                       st->scal);
+    hipLaunchKernelGGL(k_node_counts, dim3(1), dim3(1), 0, s, st->cincl, st->node, st->nquad,
+                       st->nmid, st->nlevel, st->nparent, st->nchild, st->narrive, st->node_cap,
+                       st->ctx, st->scal);
+    hipLaunchKernelGGL(k_karras, dim3(g256), dim3(256), 0, s, n, st->su, st->blo, st->bhi,
+                       st->bdelta, st->bparent, st->scal);
+    hipLaunchKernelGGL(k_emit_cells, dim3(g256), dim3(256), 0, s, st->blo, st->bhi, st->bdelta,
+                       st->bparent, st->dadj, st->cincl, st->su, n, st->node, st->nmid,
+                       st->nlevel, st->nparent, st->nchild, st->narrive, st->scal);
+    hipLaunchKernelGGL(k_emit_leaves, dim3(g256), dim3(256), 0, s, st->blo, st->bdelta,
+                       st->bparent, st->dadj, st->cincl, st->spos, st->node, st->nlevel,
+                       st->nparent, st->scal);
+    hipLaunchKernelGGL(k_count_children, dim3(nbh_blocks(st->node_cap, 256)), dim3(256), 0, s,
+                       st->nparent, st->nchild, st->scal);
+    nbh_mark(st, 3);
+    hipLaunchKernelGGL(k_summarize, dim3(g256), dim3(256), 0, s, st->cincl, st->node, st->nquad,
+                       st->nmid, st->nlevel, st->nparent, st->nchild, st->narrive, st->ctx,
+                       st->scal);
+    hipLaunchKernelGGL(k_depth_check, dim3(1), dim3(1), 0, s, st->scal);
+    nbh_mark(st, 4);
+}

@@ -1,0 +1,415 @@
+"""Host-side mirror of the reference's nbody run interface over the nbhip C ABI.
+
+The classes follow the Lua-facing API that the reference's tests drive
+(``NBodyCtx.create``, ``Potential.create``, ``NBodyState.create``,
+``st:step(ctx)``; nbody/src/nbody_lua_types/*.c), with the force step and the
+integrator running in ``libnbhip.so`` on the GPU.  Nothing here computes
+physics on the CPU: if the library is missing the calls fail.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+
+_PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG_DIR, "libnbhip.so")
+
+# NBodyStatus bits (nbody/include/nbody_types.h:541-577)
+NBODY_SUCCESS = 0
+NBODY_ERROR = 1 << 1
+NBODY_TREE_STRUCTURE_ERROR = 1 << 3
+NBODY_TREE_INCEST_FATAL = 1 << 4
+NBODY_GPU_ERROR = 1 << 7
+NBODY_CAPABILITY_ERROR = 1 << 8
+NBODY_UNSUPPORTED = 1 << 11
+NBODY_USER_ERROR = 1 << 12
+NBODY_MAX_DEPTH_ERROR = 1 << 16
+NBODY_CELL_OVERFLOW_ERROR = 1 << 17
+NBODY_TREE_INCEST_NONFATAL = 1 << 24
+NBODY_STATUS_ALL_WARNINGS = 0xFF000000
+
+CRITERIA = {"treecode": 0, "sw93": 1, "bh86": 2, "exact": 3}
+PRECISION = {"fp64": 0, "mixed": 1}
+
+
+def status_is_fatal(rc: int) -> bool:
+    return (rc & ~NBODY_STATUS_ALL_WARNINGS) != 0
+
+
+class NBodyError(RuntimeError):
+    def __init__(self, rc: int, msg: str):
+        super().__init__(f"nbhip status 0x{rc:x}: {msg}")
+        self.status = rc
+
+
+class _Potential(C.Structure):
+    _fields_ = [
+        ("sphere_type", C.c_int), ("sphere_mass", C.c_double), ("sphere_scale", C.c_double),
+        ("disk_type", C.c_int), ("disk_mass", C.c_double),
+        ("disk_scale_length", C.c_double), ("disk_scale_height", C.c_double),
+        ("disk2_type", C.c_int), ("disk2_mass", C.c_double),
+        ("disk2_scale_length", C.c_double), ("disk2_scale_height", C.c_double),
+        ("halo_type", C.c_int), ("halo_vhalo", C.c_double),
+        ("halo_scale_length", C.c_double), ("halo_flatten_z", C.c_double),
+    ]
+
+
+class _Ctx(C.Structure):
+    _fields_ = [
+        ("eps2", C.c_double), ("theta", C.c_double), ("timestep", C.c_double),
+        ("tree_rsize", C.c_double), ("criterion", C.c_int), ("use_quad", C.c_int),
+        ("allow_incest", C.c_int), ("use_potential", C.c_int), ("precision", C.c_int),
+        ("reserved", C.c_int), ("pot", _Potential),
+    ]
+
+
+class _TreeInfo(C.Structure):
+    _fields_ = [
+        ("rsize", C.c_double), ("ncell", C.c_int), ("max_depth", C.c_int),
+        ("nnode", C.c_int), ("nmassive", C.c_int), ("step", C.c_uint), ("status", C.c_uint),
+    ]
+
+
+EXPORTED = [
+    "nbhip_create", "nbhip_upload", "nbhip_download", "nbhip_gravity", "nbhip_step",
+    "nbhip_run", "nbhip_steps", "nbhip_tree_info_get", "nbhip_tree_download",
+    "nbhip_direct_accel", "nbhip_walk_accel", "nbhip_walk_counters", "nbhip_set_profiling",
+    "nbhip_phase_times", "nbhip_sync", "nbhip_release", "nbhip_last_error",
+]
+
+_LIB: Optional[C.CDLL] = None
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+def load_library() -> C.CDLL:
+    """Load the in-tree libnbhip.so; never builds, never falls back."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with `python -m milkywayathome_client_amd.build`")
+    lib = C.CDLL(LIB_PATH)
+    u = C.c_uint
+    st = C.c_void_p
+    sig = {
+        "nbhip_create": (u, [C.POINTER(_Ctx), C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
+        "nbhip_upload": (u, [st, _dp, _dp, _dp]),
+        "nbhip_download": (u, [st, _dp, _dp, _dp]),
+        "nbhip_gravity": (u, [st]),
+        "nbhip_step": (u, [st]),
+        "nbhip_run": (u, [st, C.c_uint]),
+        "nbhip_steps": (u, [st, C.c_uint]),
+        "nbhip_tree_info_get": (u, [st, C.POINTER(_TreeInfo)]),
+        "nbhip_tree_download": (u, [st, _ip, _ip, _ip, _ip, _dp, _dp, _dp, _dp]),
+        "nbhip_direct_accel": (u, [st, C.c_int, _ip, _dp]),
+        "nbhip_walk_accel": (u, [st, _dp]),
+        "nbhip_walk_counters": (u, [st, C.POINTER(C.c_ulonglong)]),
+        "nbhip_set_profiling": (u, [st, C.c_int]),
+        "nbhip_phase_times": (u, [st, _dp]),
+        "nbhip_sync": (u, [st]),
+        "nbhip_release": (None, [st]),
+        "nbhip_last_error": (C.c_char_p, []),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _LIB = lib
+    return lib
+
+
+def _ptr(a: Optional[np.ndarray], ctype=C.c_double):
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+# ---------------------------------------------------------------------------
+# Potential (nbody/src/nbody_lua_types/nbody_lua_potential.c and friends)
+# ---------------------------------------------------------------------------
+@dataclass
+class Spherical:
+    type: int = 0
+    mass: float = 0.0
+    scale: float = 0.0
+
+    @staticmethod
+    def hernquist(mass: float, scale: float) -> "Spherical":
+        return Spherical(1, mass, scale)
+
+    @staticmethod
+    def plummer(mass: float, scale: float) -> "Spherical":
+        return Spherical(2, mass, scale)
+
+    @staticmethod
+    def none(mass: float = 0.0) -> "Spherical":
+        return Spherical(0, mass, 0.0)
+
+
+@dataclass
+class Disk:
+    type: int = 0
+    mass: float = 0.0
+    scaleLength: float = 0.0
+    scaleHeight: float = 0.0
+
+    @staticmethod
+    def miyamotoNagai(mass: float, scaleLength: float, scaleHeight: float) -> "Disk":
+        return Disk(1, mass, scaleLength, scaleHeight)
+
+    @staticmethod
+    def none(mass: float = 0.0) -> "Disk":
+        return Disk(0, mass)
+
+
+@dataclass
+class Halo:
+    type: int = 0
+    vhalo: float = 0.0
+    scaleLength: float = 0.0
+    flattenZ: float = 1.0
+
+    @staticmethod
+    def logarithmic(vhalo: float, scaleLength: float, flattenZ: float) -> "Halo":
+        return Halo(1, vhalo, scaleLength, flattenZ)
+
+    @staticmethod
+    def nfw(vhalo: float, scaleLength: float) -> "Halo":
+        return Halo(2, vhalo, scaleLength, 1.0)
+
+    @staticmethod
+    def none(mass: float = 0.0) -> "Halo":
+        return Halo(0)
+
+
+@dataclass
+class Potential:
+    spherical: Spherical = field(default_factory=Spherical)
+    disk: Disk = field(default_factory=Disk)
+    disk2: Disk = field(default_factory=Disk)
+    halo: Halo = field(default_factory=Halo)
+
+    @staticmethod
+    def create(spherical=None, disk=None, disk2=None, halo=None) -> "Potential":
+        return Potential(spherical or Spherical(), disk or Disk(), disk2 or Disk(), halo or Halo())
+
+    def to_c(self) -> _Potential:
+        p = _Potential()
+        p.sphere_type, p.sphere_mass, p.sphere_scale = (
+            self.spherical.type, self.spherical.mass, self.spherical.scale)
+        p.disk_type, p.disk_mass = self.disk.type, self.disk.mass
+        p.disk_scale_length, p.disk_scale_height = self.disk.scaleLength, self.disk.scaleHeight
+        p.disk2_type, p.disk2_mass = self.disk2.type, self.disk2.mass
+        p.disk2_scale_length, p.disk2_scale_height = self.disk2.scaleLength, self.disk2.scaleHeight
+        p.halo_type, p.halo_vhalo = self.halo.type, self.halo.vhalo
+        p.halo_scale_length, p.halo_flatten_z = self.halo.scaleLength, self.halo.flattenZ
+        return p
+
+
+def milky_way_potential() -> Potential:
+    """The sample workunits' Milky Way (nbody/sample_workunits/for_developers.lua:156-161):
+    Hernquist bulge, Miyamoto-Nagai disk, logarithmic halo (vhalo 74.61 kpc/Gyr = 73 km/s)."""
+    return Potential.create(
+        spherical=Spherical.hernquist(mass=1.52954402e5, scale=0.7),
+        disk=Disk.miyamotoNagai(mass=4.45865888e5, scaleLength=6.5, scaleHeight=0.26),
+        disk2=Disk.none(mass=3.0e5),
+        halo=Halo.logarithmic(vhalo=74.61, scaleLength=12.0, flattenZ=1.0),
+    )
+
+
+# ---------------------------------------------------------------------------
+# NBodyCtx (nbody/src/nbody_lua_types/nbody_lua_nbodyctx.c:84-221)
+# ---------------------------------------------------------------------------
+@dataclass
+class NBodyCtx:
+    timestep: float
+    timeEvolve: float
+    eps2: float
+    theta: float = 0.5
+    treeRSize: float = 4.0
+    criterion: str = "bh86"
+    useQuad: bool = True
+    allowIncest: bool = False
+    nStep: int = 0
+
+    @staticmethod
+    def create(timestep: float, timeEvolve: float, eps2: float, theta: float = -1.0,
+               treeRSize: float = 4.0, criterion: str = "BH86", useQuad: bool = True,
+               allowIncest: bool = False) -> "NBodyCtx":
+        crit = criterion.lower()
+        if crit not in CRITERIA:
+            raise ValueError(f"unknown criterion {criterion!r}")
+        if crit != "exact" and theta < 0.0:
+            raise ValueError("Theta argument required for criterion != 'Exact'")
+        if crit == "exact":
+            theta, useQuad = 0.0, False
+        n_step = math.ceil(timeEvolve / timestep)
+        if n_step >= 2 ** 32 - 1:
+            raise ValueError("Number of timesteps exceeds UINT_MAX")
+        # nbCorrectTimestep (nbody/src/nbody_util.c:26-30)
+        corrected = timeEvolve / math.ceil(timeEvolve / timestep) if n_step > 0 else timestep
+        return NBodyCtx(timestep=corrected, timeEvolve=timeEvolve, eps2=eps2, theta=theta,
+                        treeRSize=treeRSize, criterion=crit, useQuad=useQuad,
+                        allowIncest=allowIncest, nStep=int(n_step))
+
+    def to_c(self, potential: Optional[Potential], precision: str = "fp64") -> _Ctx:
+        c = _Ctx()
+        c.eps2, c.theta, c.timestep, c.tree_rsize = (self.eps2, self.theta, self.timestep,
+                                                     self.treeRSize)
+        c.criterion = CRITERIA[self.criterion]
+        c.use_quad = int(bool(self.useQuad))
+        c.allow_incest = int(bool(self.allowIncest))
+        c.use_potential = 0 if potential is None else 1
+        c.precision = PRECISION[precision]
+        c.reserved = 0
+        c.pot = (potential or Potential()).to_c()
+        return c
+
+
+@dataclass
+class TreeInfo:
+    rsize: float
+    ncell: int
+    max_depth: int
+    nnode: int
+    nmassive: int
+    step: int
+    status: int
+
+
+class NBodyState:
+    """A simulation resident on one GPU (mirror of the Lua NBodyState,
+    nbody/src/nbody_lua_types/nbody_lua_nbodystate.c)."""
+
+    def __init__(self, ctx: NBodyCtx, potential: Optional[Potential], pos, vel, mass,
+                 device: int = 0, precision: str = "fp64", prestep: bool = True):
+        self.lib = load_library()
+        self.ctx = ctx
+        self.potential = potential
+        self.precision = precision
+        self.n = int(np.asarray(mass).shape[0])
+        h = C.c_void_p()
+        cctx = ctx.to_c(potential, precision)
+        rc = self.lib.nbhip_create(C.byref(cctx), device, self.n, C.byref(h))
+        if rc:
+            raise NBodyError(rc, self.lib.nbhip_last_error().decode())
+        self._h = h
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(self.n, 3)
+        vel = np.ascontiguousarray(vel, dtype=np.float64).reshape(self.n, 3)
+        mass = np.ascontiguousarray(mass, dtype=np.float64).reshape(self.n)
+        self._check(self.lib.nbhip_upload(h, _ptr(pos), _ptr(vel), _ptr(mass)))
+        self.last_status = 0
+        if prestep:
+            # NBodyState.create runs one force evaluation (nbody_lua_nbodystate.c:123-147)
+            self.last_status = self._check(self.lib.nbhip_gravity(h), allow_warning=True)
+
+    @staticmethod
+    def create(ctx: NBodyCtx, potential: Optional[Potential], bodies, **kw) -> "NBodyState":
+        pos, vel, mass = bodies
+        return NBodyState(ctx, potential, pos, vel, mass, **kw)
+
+    def _check(self, rc: int, allow_warning: bool = True) -> int:
+        if rc and (status_is_fatal(rc) or not allow_warning):
+            raise NBodyError(rc, self.lib.nbhip_last_error().decode())
+        return rc
+
+    def gravity(self) -> int:
+        self.last_status = self._check(self.lib.nbhip_gravity(self._h))
+        return self.last_status
+
+    def step(self) -> int:
+        self.last_status = self._check(self.lib.nbhip_step(self._h))
+        return self.last_status
+
+    def steps(self, nsteps: int) -> int:
+        self.last_status = self._check(self.lib.nbhip_steps(self._h, nsteps))
+        return self.last_status
+
+    def run(self, nsteps: Optional[int] = None) -> int:
+        """nbRunSystem: one force evaluation then nStep KDK steps."""
+        k = self.ctx.nStep if nsteps is None else nsteps
+        self.last_status = self._check(self.lib.nbhip_run(self._h, k))
+        return self.last_status
+
+    def bodies(self):
+        pos = np.empty((self.n, 3))
+        vel = np.empty((self.n, 3))
+        acc = np.empty((self.n, 3))
+        self._check(self.lib.nbhip_download(self._h, _ptr(pos), _ptr(vel), _ptr(acc)))
+        return pos, vel, acc
+
+    def accelerations(self) -> np.ndarray:
+        return self.bodies()[2]
+
+    def tree_info(self) -> TreeInfo:
+        ti = _TreeInfo()
+        self._check(self.lib.nbhip_tree_info_get(self._h, C.byref(ti)))
+        return TreeInfo(ti.rsize, ti.ncell, ti.max_depth, ti.nnode, ti.nmassive, ti.step,
+                        ti.status)
+
+    def tree(self) -> dict:
+        nn = self.tree_info().nnode
+        out = {
+            "kind": np.empty(nn, np.int32), "ref": np.empty(nn, np.int32),
+            "skip": np.empty(nn, np.int32), "level": np.empty(nn, np.int32),
+            "pos": np.empty((nn, 3)), "mass": np.empty(nn), "rcrit2": np.empty(nn),
+            "quad": np.empty((nn, 6)),
+        }
+        self._check(self.lib.nbhip_tree_download(
+            self._h, _ptr(out["kind"], C.c_int), _ptr(out["ref"], C.c_int),
+            _ptr(out["skip"], C.c_int), _ptr(out["level"], C.c_int), _ptr(out["pos"]),
+            _ptr(out["mass"]), _ptr(out["rcrit2"]), _ptr(out["quad"])))
+        return out
+
+    def direct_accel(self, idx) -> np.ndarray:
+        idx = np.ascontiguousarray(idx, dtype=np.int32)
+        acc = np.empty((idx.shape[0], 3))
+        self._check(self.lib.nbhip_direct_accel(self._h, idx.shape[0], _ptr(idx, C.c_int),
+                                                _ptr(acc)))
+        return acc
+
+    def walk_accel(self) -> np.ndarray:
+        acc = np.empty((self.n, 3))
+        self._check(self.lib.nbhip_walk_accel(self._h, _ptr(acc)))
+        return acc
+
+    def walk_counters(self):
+        c = (C.c_ulonglong * 3)()
+        self._check(self.lib.nbhip_walk_counters(self._h, c))
+        return int(c[0]), int(c[1]), int(c[2])
+
+    def set_profiling(self, enable: bool = True) -> None:
+        self._check(self.lib.nbhip_set_profiling(self._h, int(enable)))
+
+    def phase_times(self) -> np.ndarray:
+        ms = np.zeros(5)
+        self._check(self.lib.nbhip_phase_times(self._h, _ptr(ms)))
+        return ms
+
+    def sync(self) -> None:
+        self._check(self.lib.nbhip_sync(self._h))
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self.lib.nbhip_release(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
