@@ -1,0 +1,218 @@
+"""Benchmark: body-steps/s of the Barnes-Hut KDK step on one MI355X.
+
+Workload (default): two-component dwarf, N = 1,000,000 (500k Plummer baryons
+M=12, a=0.2 kpc + 500k NFW dark matter M200=60, a=0.8 kpc), seed 34086709,
+placed on its orbit by reversing 4 Gyr from (l,b,r) = (218, 53.5, 28.6) with
+velocity (-156, 79, 107) in the Milky Way potential (Miyamoto-Nagai disk,
+Hernquist bulge, logarithmic halo); BH86 theta = 0.5 with quadrupoles,
+dt = 1e-3 Gyr, eps2 from calculateEps2.  Inputs are resident in HBM before
+the timed region; one step = drift + tree build + walk + kicks for all bodies.
+
+Prints one JSON line (see the --help of each field in DESIGN.md).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "body-steps/s at N=1M θ=0.5: mixed vs FP64 GPU at ≤1e-5 rel accel err"
+PEAK = {"fp64": 78.6, "fp32": 157.3}     # TFLOP/s, MI355X published
+HBM_PEAK = 8000.0                        # GB/s
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_workload(n: int, seed: int):
+    from milkywayathome_client_amd import dwarf, models, nbody as nb
+    pot = nb.milky_way_potential()
+    x0 = models.lbr_to_cartesian([218.0, 53.5, 28.6])
+    dt = 1e-3
+    fpos, fvel = models.reverse_orbit(pot, x0, [-156.0, 79.0, 107.0], 4.0, dt / 10.0)
+    pos, vel, m, typ = dwarf.two_component_dwarf(n, seed=seed, position=fpos, velocity=fvel)
+    eps2 = models.calculate_eps2(n, 0.2, 0.8, 12.0, 60.0)
+    return pot, pos, vel, m, eps2, dt
+
+
+def walk_flops(counts, use_quad=True):
+    """Algorithmic flops of one walk from the reference expressions
+    (nbody/src/nbody_grav.c:53-99): 18 per interaction, +38 per accepted
+    cell with quadrupoles, 9 per opened-cell test."""
+    nbb, nbc, nop = counts[:3]
+    return 18.0 * nbb + (56.0 if use_quad else 18.0) * nbc + 9.0 * nop
+
+
+def run_gpu(precision, pot, pos, vel, m, eps2, dt, steps, warmup, theta):
+    from milkywayathome_client_amd import nbody as nb
+    ctx = nb.NBodyCtx.create(timestep=dt, timeEvolve=dt * (steps + warmup), eps2=eps2,
+                             theta=theta, criterion="BH86", useQuad=True, treeRSize=4.0)
+    st = nb.NBodyState(ctx, pot, pos, vel, m, precision=precision)
+    st.sync()
+    if warmup:
+        st.steps(warmup)
+    st.sync()
+    st.set_profiling(True)
+    phases = np.zeros(5)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st.step()
+        phases += st.phase_times()
+    st.sync()
+    wall = time.perf_counter() - t0
+    st.set_profiling(False)
+    counts = st.walk_counters()
+    return st, wall, phases / steps, counts
+
+
+def accel_errors(st, pot, n, nsample=4096, seed=1):
+    """Median / 99th-percentile relative error of total and self-gravity
+    accelerations against the FP64 direct sum on sampled bodies."""
+    from milkywayathome_client_amd import models
+    rng = np.random.default_rng(seed)
+    idx = np.sort(rng.choice(n, size=min(nsample, n), replace=False)).astype(np.int32)
+    pos, _, acc = st.bodies()
+    direct = st.direct_accel(idx)
+    ext = models.ext_accel(pot, pos[idx])
+    tot_ref = direct + ext
+    self_tree = acc[idx] - ext
+    e_tot = np.linalg.norm(acc[idx] - tot_ref, axis=1) / np.linalg.norm(tot_ref, axis=1)
+    e_self = np.linalg.norm(self_tree - direct, axis=1) / np.linalg.norm(direct, axis=1)
+    return {"total_median": float(np.median(e_tot)), "total_p99": float(np.percentile(e_tot, 99)),
+            "self_median": float(np.median(e_self)), "self_p99": float(np.percentile(e_self, 99)),
+            "nsample": int(idx.size)}
+
+
+def cpu_baseline(pot_ctx_args, pos, vel, m, threads, max_bodies):
+    """The oracle (the C restatement of the reference CPU path) timed for one
+    full step on the host cores: a bounded sample of the same workload."""
+    from oracle import oracle as orc
+    pot, eps2, dt, theta = pot_ctx_args
+    opot = orc.make_potential(
+        sphere=(pot.spherical.type, pot.spherical.mass, pot.spherical.scale),
+        disk=(pot.disk.type, pot.disk.mass, pot.disk.scaleLength, pot.disk.scaleHeight),
+        disk2=(pot.disk2.type, pot.disk2.mass, pot.disk2.scaleLength, pot.disk2.scaleHeight),
+        halo=(pot.halo.type, pot.halo.vhalo, pot.halo.scaleLength, pot.halo.flattenZ))
+    ctx = orc.make_ctx(eps2=eps2, theta=theta, timestep=dt, criterion="bh86", use_quad=True,
+                       potential=opot)
+    n = pos.shape[0]
+    o = orc.Oracle(ctx, pos, vel, m, threads=threads)
+    t0 = time.perf_counter()
+    o.gravity()
+    t_pre = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    o.step()
+    t_step = time.perf_counter() - t0
+    return {"value": n / t_step, "unit": "body-steps/s", "cores": o.lib.orc_get_threads(),
+            "kind": "port",
+            "sample": f"1 full KDK step (tree build + walk + potential) of the same N={n} "
+                      f"workload after one force evaluation; {t_step:.2f} s step, "
+                      f"{t_pre:.2f} s pre-step force"}
+
+
+def load_traffic():
+    path = os.path.join(ROOT, "profiles", "walk_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--seed", type=int, default=34086709)
+    ap.add_argument("--theta", type=float, default=0.5)
+    ap.add_argument("--precision", choices=["both", "fp64", "mixed"], default="both")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
+                                   "(multi-GPU sharding is design-only)"}))
+        sys.exit(2)
+
+    log(f"[bench] building workload N={args.n}")
+    pot, pos, vel, m, eps2, dt = make_workload(args.n, args.seed)
+    log(f"[bench] eps2={eps2:.6g} dt={dt} centre={pos.mean(0)}")
+
+    from milkywayathome_client_amd import nbody as nb
+    results = {}
+    precs = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
+    for prec in precs:
+        log(f"[bench] {prec}: warmup {args.warmup}, timed {args.steps}")
+        try:
+            st, wall, phases, counts = run_gpu(prec, pot, pos, vel, m, eps2, dt, args.steps,
+                                               args.warmup, args.theta)
+        except nb.NBodyError as e:
+            log(f"[bench] {prec} failed: {e}")
+            continue
+        errs = accel_errors(st, pot, args.n)
+        results[prec] = {"wall": wall, "phases_ms": phases, "counts": counts, "errors": errs,
+                         "body_steps_per_s": args.n * args.steps / wall}
+        log(f"[bench] {prec}: {results[prec]['body_steps_per_s']:.4g} body-steps/s, "
+            f"ms/step {1e3 * wall / args.steps:.3f}, phases {np.round(phases, 3)}, errs {errs}")
+        st.close()
+
+    if not results:
+        print(json.dumps({"metric": METRIC, "error": "no GPU result (see stderr)"}))
+        sys.exit(1)
+    head = "mixed" if "mixed" in results else "fp64"
+    r = results[head]
+    walk_ms = float(r["phases_ms"][4])
+    flops = walk_flops(r["counts"])
+    achieved = flops / (walk_ms * 1e-3) / 1e12
+    peak = PEAK["fp32" if head == "mixed" else "fp64"]
+    traffic = load_traffic()
+    line = {
+        "metric": METRIC,
+        "value": r["body_steps_per_s"],
+        "unit": "body-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * r["wall"] / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 walk + f64 state" if head == "mixed" else "f64",
+        "data": "synthetic (two-component dwarf ICs sampled in-process)",
+        "config": {"workload": "two-component dwarf N=1M (500k Plummer 12/0.2 + 500k NFW 60/0.8) "
+                               "on its orbit in the MW potential, BH86 theta=0.5 + quadrupoles",
+                   "n": args.n, "theta": args.theta, "dt": dt, "eps2": eps2, "seed": args.seed,
+                   "precision": head},
+        "roofline": {"bound": "valu", "kernel": "k_walk (tree walk + potential + kick)",
+                     "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                     "frac": achieved / peak,
+                     "traffic": None if traffic is None else traffic.get("bytes_per_launch"),
+                     "flops_per_launch": flops, "walk_ms": walk_ms,
+                     "lane_utilisation": sum(r["counts"][:3]) / (64.0 * max(r["counts"][3], 1))},
+        "phases_ms": {k: float(v) for k, v in zip(
+            ["drift_bbox", "keys_sort", "build", "summarize", "walk"], r["phases_ms"])},
+        "accel_error_vs_fp64_direct": r["errors"],
+    }
+    if "fp64" in results:
+        line["fp64_body_steps_per_s"] = results["fp64"]["body_steps_per_s"]
+    if "mixed" in results and "fp64" in results:
+        line["mixed_over_fp64"] = (results["mixed"]["body_steps_per_s"]
+                                   / results["fp64"]["body_steps_per_s"])
+    if not args.no_cpu:
+        log(f"[bench] CPU baseline with {args.cpu_threads} threads")
+        line["cpu_baseline"] = cpu_baseline((pot, eps2, dt, args.theta), pos, vel, m,
+                                            args.cpu_threads, args.n)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -147,9 +147,11 @@ unsigned nbhip_direct_accel(nbhip_state* st, int nsample, const int* idx, double
  * current tree for all bodies, in the precision of the context. */
 unsigned nbhip_walk_accel(nbhip_state* st, double* acc);
 
-/* Walk counters of the last force evaluation: body-body interactions,
- * body-cell interactions, opened cells (flop accounting). */
-unsigned nbhip_walk_counters(nbhip_state* st, unsigned long long* counts3);
+/* Walk counters of one extra counting walk over the current tree:
+ * [0] body-body interactions, [1] body-cell interactions, [2] opened cells
+ * (flop accounting), [3] wavefront node steps (64 x [3] lane slots were
+ * issued; ([0]+[1]+[2]) / (64 x [3]) is the walk's lane utilisation). */
+unsigned nbhip_walk_counters(nbhip_state* st, unsigned long long* counts4);
 
 /* Per-phase device time of the last step in milliseconds, measured with HIP
  * events on the library's stream: [0] drift+bbox, [1] keys+sort, [2] build,

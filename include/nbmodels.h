@@ -39,6 +39,29 @@ int nbm_plummer(nbm_dsfmt* prng, int nbody, double mass, double radius_scale,
                 const double* rshift, const double* vshift, int ignore,
                 double* pos, double* vel, double* body_mass, int* type);
 
+/* External potential for host-side orbit integration: the same components
+ * and operation order as the device nbh_ext_accel (nbExtAcceleration,
+ * nbody/src/nbody_potential.c:627-762).  Layout of pot[15]:
+ * sphere_type, sphere_mass, sphere_scale, disk_type, disk_mass, disk_a,
+ * disk_b, disk2_type, disk2_mass, disk2_a, disk2_b, halo_type, halo_vhalo,
+ * halo_d, halo_q (types as doubles). */
+void nbm_ext_accel(const double* pot15, const double* pos, double* acc);
+
+/* lbrToCartesian (nbody/src/nbody_coordinates.c:51-71), l and b in degrees. */
+void nbm_lbr_to_cartesian(const double* lbr, double sun_gc_dist, double* xyz);
+
+/* nbReverseOrbit (nbody/src/nbody_orbit_integrator.c:44-86): test-particle
+ * KDK backwards for tstop with step dt; returns the reversed-back state. */
+void nbm_reverse_orbit(const double* pot15, const double* pos, const double* vel,
+                       double tstop, double dt, double* final_pos, double* final_vel);
+
+/* nbCalculateVirial (nbody/src/nbody_virial.c:38-102) and
+ * nbCalculateEps2 (nbody/src/nbody_lua_models.c:158-169). */
+double nbm_virial_radius(double a_b, double a_d, double m_b, double m_d);
+double nbm_calculate_eps2(double nbody, double a_b, double a_d, double m_b, double m_d);
+/* nbCalculateTimestep (nbody/src/nbody_lua_models.c:139-142) */
+double nbm_calculate_timestep(double mass, double r0);
+
 #ifdef __cplusplus
 }
 #endif

@@ -81,8 +81,8 @@ static void nbh_free_all(nbhip_state* st)
                      st->m,    st->ax,     st->ay,    st->az,      st->key,    st->key_alt,
                      st->perm, st->perm_alt, st->hist, st->spos,   st->su,     st->dadj,
                      st->cincl, st->scan_part, st->blo, st->bhi,   st->bdelta, st->bparent,
-                     st->node, st->nquad,  st->nmid,  st->nlevel,  st->nparent, st->nchild,
-                     st->narrive, st->scal };
+                     st->node, st->nquad,  st->nmid,  st->nlevel,  st->nparent,
+                     st->level_list, st->nodef, st->scal };
     for (void* p : ptrs)
         if (p) (void) hipFree(p);
 }
@@ -125,7 +125,8 @@ extern "C" unsigned nbhip_create(const nbhip_ctx* ctx, int device, int nbody, nb
     A(scan_part, 4096);
     A(blo, n); A(bhi, n); A(bdelta, n); A(bparent, 2 * n);
     A(node, nc); A(nquad, 6 * nc); A(nmid, 3 * nc); A(nlevel, nc); A(nparent, nc);
-    A(nchild, nc); A(narrive, nc);
+    A(level_list, nc);
+    if (ctx->precision == NBHIP_PRECISION_MIXED) A(nodef, nc);
     A(scal, 1);
 #undef A
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&st->stream, hipStreamNonBlocking);
@@ -448,9 +449,9 @@ extern "C" unsigned nbhip_walk_accel(nbhip_state* st, double* acc)
     return NBHIP_SUCCESS;
 }
 
-extern "C" unsigned nbhip_walk_counters(nbhip_state* st, unsigned long long* counts3)
+extern "C" unsigned nbhip_walk_counters(nbhip_state* st, unsigned long long* counts4)
 {
-    if (!st || !counts3) return NBHIP_USER_ERROR;
+    if (!st || !counts4) return NBHIP_USER_ERROR;
     if (!st->have_tree) {
         nbh_set_error("no tree: call nbhip_gravity first");
         return NBHIP_USER_ERROR;
@@ -459,7 +460,7 @@ extern "C" unsigned nbhip_walk_counters(nbhip_state* st, unsigned long long* cou
     const size_t n = (size_t) st->n;
     double* t = nullptr;
     NBH_CHECK(hipMalloc(&t, 3 * (n ? n : 1) * sizeof(double)));
-    unsigned long long zero[3] = { 0, 0, 0 };
+    unsigned long long zero[4] = { 0, 0, 0, 0 };
     hipError_t e = hipMemcpyAsync(&st->scal->counters[0], zero, sizeof(zero),
                                   hipMemcpyHostToDevice, st->stream);
     if (e == hipSuccess) {
@@ -467,7 +468,7 @@ extern "C" unsigned nbhip_walk_counters(nbhip_state* st, unsigned long long* cou
         e = hipGetLastError();
     }
     if (e == hipSuccess)
-        e = hipMemcpyAsync(counts3, &st->scal->counters[0], 3 * sizeof(unsigned long long),
+        e = hipMemcpyAsync(counts4, &st->scal->counters[0], 4 * sizeof(unsigned long long),
                            hipMemcpyDeviceToHost, st->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(st->stream);
     (void) hipFree(t);

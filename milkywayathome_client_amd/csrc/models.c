@@ -178,3 +178,162 @@ int nbm_plummer(nbm_dsfmt* prng, int nbody, double mass, double radius_scale,
     }
     return 0;
 }
+
+/* ---------------------------------------------------------------------- */
+/* host external potential (same order as the device nbh_ext_accel)       */
+/* ---------------------------------------------------------------------- */
+static double nbm_pow1p5(double x)
+{
+    double s = sqrt(x);
+    double e = fma(-s, s, x);
+    double ds = e / (2.0 * s);
+    double p = x * s;
+    double pe = fma(x, s, -p);
+    return p + (pe + x * ds);
+}
+
+static double nbm_cube(double t)
+{
+    double h = t * t;
+    double l = fma(t, t, -h);
+    double p = h * t;
+    double pe = fma(h, t, -p);
+    return p + (pe + l * t);
+}
+
+static void nbm_mn(double M, double da, double db, double x, double y, double z, double* t)
+{
+    double zp = sqrt(z * z + db * db);
+    double azp = da + zp;
+    double rp = x * x + y * y + azp * azp;
+    double rth = nbm_pow1p5(rp);
+    t[0] = -M * x / rth;
+    t[1] = -M * y / rth;
+    t[2] = -M * z * azp / (zp * rth);
+}
+
+void nbm_ext_accel(const double* P, const double* pos, double* acc)
+{
+    const double x = pos[0], y = pos[1], z = pos[2];
+    const double limit = 0.00390625;
+    double ab = sqrt(z * z + (y * y + x * x));
+    double r = (double) (ab <= limit) * limit + (double) (ab > limit) * ab;
+    double a[3] = { 0.0, 0.0, 0.0 }, t[3] = { 0.0, 0.0, 0.0 };
+    if ((int) P[3] == 1) nbm_mn(P[4], P[5], P[6], x, y, z, a);
+    t[0] = t[1] = t[2] = 0.0;
+    if ((int) P[7] == 1) nbm_mn(P[8], P[9], P[10], x, y, z, t);
+    a[0] += t[0]; a[1] += t[1]; a[2] += t[2];
+    t[0] = t[1] = t[2] = 0.0;
+    if ((int) P[11] == 1) {
+        double v0 = P[12], d = P[13], q = P[14];
+        double zq = z / q;
+        double denom = d * d + x * x + y * y + zq * zq;
+        double k = -2.0 * v0 * v0 / denom;
+        t[0] = k * x; t[1] = k * y; t[2] = k * z / (q * q);
+    } else if ((int) P[11] == 2) {
+        double ha = P[13];
+        double M = P[12] * P[12] * ha / 0.2162165954;
+        double ar = ha + r;
+        double c = (-M / (r * r)) * (log(ar / ha) / r - 1 / ar);
+        t[0] = c * x; t[1] = c * y; t[2] = c * z;
+    }
+    a[0] += t[0]; a[1] += t[1]; a[2] += t[2];
+    t[0] = t[1] = t[2] = 0.0;
+    if ((int) P[0] == 1) {
+        double tmp = P[2] + r;
+        double s = -P[1] / (r * (tmp * tmp));
+        t[0] = s * x; t[1] = s * y; t[2] = s * z;
+    } else if ((int) P[0] == 2) {
+        double tmp = sqrt(P[2] * P[2] + r * r);
+        double s = -P[1] / nbm_cube(tmp);
+        t[0] = s * x; t[1] = s * y; t[2] = s * z;
+    }
+    acc[0] = a[0] + t[0];
+    acc[1] = a[1] + t[1];
+    acc[2] = a[2] + t[2];
+}
+
+void nbm_lbr_to_cartesian(const double* lbr, double sun, double* xyz)
+{
+    const double l = lbr[0] * 3.14159265358979323846 / 180.0;
+    const double b = lbr[1] * 3.14159265358979323846 / 180.0;
+    const double r = lbr[2];
+    xyz[0] = r * cos(l) * cos(b) - sun;
+    xyz[1] = r * sin(l) * cos(b);
+    xyz[2] = r * sin(b);
+}
+
+void nbm_reverse_orbit(const double* pot, const double* pos, const double* vel, double tstop,
+                       double dt, double* fpos, double* fvel)
+{
+    double x[3] = { pos[0], pos[1], pos[2] };
+    double v[3] = { -vel[0], -vel[1], -vel[2] };
+    double a[3];
+    const double dt_half = dt / 2.0;
+    nbm_ext_accel(pot, x, a);
+    for (double t = 0; t >= tstop * (-1); t -= dt) {
+        for (int d = 0; d < 3; ++d) v[d] += dt_half * a[d];
+        for (int d = 0; d < 3; ++d) x[d] += dt * v[d];
+        nbm_ext_accel(pot, x, a);
+        for (int d = 0; d < 3; ++d) v[d] += dt_half * a[d];
+    }
+    for (int d = 0; d < 3; ++d) {
+        fpos[d] = x[d];
+        fvel[d] = -v[d];
+    }
+}
+
+/* two-component Plummer density and enclosed-mass integrand used for the
+ * Henon length unit (nbody/src/nbody_virial.c:22-36) */
+static double nbm_dwarf_density(double r, double ab, double ad, double mb, double md)
+{
+    double b = mb / pow(ab, 3.0) * pow(1.0 + r * r / ab / ab, -2.5);
+    double d = md / pow(ad, 3.0) * pow(1.0 + r * r / ad / ad, -2.5);
+    return 3.0 / 4.0 / 3.14159265358979323846 * (b + d);
+}
+
+static double nbm_dwarf_integral(double r, double ab, double ad, double mb, double md)
+{
+    double b = mb / ab * r * r * pow(1.0 + r * r / ab / ab, -0.5);
+    double d = md / ad * r * r * pow(1.0 + r * r / ad / ad, -0.5);
+    return 1.0 / 4.0 / 3.14159265358979323846 * (b + d);
+}
+
+double nbm_virial_radius(double a_b, double a_d, double m_b, double m_d)
+{
+    static const double w[5] = { 0.2369268850561891, 0.4786286704993665, 0.5688888888888889,
+                                 0.4786286704993665, 0.2369268850561891 };
+    static const double x[5] = { -0.9061798459386640, -0.5384693101056831, 0.0,
+                                 0.5384693101056831, 0.9061798459386640 };
+    const int n = 100000;
+    const double upper = 1000.0 * (a_b > a_d ? a_b : a_d);
+    const double width = (upper - 0.0) / (n * 1.0);
+    double integral = 0.0;
+    for (int i = 0; i < n; ++i) {
+        double a1 = 0.0 + i * width;
+        double b1 = 0.0 + (i + 1) * width;
+        for (int j = 0; j < 5; ++j) {
+            double r1 = width * x[j] / 2.0 + (a1 + b1) / 2.0;
+            integral += w[j] * nbm_dwarf_density(r1, a_b, a_d, m_b, m_d) *
+                        nbm_dwarf_integral(r1, a_b, a_d, m_b, m_d) * width / 2.0;
+        }
+    }
+    const double mt = m_b + m_d;
+    const double pi = 3.14159265358979323846;
+    return mt * mt / 16.0 / pi / pi / integral;
+}
+
+double nbm_calculate_eps2(double nbody, double a_b, double a_d, double m_b, double m_d)
+{
+    double rv = nbm_virial_radius(a_b, a_d, m_b, m_d);
+    double eps = rv * 0.98 * pow(nbody, -0.26);
+    double eps2 = (eps * eps) / 1.0;
+    if (eps2 <= 2.220446049250313e-16) eps2 = 2.220446049250313e-16;
+    return eps2;
+}
+
+double nbm_calculate_timestep(double mass, double r0)
+{
+    const double pi_4_3 = 4.18879020478639098461685784437;
+    return (1.0 / 10.0) * (1.0 / 10.0) * sqrt((pi_4_3 * (r0 * r0 * r0)) / mass);
+}

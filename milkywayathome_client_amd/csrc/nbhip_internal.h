@@ -27,6 +27,18 @@ struct __attribute__((aligned(16))) NodeW {
     int body;
 };
 
+/* Walk record of the mixed-precision walk (64 B, one scalar load).
+ * Position = origin + (hx + lx, hy + ly, hz + lz): the FP64 offset from the
+ * root centre of mass split into two floats, so that a float difference of
+ * two records keeps ~1e-7 relative accuracy at any distance from the
+ * origin.  q[6] is the quadrupole in FP32. */
+struct __attribute__((aligned(16))) NodeF {
+    float hx, hy, hz, m;
+    float lx, ly, lz, rcrit2;
+    int next, body;
+    float q[6];
+};
+
 /* Device-resident scalars of one simulation. */
 struct DevScalars {
     double rsize;                  /* st->tree.rsize, persists, grows only */
@@ -38,7 +50,10 @@ struct DevScalars {
     int maxdepth;
     unsigned status;               /* sticky NBHIP_* bits of device checks */
     unsigned incest;               /* a body missed itself in the walk */
-    unsigned long long counters[3];/* body-body, body-cell, opened */
+    unsigned long long counters[4];/* body-body, body-cell, opened, wave node steps */
+    int level_count[64];           /* cells per level */
+    int level_fill[64];
+    int level_off[65];             /* exclusive scan of level_count */
 };
 
 #define NBH_SORT_TILE 4096   /* keys per radix-sort block: 256 threads x 16 */
@@ -70,7 +85,8 @@ struct nbhip_state {
     double *nquad;                /* 6 per node */
     double *nmid;                 /* 3 per node (cells) */
     int *nlevel, *nparent;
-    unsigned *nchild, *narrive;
+    int *level_list;              /* cell pre-order indices grouped by level */
+    NodeF *nodef;                 /* mixed-precision walk records (precision MIXED) */
 
     DevScalars *scal;
 

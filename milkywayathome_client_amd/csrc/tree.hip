@@ -79,7 +79,11 @@ __global__ void k_rsize(DevScalars* scal)
     scal->ncell = 0;
     scal->nnode = 0;
     scal->maxdepth = 0;
-    scal->counters[0] = scal->counters[1] = scal->counters[2] = 0ull;
+    scal->counters[0] = scal->counters[1] = scal->counters[2] = scal->counters[3] = 0ull;
+    for (int l = 0; l < 64; ++l) {
+        scal->level_count[l] = 0;
+        scal->level_fill[l] = 0;
+    }
 }
 
 /* ---------------------------------------------------------------------- */
@@ -447,7 +451,6 @@ __device__ __forceinline__ double nbh_rcrit2(const nbhip_ctx& c, double mx, doub
 __global__ void k_node_counts(const unsigned* __restrict__ cincl, NodeW* __restrict__ node,
                               double* __restrict__ nquad, double* __restrict__ nmid,
                               int* __restrict__ nlevel, int* __restrict__ nparent,
-                              unsigned* __restrict__ nchild, unsigned* __restrict__ narrive,
                               int node_cap, nbhip_ctx ctx, DevScalars* __restrict__ scal)
 {
     const int M = scal->nmassive;
@@ -472,8 +475,7 @@ __global__ void k_node_counts(const unsigned* __restrict__ cincl, NodeW* __restr
         nmid[0] = nmid[1] = nmid[2] = 0.0;
         nlevel[0] = 0;
         nparent[0] = -1;
-        nchild[0] = 0;
-        narrive[0] = 0;
+        scal->level_count[0] = 1;
     }
 }
 
@@ -557,20 +559,12 @@ __device__ __forceinline__ int nbh_owner_pre(const EmitView& v, int k, int M)
     return lo + nbh_excl(v, lo) + (nbh_levels(v.bdelta[k]) - nbh_first_level(v, lo));
 }
 
-__global__ __launch_bounds__(256) void k_emit_cells(const int* __restrict__ blo,
-                                                    const int* __restrict__ bhi,
-                                                    const int* __restrict__ bdelta,
-                                                    const int* __restrict__ bparent,
-                                                    const int* __restrict__ dadj,
-                                                    const unsigned* __restrict__ cincl,
-                                                    const unsigned long long* __restrict__ su,
-                                                    int n, NodeW* __restrict__ node,
-                                                    double* __restrict__ nmid,
-                                                    int* __restrict__ nlevel,
-                                                    int* __restrict__ nparent,
-                                                    unsigned* __restrict__ nchild,
-                                                    unsigned* __restrict__ narrive,
-                                                    DevScalars* __restrict__ scal)
+__device__ __forceinline__ void nbh_emit_cells_body(
+    const int* __restrict__ blo, const int* __restrict__ bhi, const int* __restrict__ bdelta,
+    const int* __restrict__ bparent, const int* __restrict__ dadj,
+    const unsigned* __restrict__ cincl, const unsigned long long* __restrict__ su, int n,
+    NodeW* __restrict__ node, double* __restrict__ nmid, int* __restrict__ nlevel,
+    int* __restrict__ nparent, DevScalars* __restrict__ scal, int* lh)
 {
     const int k = blockIdx.x * 256 + threadIdx.x;
     const int M = scal->nmassive;
@@ -608,8 +602,6 @@ __global__ __launch_bounds__(256) void k_emit_cells(const int* __restrict__ blo,
         node[pre] = c;
         nlevel[pre] = l;
         nparent[pre] = (l == Lp + 1) ? top_parent : pre - 1;
-        nchild[pre] = 0;
-        narrive[pre] = 0;
         double mx = 0.0, my = 0.0, mz = 0.0;
         if (l > 0) {
             const double h = ldexp(R, -(l + 1));
@@ -621,8 +613,32 @@ __global__ __launch_bounds__(256) void k_emit_cells(const int* __restrict__ blo,
         nmid[3 * pre + 0] = mx;
         nmid[3 * pre + 1] = my;
         nmid[3 * pre + 2] = mz;
+        atomicAdd(&lh[l], 1);
     }
     atomicMax(&scal->maxdepth, Lk);
+}
+
+__global__ __launch_bounds__(256) void k_emit_cells(const int* __restrict__ blo,
+                                                    const int* __restrict__ bhi,
+                                                    const int* __restrict__ bdelta,
+                                                    const int* __restrict__ bparent,
+                                                    const int* __restrict__ dadj,
+                                                    const unsigned* __restrict__ cincl,
+                                                    const unsigned long long* __restrict__ su,
+                                                    int n, NodeW* __restrict__ node,
+                                                    double* __restrict__ nmid,
+                                                    int* __restrict__ nlevel,
+                                                    int* __restrict__ nparent,
+                                                    DevScalars* __restrict__ scal)
+{
+    __shared__ int lh[64];
+    if (threadIdx.x < 64) lh[threadIdx.x] = 0;
+    __syncthreads();
+    nbh_emit_cells_body(blo, bhi, bdelta, bparent, dadj, cincl, su, n, node, nmid, nlevel,
+                        nparent, scal, lh);
+    __syncthreads();
+    if (threadIdx.x < 64 && lh[threadIdx.x]) atomicAdd(&scal->level_count[threadIdx.x],
+                                                       lh[threadIdx.x]);
 }
 
 __global__ __launch_bounds__(256) void k_emit_leaves(const int* __restrict__ blo,
@@ -658,120 +674,127 @@ __global__ __launch_bounds__(256) void k_emit_leaves(const int* __restrict__ blo
     nparent[pre] = parent;
 }
 
-__global__ __launch_bounds__(256) void k_count_children(const int* __restrict__ nparent,
-                                                        unsigned* __restrict__ nchild,
-                                                        DevScalars* __restrict__ scal)
+/* Cells grouped by level: exclusive scan of the per-level counts, then
+ * each cell takes a slot in its level's segment (order within a level is
+ * irrelevant: cells of one level are independent). */
+__global__ void k_level_scan(DevScalars* scal)
 {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (scal->status & NBHIP_CELL_OVERFLOW_ERROR) return;
-    if (i >= scal->nnode) return;
-    const int p = nparent[i];
-    if (p >= i) {
-        atomicOr(&scal->status, NBHIP_ERROR);
-        return;
+    int acc = 0;
+    for (int l = 0; l < 64; ++l) {
+        scal->level_off[l] = acc;
+        acc += scal->level_count[l];
     }
-    if (p >= 0) atomicAdd(&nchild[p], 1u);
+    scal->level_off[64] = acc;
+}
+
+__global__ __launch_bounds__(256) void k_level_scatter(const int* __restrict__ nlevel,
+                                                       int* __restrict__ level_list,
+                                                       DevScalars* __restrict__ scal)
+{
+    __shared__ int lh[64];
+    __shared__ int base[64];
+    if (threadIdx.x < 64) lh[threadIdx.x] = 0;
+    __syncthreads();
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const bool ok = !(scal->status & (NBHIP_CELL_OVERFLOW_ERROR | NBHIP_ERROR)) &&
+                    i < scal->nnode;
+    const int l = ok ? nlevel[i] : -1;
+    const bool cell = l >= 0 && l <= 63;
+    const int rank = cell ? atomicAdd(&lh[l], 1) : 0;
+    __syncthreads();
+    if (threadIdx.x < 64 && lh[threadIdx.x])
+        base[threadIdx.x] = atomicAdd(&scal->level_fill[threadIdx.x], lh[threadIdx.x]);
+    __syncthreads();
+    if (cell) level_list[scal->level_off[l] + base[l] + rank] = i;
 }
 
 /* ---------------------------------------------------------------------- */
-/* bottom-up summarisation: the last child to arrive finishes its parent   */
+/* bottom-up summarisation, one launch per level, deepest first            */
 /* ---------------------------------------------------------------------- */
 /* hackCofM + findRCrit + nbCheckTreeStructure (nbody/src/nbody_tree.c:
- * 345-424) and hackQuad (:70-121), children in octant (pre-)order. */
-__global__ __launch_bounds__(256) void k_summarize(const unsigned* __restrict__ cincl,
-                                                   NodeW* node, double* nquad,
-                                                   const double* __restrict__ nmid,
-                                                   const int* __restrict__ nlevel,
-                                                   const int* __restrict__ nparent,
-                                                   const unsigned* __restrict__ nchild,
-                                                   unsigned* narrive, nbhip_ctx ctx,
-                                                   DevScalars* scal)
+ * 345-424) and hackQuad (:70-121), children in octant (pre-)order.  All
+ * children of a level-l cell are bodies or level-(l+1) cells, finished by an
+ * earlier launch. */
+__global__ __launch_bounds__(256) void k_summarize_level(const int* __restrict__ level_list,
+                                                         int level, NodeW* __restrict__ node,
+                                                         double* __restrict__ nquad,
+                                                         const double* __restrict__ nmid,
+                                                         nbhip_ctx ctx, DevScalars* scal)
 {
-    const int s = blockIdx.x * 256 + threadIdx.x;
-    const int M = scal->nmassive;
-    if (s >= M) return;
-    if (scal->status & NBHIP_CELL_OVERFLOW_ERROR) return;
-    if (scal->status & NBHIP_ERROR) return;
-    const double rsize = scal->rsize;
+    if (scal->status & (NBHIP_CELL_OVERFLOW_ERROR | NBHIP_ERROR)) return;
+    const int cnt = scal->level_count[level];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= cnt) return;
+    const int p = level_list[scal->level_off[level] + i];
     const int nn = scal->nnode;
-    int self = s + (int) cincl[s];
-    if (self >= nn) return;
-    int p = nparent[self];
-    while (p >= 0) {
-        if (p >= self) {
-            atomicOr(&scal->status, NBHIP_ERROR);
-            return;
-        }
-        self = p;
-        __threadfence();
-        const unsigned old = atomicAdd(&narrive[p], 1u);
-        if (old + 1u != nchild[p]) return;
-        __threadfence();
+    const double rsize = scal->rsize;
 
-        const int end = node[p].next;
-        if (end <= p || end > nn) {
+    const int end = node[p].next;
+    if (p < 0 || end <= p || end > nn) {
+        atomicOr(&scal->status, NBHIP_ERROR);
+        return;
+    }
+    double mass = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
+    for (int c = p + 1; c < end;) {
+        const NodeW ch = node[c];
+        mass += ch.m;
+        cx += ch.m * ch.x;
+        cy += ch.m * ch.y;
+        cz += ch.m * ch.z;
+        if (ch.next <= c || ch.next > end) {
             atomicOr(&scal->status, NBHIP_ERROR);
             return;
         }
-        double mass = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
+        c = ch.next;
+    }
+    const double mx = nmid[3 * p + 0], my = nmid[3 * p + 1], mz = nmid[3 * p + 2];
+    if (mass > 0.0) {
+        cx /= mass;
+        cy /= mass;
+        cz /= mass;
+    } else {
+        cx = mx; cy = my; cz = mz;
+    }
+    const double psize = ldexp(rsize, -level);
+    const double half = 0.5 * psize;
+    if (cx < mx - half || cx > mx + half || cy < my - half || cy > my + half ||
+        cz < mz - half || cz > mz + half)
+        atomicOr(&scal->status, NBHIP_TREE_STRUCTURE_ERROR);
+    const double rc2 = nbh_rcrit2(ctx, mx, my, mz, rsize, cx, cy, cz, psize);
+
+    if (ctx.use_quad) {
+        double q0 = 0.0, q1 = 0.0, q2 = 0.0, q3 = 0.0, q4 = 0.0, q5 = 0.0;
         for (int c = p + 1; c < end;) {
             const NodeW ch = node[c];
-            mass += ch.m;
-            cx += ch.m * ch.x;
-            cy += ch.m * ch.y;
-            cz += ch.m * ch.z;
-            if (ch.next <= c || ch.next > end) {
-                atomicOr(&scal->status, NBHIP_ERROR);
-                return;
+            const double dx = ch.x - cx, dy = ch.y - cy, dz = ch.z - cz;
+            const double drsq = dz * dz + (dy * dy + dx * dx);
+            const double m = ch.m;
+            double e0 = m * (3.0 * (dx * dx) - drsq);
+            double e1 = m * (3.0 * (dx * dy));
+            double e2 = m * (3.0 * (dx * dz));
+            double e3 = m * (3.0 * (dy * dy) - drsq);
+            double e4 = m * (3.0 * (dy * dz));
+            double e5 = m * (3.0 * (dz * dz) - drsq);
+            if (ch.body < 0) {
+                const double* cq = &nquad[6 * c];
+                e0 += cq[0]; e1 += cq[1]; e2 += cq[2];
+                e3 += cq[3]; e4 += cq[4]; e5 += cq[5];
             }
+            q0 += e0; q1 += e1; q2 += e2; q3 += e3; q4 += e4; q5 += e5;
             c = ch.next;
         }
-        const double mx = nmid[3 * p + 0], my = nmid[3 * p + 1], mz = nmid[3 * p + 2];
-        if (mass > 0.0) {
-            cx /= mass;
-            cy /= mass;
-            cz /= mass;
-        } else {
-            cx = mx; cy = my; cz = mz;
-        }
-        const double psize = ldexp(rsize, -nlevel[p]);
-        const double half = 0.5 * psize;
-        if (cx < mx - half || cx > mx + half || cy < my - half || cy > my + half ||
-            cz < mz - half || cz > mz + half)
-            atomicOr(&scal->status, NBHIP_TREE_STRUCTURE_ERROR);
-        const double rc2 = nbh_rcrit2(ctx, mx, my, mz, rsize, cx, cy, cz, psize);
-
-        if (ctx.use_quad) {
-            double q0 = 0.0, q1 = 0.0, q2 = 0.0, q3 = 0.0, q4 = 0.0, q5 = 0.0;
-            for (int c = p + 1; c < end;) {
-                const NodeW ch = node[c];
-                const double dx = ch.x - cx, dy = ch.y - cy, dz = ch.z - cz;
-                const double drsq = dz * dz + (dy * dy + dx * dx);
-                const double m = ch.m;
-                double e0 = m * (3.0 * (dx * dx) - drsq);
-                double e1 = m * (3.0 * (dx * dy));
-                double e2 = m * (3.0 * (dx * dz));
-                double e3 = m * (3.0 * (dy * dy) - drsq);
-                double e4 = m * (3.0 * (dy * dz));
-                double e5 = m * (3.0 * (dz * dz) - drsq);
-                if (ch.body < 0) {
-                    const double* cq = &nquad[6 * c];
-                    e0 += cq[0]; e1 += cq[1]; e2 += cq[2];
-                    e3 += cq[3]; e4 += cq[4]; e5 += cq[5];
-                }
-                q0 += e0; q1 += e1; q2 += e2; q3 += e3; q4 += e4; q5 += e5;
-                c = ch.next;
-            }
-            double* pq = &nquad[6 * p];
-            pq[0] = q0; pq[1] = q1; pq[2] = q2; pq[3] = q3; pq[4] = q4; pq[5] = q5;
-        }
-        node[p].x = cx;
-        node[p].y = cy;
-        node[p].z = cz;
-        node[p].m = mass;
-        node[p].rcrit2 = rc2;
-        p = nparent[p];
+        double* pq = &nquad[6 * p];
+        pq[0] = q0; pq[1] = q1; pq[2] = q2; pq[3] = q3; pq[4] = q4; pq[5] = q5;
     }
+    NodeW w;
+    w.x = cx;
+    w.y = cy;
+    w.z = cz;
+    w.m = mass;
+    w.rcrit2 = rc2;
+    w.next = end;
+    w.body = -1;
+    node[p] = w;
 }
 
 /* Structure-error rule of nbLoadBody (nbody/src/nbody_tree.c:256-264):
@@ -843,22 +866,37 @@ void nbh_build(nbhip_state* st)
     hipLaunchKernelGGL(k_scan_add, dim3(nscan), dim3(1024), 0, s, st->cincl, st->scan_part,
                        st->scal);
     hipLaunchKernelGGL(k_node_counts, dim3(1), dim3(1), 0, s, st->cincl, st->node, st->nquad,
-                       st->nmid, st->nlevel, st->nparent, st->nchild, st->narrive, st->node_cap,
+                       st->nmid, st->nlevel, st->nparent, st->node_cap,
                        st->ctx, st->scal);
     hipLaunchKernelGGL(k_karras, dim3(g256), dim3(256), 0, s, n, st->su, st->blo, st->bhi,
                        st->bdelta, st->bparent, st->scal);
     hipLaunchKernelGGL(k_emit_cells, dim3(g256), dim3(256), 0, s, st->blo, st->bhi, st->bdelta,
                        st->bparent, st->dadj, st->cincl, st->su, n, st->node, st->nmid,
-                       st->nlevel, st->nparent, st->nchild, st->narrive, st->scal);
+                       st->nlevel, st->nparent, st->scal);
     hipLaunchKernelGGL(k_emit_leaves, dim3(g256), dim3(256), 0, s, st->blo, st->bdelta,
                        st->bparent, st->dadj, st->cincl, st->spos, st->node, st->nlevel,
                        st->nparent, st->scal);
-    hipLaunchKernelGGL(k_count_children, dim3(nbh_blocks(st->node_cap, 256)), dim3(256), 0, s,
-                       st->nparent, st->nchild, st->scal);
-    nbh_mark(st, 3);
-    hipLaunchKernelGGL(k_summarize, dim3(g256), dim3(256), 0, s, st->cincl, st->node, st->nquad,
-                       st->nmid, st->nlevel, st->nparent, st->nchild, st->narrive, st->ctx,
-                       st->scal);
+    hipLaunchKernelGGL(k_level_scan, dim3(1), dim3(1), 0, s, st->scal);
+    hipLaunchKernelGGL(k_level_scatter, dim3(nbh_blocks(st->node_cap, 256)), dim3(256), 0, s,
+                       st->nlevel, st->level_list, st->scal);
     hipLaunchKernelGGL(k_depth_check, dim3(1), dim3(1), 0, s, st->scal);
+    nbh_mark(st, 3);
+    /* per-level launches sized from the device counts (one small readback) */
+    int counts[64];
+    int maxdepth = 63;
+    const bool have = hipMemcpyAsync(counts, st->scal->level_count, sizeof(counts),
+                                     hipMemcpyDeviceToHost, s) == hipSuccess &&
+                      hipStreamSynchronize(s) == hipSuccess;
+    if (have) {
+        maxdepth = 0;
+        for (int l = 0; l < 64; ++l)
+            if (counts[l] > 0) maxdepth = l;
+    }
+    for (int l = maxdepth; l >= 0; --l) {
+        const int g = have ? nbh_blocks(counts[l], 256) : nbh_blocks(st->node_cap, 256);
+        if (have && counts[l] == 0) continue;
+        hipLaunchKernelGGL(k_summarize_level, dim3(g), dim3(256), 0, s, st->level_list, l,
+                           st->node, st->nquad, st->nmid, st->ctx, st->scal);
+    }
     nbh_mark(st, 4);
 }

@@ -15,131 +15,309 @@
  */
 #include "nbhip_device.h"
 
-struct WalkArgs {
+/* Output side of one walk launch. */
+struct WalkOut {
     int n;
     int kick;
     int use_pot;
     int pad;
-    double eps2;
     double dtHalf;
-    const NodeW* node;
-    const double* nquad;
-    const double4* spos;
-    const unsigned* perm;
-    double* oax;
-    double* oay;
-    double* oaz;
-    double* vx;
-    double* vy;
-    double* vz;
-    DevScalars* scal;
+    double eps2;
     nbhip_potential pot;
 };
 
-template <bool QUAD, bool COUNT>
-__global__ __launch_bounds__(256) void k_walk64(WalkArgs a)
+/* Closing half of the force evaluation shared by both walks: external
+ * potential (nbMapForceBody, nbody/src/nbody_grav.c:164-182), scatter to
+ * the original body order, optional closing half-kick (advanceVelocities,
+ * nbody/src/nbody_plain.c:301-315). */
+__device__ __forceinline__ void nbh_walk_finish(const WalkOut& o, double px, double py,
+                                                double pz, double ax, double ay, double az,
+                                                unsigned j, double* __restrict__ oax,
+                                                double* __restrict__ oay,
+                                                double* __restrict__ oaz,
+                                                double* __restrict__ vx, double* __restrict__ vy,
+                                                double* __restrict__ vz)
 {
-    const int s = blockIdx.x * 256 + threadIdx.x;
-    if (a.scal->status & (NBHIP_CELL_OVERFLOW_ERROR | NBHIP_ERROR)) return;
-    const int M = a.scal->nmassive;
-    const int nn = a.scal->nnode;
-    const bool valid = s < a.n;
-    if (__ballot(valid) == 0ull) return;
-    const double4 p = valid ? a.spos[s] : make_double4(0.0, 0.0, 0.0, 0.0);
-    const int self = (valid && s < M) ? s : -2;
-    int lp = valid ? 0 : nn;
-    double ax = 0.0, ay = 0.0, az = 0.0;
-    bool met = false;
-    unsigned nbb = 0, nbc = 0, nop = 0;
-    const double eps2 = a.eps2;
-
-    int q = 0;
-    while (q < nn) {
-        q = __builtin_amdgcn_readfirstlane(q);
-        const NodeW nd = a.node[q];
-        bool open = false;
-        if (lp == q) {
-            const double dx = nd.x - p.x;
-            const double dy = nd.y - p.y;
-            const double dz = nd.z - p.z;
-            double drSq = dz * dz + (dy * dy + dx * dx);
-            if (nd.body >= 0 || drSq >= nd.rcrit2) {
-                if (nd.body != self) {
-                    drSq += eps2;
-                    const double drab = sqrt(drSq);
-                    const double phii = nd.m / drab;
-                    const double mor3 = phii / drSq;
-                    ax += mor3 * dx;
-                    ay += mor3 * dy;
-                    az += mor3 * dz;
-                    if (QUAD && nd.body < 0) {
-                        const double* Q = a.nquad + 6 * q;
-                        const double qxx = Q[0], qxy = Q[1], qxz = Q[2];
-                        const double qyy = Q[3], qyz = Q[4], qzz = Q[5];
-                        const double qx = qxx * dx + qxy * dy + qxz * dz;
-                        const double qy = qxy * dx + qyy * dy + qyz * dz;
-                        const double qz = qxz * dx + qyz * dy + qzz * dz;
-                        const double drQdr = qx * dx + qy * dy + qz * dz;
-                        const double dr5inv = 1.0 / ((drSq * drSq) * drab);
-                        const double phiQ = 2.5 * (dr5inv * drQdr) / drSq;
-                        ax += phiQ * dx;
-                        ay += phiQ * dy;
-                        az += phiQ * dz;
-                        ax -= dr5inv * qx;
-                        ay -= dr5inv * qy;
-                        az -= dr5inv * qz;
-                    }
-                    if (COUNT) {
-                        if (nd.body >= 0) ++nbb; else ++nbc;
-                    }
-                } else {
-                    met = true;
-                }
-                lp = nd.next;
-            } else {
-                lp = q + 1;
-                open = true;
-                if (COUNT) ++nop;
-            }
-        }
-        const int qn = __ballot(open) ? q + 1 : nd.next;
-        if (qn <= q) {
-            /* a Next() link that does not advance: corrupt table, stop */
-            if (__lane_id() == 0) atomicOr(&a.scal->status, NBHIP_ERROR);
-            break;
-        }
-        q = qn;
-    }
-
-    if (COUNT) {
-        unsigned long long c0 = nbh_wave_sum_u64(nbb);
-        unsigned long long c1 = nbh_wave_sum_u64(nbc);
-        unsigned long long c2 = nbh_wave_sum_u64(nop);
-        if (__lane_id() == 0) {
-            atomicAdd(&a.scal->counters[0], c0);
-            atomicAdd(&a.scal->counters[1], c1);
-            atomicAdd(&a.scal->counters[2], c2);
-        }
-    }
-    if (!valid) return;
-    if (!met) atomicOr(&a.scal->incest, 1u);
-    if (a.use_pot) {
+    if (o.use_pot) {
         double ex, ey, ez;
-        nbh_ext_accel(a.pot, p.x, p.y, p.z, ex, ey, ez);
+        nbh_ext_accel(o.pot, px, py, pz, ex, ey, ez);
         ax = ax + ex;
         ay = ay + ey;
         az = az + ez;
     }
-    const unsigned j = a.perm[s];
-    a.oax[j] = ax;
-    a.oay[j] = ay;
-    a.oaz[j] = az;
-    if (a.kick) {
-        /* advanceVelocities (nbody/src/nbody_plain.c:301-315) */
-        a.vx[j] = a.vx[j] + a.dtHalf * (ax + 0.0);
-        a.vy[j] = a.vy[j] + a.dtHalf * (ay + 0.0);
-        a.vz[j] = a.vz[j] + a.dtHalf * (az + 0.0);
+    oax[j] = ax;
+    oay[j] = ay;
+    oaz[j] = az;
+    if (o.kick) {
+        vx[j] = vx[j] + o.dtHalf * (ax + 0.0);
+        vy[j] = vy[j] + o.dtHalf * (ay + 0.0);
+        vz[j] = vz[j] + o.dtHalf * (az + 0.0);
     }
+}
+
+__device__ __forceinline__ void nbh_walk_counters(DevScalars* scal, unsigned nbb, unsigned nbc,
+                                                  unsigned nop, unsigned steps)
+{
+    unsigned long long c0 = nbh_wave_sum_u64(nbb);
+    unsigned long long c1 = nbh_wave_sum_u64(nbc);
+    unsigned long long c2 = nbh_wave_sum_u64(nop);
+    if (__lane_id() == 0) {
+        atomicAdd(&scal->counters[0], c0);
+        atomicAdd(&scal->counters[1], c1);
+        atomicAdd(&scal->counters[2], c2);
+        atomicAdd(&scal->counters[3], (unsigned long long) steps);
+    }
+}
+
+/* Plain form: FP64 end to end, reference operation order. */
+template <bool QUAD, bool COUNT>
+__global__ __launch_bounds__(256) void k_walk64(const NodeW* __restrict__ node,
+                                                const double* __restrict__ nquad,
+                                                const double4* __restrict__ spos,
+                                                const unsigned* __restrict__ perm,
+                                                double* __restrict__ oax,
+                                                double* __restrict__ oay,
+                                                double* __restrict__ oaz,
+                                                double* __restrict__ vx,
+                                                double* __restrict__ vy,
+                                                double* __restrict__ vz,
+                                                DevScalars* __restrict__ scal, WalkOut o)
+{
+    const int s = blockIdx.x * 256 + threadIdx.x;
+    if (scal->status & (NBHIP_CELL_OVERFLOW_ERROR | NBHIP_ERROR)) return;
+    const int M = scal->nmassive;
+    const int nn = scal->nnode;
+    const bool valid = s < o.n;
+    if (__ballot(valid) == 0ull) return;
+    const double4 p = valid ? spos[s] : make_double4(0.0, 0.0, 0.0, 0.0);
+    const int self = (valid && s < M) ? s : -2;
+    int lp = valid ? 0 : nn;
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    bool met = false;
+    unsigned nbb = 0, nbc = 0, nop = 0, steps = 0;
+    const double eps2 = o.eps2;
+
+    /* The opening tests come first: once the wave's vote is known the
+     * successor node is fetched, and its scalar-load latency overlaps the
+     * interaction math of the current node. */
+    bool bad = false;
+    int q = 0;
+    NodeW nd = node[0];
+    while (q < nn) {
+        if (COUNT) ++steps;
+        const bool active = lp == q;
+        const double dx = nd.x - p.x;
+        const double dy = nd.y - p.y;
+        const double dz = nd.z - p.z;
+        double drSq = dz * dz + (dy * dy + dx * dx);
+        const bool accept = active && (nd.body >= 0 || drSq >= nd.rcrit2);
+        const bool open = active && !accept;
+        const bool any_open = __ballot(open) != 0ull;
+        const int qn = __builtin_amdgcn_readfirstlane(any_open ? q + 1 : nd.next);
+        if (qn <= q) {
+            /* a Next() link that does not advance: corrupt table, stop */
+            bad = true;
+            break;
+        }
+        const NodeW nxt = node[min(qn, nn - 1)];
+        if (accept) {
+            if (nd.body != self) {
+                drSq += eps2;
+                const double drab = sqrt(drSq);
+                const double phii = nd.m / drab;
+                const double mor3 = phii / drSq;
+                ax += mor3 * dx;
+                ay += mor3 * dy;
+                az += mor3 * dz;
+                if (QUAD && nd.body < 0) {
+                    const double* Q = nquad + 6 * q;
+                    const double qxx = Q[0], qxy = Q[1], qxz = Q[2];
+                    const double qyy = Q[3], qyz = Q[4], qzz = Q[5];
+                    const double qx = qxx * dx + qxy * dy + qxz * dz;
+                    const double qy = qxy * dx + qyy * dy + qyz * dz;
+                    const double qz = qxz * dx + qyz * dy + qzz * dz;
+                    const double drQdr = qx * dx + qy * dy + qz * dz;
+                    const double dr5inv = 1.0 / ((drSq * drSq) * drab);
+                    const double phiQ = 2.5 * (dr5inv * drQdr) / drSq;
+                    ax += phiQ * dx;
+                    ay += phiQ * dy;
+                    az += phiQ * dz;
+                    ax -= dr5inv * qx;
+                    ay -= dr5inv * qy;
+                    az -= dr5inv * qz;
+                }
+                if (COUNT) {
+                    if (nd.body >= 0) ++nbb; else ++nbc;
+                }
+            } else {
+                met = true;
+            }
+            lp = nd.next;
+        } else if (open) {
+            lp = q + 1;
+            if (COUNT) ++nop;
+        }
+        q = qn;
+        nd = nxt;
+    }
+
+    if (bad && __lane_id() == 0) atomicOr(&scal->status, NBHIP_ERROR);
+    if (COUNT) nbh_walk_counters(scal, nbb, nbc, nop, steps);
+    if (!valid) return;
+    if (!met) atomicOr(&scal->incest, 1u);
+    nbh_walk_finish(o, p.x, p.y, p.z, ax, ay, az, perm[s], oax, oay, oaz, vx, vy, vz);
+}
+
+/* Mixed form: displacements and interactions in FP32, FP32 partial sums
+ * over batches of NBH_MIXED_BATCH node steps, batches added into FP64 with
+ * Kahan compensation.  Positions, potential and kicks stay FP64. */
+#define NBH_MIXED_BATCH 32
+
+__device__ __forceinline__ void nbh_kahan_add(double& sum, double& comp, double v)
+{
+    const double y = v - comp;
+    const double t = sum + y;
+    comp = (t - sum) - y;
+    sum = t;
+}
+
+template <bool QUAD, bool COUNT>
+__global__ __launch_bounds__(256) void k_walk32(const NodeF* __restrict__ node,
+                                                const NodeW* __restrict__ root,
+                                                const double4* __restrict__ spos,
+                                                const unsigned* __restrict__ perm,
+                                                double* __restrict__ oax,
+                                                double* __restrict__ oay,
+                                                double* __restrict__ oaz,
+                                                double* __restrict__ vx,
+                                                double* __restrict__ vy,
+                                                double* __restrict__ vz,
+                                                DevScalars* __restrict__ scal, WalkOut o)
+{
+    const int s = blockIdx.x * 256 + threadIdx.x;
+    if (scal->status & (NBHIP_CELL_OVERFLOW_ERROR | NBHIP_ERROR)) return;
+    const int M = scal->nmassive;
+    const int nn = scal->nnode;
+    const bool valid = s < o.n;
+    if (__ballot(valid) == 0ull) return;
+    const double4 p = valid ? spos[s] : make_double4(0.0, 0.0, 0.0, 0.0);
+    const NodeW r0 = root[0];
+    const double ox = p.x - r0.x, oy = p.y - r0.y, oz = p.z - r0.z;
+    const float phx = (float) ox, phy = (float) oy, phz = (float) oz;
+    const float plx = (float) (ox - (double) phx);
+    const float ply = (float) (oy - (double) phy);
+    const float plz = (float) (oz - (double) phz);
+    const int self = (valid && s < M) ? s : -2;
+    int lp = valid ? 0 : nn;
+    const float eps2 = (float) o.eps2;
+    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+    double ax = 0.0, ay = 0.0, az = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
+    bool met = false;
+    unsigned nbb = 0, nbc = 0, nop = 0, steps = 0;
+    int batch = 0;
+
+    bool bad = false;
+    int q = 0;
+    NodeF nd = node[0];
+    while (q < nn) {
+        if (COUNT) ++steps;
+        const bool active = lp == q;
+        const float dx = (nd.hx - phx) + (nd.lx - plx);
+        const float dy = (nd.hy - phy) + (nd.ly - ply);
+        const float dz = (nd.hz - phz) + (nd.lz - plz);
+        float drSq = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+        const bool accept = active && (nd.body >= 0 || drSq >= nd.rcrit2);
+        const bool open = active && !accept;
+        const bool any_open = __ballot(open) != 0ull;
+        const int qn = __builtin_amdgcn_readfirstlane(any_open ? q + 1 : nd.next);
+        if (qn <= q) {
+            bad = true;
+            break;
+        }
+        const NodeF nxt = node[min(qn, nn - 1)];
+        if (accept) {
+            if (nd.body != self) {
+                drSq += eps2;
+                const float rinv = __builtin_amdgcn_rsqf(drSq);
+                const float rinv2 = rinv * rinv;
+                const float mor3 = nd.m * rinv * rinv2;
+                float fx = mor3 * dx, fy = mor3 * dy, fz = mor3 * dz;
+                if (QUAD && nd.body < 0) {
+                    const float qx = __builtin_fmaf(nd.q[0], dx, __builtin_fmaf(nd.q[1], dy, nd.q[2] * dz));
+                    const float qy = __builtin_fmaf(nd.q[1], dx, __builtin_fmaf(nd.q[3], dy, nd.q[4] * dz));
+                    const float qz = __builtin_fmaf(nd.q[2], dx, __builtin_fmaf(nd.q[4], dy, nd.q[5] * dz));
+                    const float drQdr = __builtin_fmaf(qx, dx, __builtin_fmaf(qy, dy, qz * dz));
+                    const float dr5inv = rinv2 * rinv2 * rinv;
+                    const float phiQ = 2.5f * dr5inv * drQdr * rinv2;
+                    fx = __builtin_fmaf(phiQ, dx, __builtin_fmaf(-dr5inv, qx, fx));
+                    fy = __builtin_fmaf(phiQ, dy, __builtin_fmaf(-dr5inv, qy, fy));
+                    fz = __builtin_fmaf(phiQ, dz, __builtin_fmaf(-dr5inv, qz, fz));
+                }
+                sx += fx;
+                sy += fy;
+                sz += fz;
+                if (COUNT) {
+                    if (nd.body >= 0) ++nbb; else ++nbc;
+                }
+            } else {
+                met = true;
+            }
+            lp = nd.next;
+        } else if (open) {
+            lp = q + 1;
+            if (COUNT) ++nop;
+        }
+        if (++batch == NBH_MIXED_BATCH) {
+            batch = 0;
+            nbh_kahan_add(ax, cx, (double) sx);
+            nbh_kahan_add(ay, cy, (double) sy);
+            nbh_kahan_add(az, cz, (double) sz);
+            sx = sy = sz = 0.0f;
+        }
+        q = qn;
+        nd = nxt;
+    }
+    nbh_kahan_add(ax, cx, (double) sx);
+    nbh_kahan_add(ay, cy, (double) sy);
+    nbh_kahan_add(az, cz, (double) sz);
+
+    if (bad && __lane_id() == 0) atomicOr(&scal->status, NBHIP_ERROR);
+    if (COUNT) nbh_walk_counters(scal, nbb, nbc, nop, steps);
+    if (!valid) return;
+    if (!met) atomicOr(&scal->incest, 1u);
+    nbh_walk_finish(o, p.x, p.y, p.z, ax, ay, az, perm[s], oax, oay, oaz, vx, vy, vz);
+}
+
+/* FP32 walk records from the FP64 tree, offsets from the root centre of
+ * mass. */
+__global__ __launch_bounds__(256) void k_pack32(const NodeW* __restrict__ node,
+                                                const double* __restrict__ nquad,
+                                                NodeF* __restrict__ nodef,
+                                                const DevScalars* __restrict__ scal)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (scal->status & (NBHIP_CELL_OVERFLOW_ERROR | NBHIP_ERROR)) return;
+    if (i >= scal->nnode) return;
+    const NodeW r0 = node[0];
+    const NodeW w = node[i];
+    const double dx = w.x - r0.x, dy = w.y - r0.y, dz = w.z - r0.z;
+    NodeF f;
+    f.hx = (float) dx;
+    f.hy = (float) dy;
+    f.hz = (float) dz;
+    f.lx = (float) (dx - (double) f.hx);
+    f.ly = (float) (dy - (double) f.hy);
+    f.lz = (float) (dz - (double) f.hz);
+    f.m = (float) w.m;
+    f.rcrit2 = (float) w.rcrit2;
+    f.next = w.next;
+    f.body = w.body;
+    if (w.body < 0) {
+        for (int k = 0; k < 6; ++k) f.q[k] = (float) nquad[6 * i + k];
+    } else {
+        for (int k = 0; k < 6; ++k) f.q[k] = 0.0f;
+    }
+    nodef[i] = f;
 }
 
 /* nbMapForceBody_Exact (nbody/src/nbody_grav.c:229-293): O(N^2) in body
@@ -246,33 +424,33 @@ void nbh_walk(nbhip_state* st, int kick, int count, double* oax, double* oay, do
                            oay, oaz, st->vx, st->vy, st->vz);
         return;
     }
-    WalkArgs a;
-    a.n = st->n;
-    a.kick = kick;
-    a.use_pot = self_only ? 0 : c.use_potential;
-    a.pad = 0;
-    a.eps2 = c.eps2;
-    a.dtHalf = 0.5 * c.timestep;
-    a.node = st->node;
-    a.nquad = st->nquad;
-    a.spos = st->spos;
-    a.perm = st->perm;
-    a.oax = oax;
-    a.oay = oay;
-    a.oaz = oaz;
-    a.vx = st->vx;
-    a.vy = st->vy;
-    a.vz = st->vz;
-    a.scal = st->scal;
-    a.pot = c.pot;
+    WalkOut o;
+    o.n = st->n;
+    o.kick = kick;
+    o.use_pot = self_only ? 0 : c.use_potential;
+    o.pad = 0;
+    o.dtHalf = 0.5 * c.timestep;
+    o.eps2 = c.eps2;
+    o.pot = c.pot;
     dim3 grid(nbh_blocks(st->n, 256)), block(256);
-    if (c.use_quad) {
-        if (count) hipLaunchKernelGGL((k_walk64<true, true>), grid, block, 0, st->stream, a);
-        else hipLaunchKernelGGL((k_walk64<true, false>), grid, block, 0, st->stream, a);
-    } else {
-        if (count) hipLaunchKernelGGL((k_walk64<false, true>), grid, block, 0, st->stream, a);
-        else hipLaunchKernelGGL((k_walk64<false, false>), grid, block, 0, st->stream, a);
+    hipStream_t s = st->stream;
+    if (c.precision == NBHIP_PRECISION_MIXED) {
+        hipLaunchKernelGGL(k_pack32, dim3(nbh_blocks(st->node_cap, 256)), block, 0, s, st->node,
+                           st->nquad, st->nodef, st->scal);
+#define NBH_W32(Q, C)                                                                        \
+    hipLaunchKernelGGL((k_walk32<Q, C>), grid, block, 0, s, st->nodef, st->node, st->spos,    \
+                       st->perm, oax, oay, oaz, st->vx, st->vy, st->vz, st->scal, o)
+        if (c.use_quad) { if (count) NBH_W32(true, true); else NBH_W32(true, false); }
+        else { if (count) NBH_W32(false, true); else NBH_W32(false, false); }
+#undef NBH_W32
+        return;
     }
+#define NBH_W64(Q, C)                                                                        \
+    hipLaunchKernelGGL((k_walk64<Q, C>), grid, block, 0, s, st->node, st->nquad, st->spos,    \
+                       st->perm, oax, oay, oaz, st->vx, st->vy, st->vz, st->scal, o)
+    if (c.use_quad) { if (count) NBH_W64(true, true); else NBH_W64(true, false); }
+    else { if (count) NBH_W64(false, true); else NBH_W64(false, false); }
+#undef NBH_W64
 }
 
 void nbh_direct(nbhip_state* st, int nsample, const int* d_idx, double* d_acc)

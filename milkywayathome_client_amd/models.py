@@ -44,6 +44,18 @@ def load_models() -> C.CDLL:
     lib.nbm_plummer.argtypes = [sp, C.c_int, C.c_double, C.c_double, dp, dp, C.c_int, dp, dp, dp,
                                 C.POINTER(C.c_int)]
     lib.nbm_plummer.restype = C.c_int
+    lib.nbm_ext_accel.argtypes = [dp, dp, dp]
+    lib.nbm_ext_accel.restype = None
+    lib.nbm_lbr_to_cartesian.argtypes = [dp, C.c_double, dp]
+    lib.nbm_lbr_to_cartesian.restype = None
+    lib.nbm_reverse_orbit.argtypes = [dp, dp, dp, C.c_double, C.c_double, dp, dp]
+    lib.nbm_reverse_orbit.restype = None
+    lib.nbm_virial_radius.argtypes = [C.c_double] * 4
+    lib.nbm_virial_radius.restype = C.c_double
+    lib.nbm_calculate_eps2.argtypes = [C.c_double] * 5
+    lib.nbm_calculate_eps2.restype = C.c_double
+    lib.nbm_calculate_timestep.argtypes = [C.c_double] * 2
+    lib.nbm_calculate_timestep.restype = C.c_double
     _LIB = lib
     return lib
 
@@ -94,3 +106,58 @@ def plummer(nbody: int, prng: DSFMT, mass: float, scaleRadius: float,
     if rc != 0:
         raise ValueError(f"plummer: bad arguments (nbody={nbody})")
     return pos, vel, m, typ
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def potential_array(potential) -> np.ndarray:
+    """Flatten a nbody.Potential into the 15-double layout of nbm_ext_accel."""
+    p = potential
+    return np.array([p.spherical.type, p.spherical.mass, p.spherical.scale,
+                     p.disk.type, p.disk.mass, p.disk.scaleLength, p.disk.scaleHeight,
+                     p.disk2.type, p.disk2.mass, p.disk2.scaleLength, p.disk2.scaleHeight,
+                     p.halo.type, p.halo.vhalo, p.halo.scaleLength, p.halo.flattenZ],
+                    dtype=np.float64)
+
+
+def lbr_to_cartesian(lbr, sun_gc_dist: float = 8.0) -> np.ndarray:
+    """lbrToCartesian (nbody/src/nbody_coordinates.c:68-71); degrees, kpc."""
+    lib = load_models()
+    a = np.ascontiguousarray(lbr, dtype=np.float64)
+    out = np.empty(3)
+    lib.nbm_lbr_to_cartesian(_dp(a), sun_gc_dist, _dp(out))
+    return out
+
+
+def reverse_orbit(potential, position, velocity, tstop: float, dt: float):
+    """reverseOrbit{potential, position, velocity, tstop, dt}
+    (nbody/src/nbody_orbit_integrator.c:44-86)."""
+    lib = load_models()
+    P = potential_array(potential)
+    x = np.ascontiguousarray(position, dtype=np.float64)
+    v = np.ascontiguousarray(velocity, dtype=np.float64)
+    fx, fv = np.empty(3), np.empty(3)
+    lib.nbm_reverse_orbit(_dp(P), _dp(x), _dp(v), tstop, dt, _dp(fx), _dp(fv))
+    return fx, fv
+
+
+def ext_accel(potential, pos) -> np.ndarray:
+    lib = load_models()
+    P = potential_array(potential)
+    pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+    out = np.empty_like(pos)
+    for i in range(pos.shape[0]):
+        lib.nbm_ext_accel(_dp(P), _dp(pos[i]), _dp(out[i]))
+    return out
+
+
+def calculate_eps2(nbody: float, a_b: float, a_d: float, m_b: float, m_d: float) -> float:
+    """calculateEps2(nbody, a_b, a_d, M_b, M_d, 0) (nbody/src/nbody_lua_models.c:158-169)."""
+    return load_models().nbm_calculate_eps2(float(nbody), a_b, a_d, m_b, m_d)
+
+
+def calculate_timestep(mass: float, r0: float) -> float:
+    """calculateTimestep (nbody/src/nbody_lua_models.c:139-142)."""
+    return load_models().nbm_calculate_timestep(mass, r0)

@@ -382,9 +382,10 @@ class NBodyState:
         return acc
 
     def walk_counters(self):
-        c = (C.c_ulonglong * 3)()
+        """(body-body, body-cell, opened, wave node steps) of a counting walk."""
+        c = (C.c_ulonglong * 4)()
         self._check(self.lib.nbhip_walk_counters(self._h, c))
-        return int(c[0]), int(c[1]), int(c[2])
+        return int(c[0]), int(c[1]), int(c[2]), int(c[3])
 
     def set_profiling(self, enable: bool = True) -> None:
         self._check(self.lib.nbhip_set_profiling(self._h, int(enable)))
