@@ -79,7 +79,7 @@ static void nbh_free_all(nbhip_state* st)
 {
     void* ptrs[] = { st->px,   st->py,     st->pz,    st->vx,      st->vy,     st->vz,
                      st->m,    st->ax,     st->ay,    st->az,      st->key,    st->key_alt,
-                     st->perm, st->perm_alt, st->hist, st->spos,   st->su,     st->dadj,
+                     st->perm, st->perm_alt, st->hist, st->dtot, st->spos,   st->su,     st->dadj,
                      st->cincl, st->scan_part, st->blo, st->bhi,   st->bdelta, st->bparent,
                      st->node, st->nquad,  st->nmid,  st->nlevel,  st->nparent,
                      st->level_list, st->nodef, st->scal };
@@ -121,6 +121,7 @@ extern "C" unsigned nbhip_create(const nbhip_ctx* ctx, int device, int nbody, nb
     A(ax, n); A(ay, n); A(az, n);
     A(key, n); A(key_alt, n); A(perm, n); A(perm_alt, n);
     A(hist, 256 * (size_t) st->sort_blocks);
+    A(dtot, 256);
     A(spos, n); A(su, 3 * n); A(dadj, n); A(cincl, n);
     A(scan_part, 4096);
     A(blo, n); A(bhi, n); A(bdelta, n); A(bparent, 2 * n);

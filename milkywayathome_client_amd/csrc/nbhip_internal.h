@@ -72,6 +72,7 @@ struct nbhip_state {
     unsigned long long *key, *key_alt;
     unsigned *perm, *perm_alt;
     unsigned *hist;
+    unsigned *dtot;               /* per-digit totals of one radix pass */
 
     double4 *spos;
     unsigned long long *su;       /* 3 * n: ux | uy | uz */

@@ -100,22 +100,30 @@ __global__ __launch_bounds__(256) void k_keys(int n, const double* __restrict__ 
                                               DevScalars* __restrict__ scal)
 {
     const int i = blockIdx.x * 256 + threadIdx.x;
-    unsigned massive = 0;
-    if (i < n) {
-        const double ks = scal->key_scale;
-        unsigned long long k = ~0ull;
-        if (m[i] != 0.0) {
-            unsigned long long ux = nbh_fixed(px[i], ks);
-            unsigned long long uy = nbh_fixed(py[i], ks);
-            unsigned long long uz = nbh_fixed(pz[i], ks);
-            k = nbh_morton21(ux >> 42, uy >> 42, uz >> 42);
-            massive = 1;
-        }
-        key[i] = k;
-        perm[i] = (unsigned) i;
+    if (i >= n) return;
+    const double ks = scal->key_scale;
+    unsigned long long k = ~0ull;
+    if (m[i] != 0.0) {
+        unsigned long long ux = nbh_fixed(px[i], ks);
+        unsigned long long uy = nbh_fixed(py[i], ks);
+        unsigned long long uz = nbh_fixed(pz[i], ks);
+        k = nbh_morton21(ux >> 42, uy >> 42, uz >> 42);
     }
-    massive = nbh_wave_sum_u32(massive);
-    if (__lane_id() == 0 && massive) atomicAdd(&scal->nmassive, (int) massive);
+    key[i] = k;
+    perm[i] = (unsigned) i;
+}
+
+/* Massive keys are < 2^63, massless keys all-ones: after the sort the
+ * massive bodies are the prefix [0, M). */
+__global__ __launch_bounds__(256) void k_count_massive(int n,
+                                                       const unsigned long long* __restrict__ key,
+                                                       DevScalars* __restrict__ scal)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const bool massive = key[i] != ~0ull;
+    if (i == 0 && !massive) scal->nmassive = 0;
+    if (massive && (i + 1 == n || key[i + 1] == ~0ull)) scal->nmassive = i + 1;
 }
 
 /* ---------------------------------------------------------------------- */
@@ -139,20 +147,19 @@ __global__ __launch_bounds__(256) void k_radix_hist(int n, int shift,
     hist[threadIdx.x * nb + blockIdx.x] = cnt[threadIdx.x];
 }
 
-/* Exclusive scan of the digit-major histogram (256 * nb entries), one block. */
-__global__ __launch_bounds__(1024) void k_radix_scan(unsigned* __restrict__ hist, int nb)
+/* Exclusive scan of each digit's row of the digit-major histogram (block d
+ * owns digit d); the row total goes to dtot[d]. */
+__global__ __launch_bounds__(256) void k_radix_scan(unsigned* __restrict__ hist,
+                                                    unsigned* __restrict__ dtot, int nb)
 {
-    __shared__ unsigned wsum[16];
-    const int total = 256 * nb;
-    const int chunk = (total + 1023) / 1024;
+    __shared__ unsigned wsum[4];
+    unsigned* row = hist + (size_t) blockIdx.x * nb;
     const int t = threadIdx.x;
+    const int chunk = (nb + 255) / 256;
     const int b0 = t * chunk;
     unsigned s = 0;
-    for (int k = 0; k < chunk; ++k) {
-        int idx = b0 + k;
-        if (idx < total) s += hist[idx];
-    }
-    /* block exclusive scan of s */
+    for (int k = 0; k < chunk; ++k)
+        if (b0 + k < nb) s += row[b0 + k];
     unsigned v = s;
     for (int off = 1; off < 64; off <<= 1) {
         unsigned u = __shfl_up(v, off, 64);
@@ -160,24 +167,17 @@ __global__ __launch_bounds__(1024) void k_radix_scan(unsigned* __restrict__ hist
     }
     if (__lane_id() == 63) wsum[t >> 6] = v;
     __syncthreads();
-    if (t < 16) {
-        unsigned w = wsum[t];
-        for (int off = 1; off < 16; off <<= 1) {
-            unsigned u = __shfl_up(w, off, 16);
-            if (t >= off) w += u;
-        }
-        wsum[t] = w;
-    }
-    __syncthreads();
-    unsigned excl = v - s + ((t >> 6) ? wsum[(t >> 6) - 1] : 0u);
+    unsigned before = 0;
+    for (int w = 0; w < (t >> 6); ++w) before += wsum[w];
+    unsigned excl = v - s + before;
     for (int k = 0; k < chunk; ++k) {
-        int idx = b0 + k;
-        if (idx < total) {
-            unsigned h = hist[idx];
-            hist[idx] = excl;
+        if (b0 + k < nb) {
+            unsigned h = row[b0 + k];
+            row[b0 + k] = excl;
             excl += h;
         }
     }
+    if (t == 255) dtot[blockIdx.x] = v + before;
 }
 
 __global__ __launch_bounds__(256) void k_radix_scatter(int n, int shift,
@@ -185,14 +185,29 @@ __global__ __launch_bounds__(256) void k_radix_scatter(int n, int shift,
                                                        const unsigned* __restrict__ vin,
                                                        unsigned long long* __restrict__ kout,
                                                        unsigned* __restrict__ vout,
-                                                       const unsigned* __restrict__ hist, int nb)
+                                                       const unsigned* __restrict__ hist,
+                                                       const unsigned* __restrict__ dtot, int nb)
 {
     __shared__ unsigned goff[256];
     __shared__ unsigned running[256];
     __shared__ unsigned wcnt[4][256];
+    __shared__ unsigned wsum[4];
     const int t = threadIdx.x;
     const int w = t >> 6;
-    goff[t] = hist[t * nb + blockIdx.x];
+    /* digit bases: exclusive scan of the 256 digit totals */
+    {
+        const unsigned s0 = dtot[t];
+        unsigned v = s0;
+        for (int off = 1; off < 64; off <<= 1) {
+            unsigned u = __shfl_up(v, off, 64);
+            if ((int) __lane_id() >= off) v += u;
+        }
+        if (__lane_id() == 63) wsum[w] = v;
+        __syncthreads();
+        unsigned before = 0;
+        for (int ww = 0; ww < w; ++ww) before += wsum[ww];
+        goff[t] = v - s0 + before + hist[t * nb + blockIdx.x];
+    }
     running[t] = 0;
     wcnt[0][t] = wcnt[1][t] = wcnt[2][t] = wcnt[3][t] = 0;
     __syncthreads();
@@ -615,7 +630,6 @@ __device__ __forceinline__ void nbh_emit_cells_body(
         nmid[3 * pre + 2] = mz;
         atomicAdd(&lh[l], 1);
     }
-    atomicMax(&scal->maxdepth, Lk);
 }
 
 __global__ __launch_bounds__(256) void k_emit_cells(const int* __restrict__ blo,
@@ -639,6 +653,13 @@ __global__ __launch_bounds__(256) void k_emit_cells(const int* __restrict__ blo,
     __syncthreads();
     if (threadIdx.x < 64 && lh[threadIdx.x]) atomicAdd(&scal->level_count[threadIdx.x],
                                                        lh[threadIdx.x]);
+    /* deepest level of this block = highest non-empty bin */
+    if (threadIdx.x == 0) {
+        int d = -1;
+        for (int l = 63; l >= 0; --l)
+            if (lh[l]) { d = l; break; }
+        if (d >= 0) atomicMax(&scal->maxdepth, d);
+    }
 }
 
 __global__ __launch_bounds__(256) void k_emit_leaves(const int* __restrict__ blo,
@@ -844,13 +865,14 @@ void nbh_build(nbhip_state* st)
     for (int pass = 0; pass < 8; ++pass) {
         const int shift = 8 * pass;
         hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(256), 0, s, n, shift, kin, st->hist, nb);
-        hipLaunchKernelGGL(k_radix_scan, dim3(1), dim3(1024), 0, s, st->hist, nb);
+        hipLaunchKernelGGL(k_radix_scan, dim3(256), dim3(256), 0, s, st->hist, st->dtot, nb);
         hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(256), 0, s, n, shift, kin, vin, kout,
-                           vout, st->hist, nb);
+                           vout, st->hist, st->dtot, nb);
         unsigned long long* tk = kin; kin = kout; kout = tk;
         unsigned* tv = vin; vin = vout; vout = tv;
     }
     /* 8 passes: sorted data is back in st->key / st->perm */
+    hipLaunchKernelGGL(k_count_massive, dim3(g256), dim3(256), 0, s, n, st->key, st->scal);
     hipLaunchKernelGGL(k_tiefix, dim3(g256), dim3(256), 0, s, st->key, st->perm, st->px, st->py,
                        st->pz, st->scal);
     nbh_mark(st, 2);

@@ -96,6 +96,7 @@ __global__ __launch_bounds__(256) void k_walk64(const NodeW* __restrict__ node,
     bool met = false;
     unsigned nbb = 0, nbc = 0, nop = 0, steps = 0;
     const double eps2 = o.eps2;
+    const bool eps2_pos = eps2 > 0.0;
 
     /* The opening tests come first: once the wave's vote is known the
      * successor node is fetched, and its scalar-load latency overlaps the
@@ -121,7 +122,9 @@ __global__ __launch_bounds__(256) void k_walk64(const NodeW* __restrict__ node,
         }
         const NodeW nxt = node[min(qn, nn - 1)];
         if (accept) {
-            if (nd.body != self) {
+            /* with eps2 > 0 the self term adds exactly +0 (dx = dy = dz = 0),
+             * so lanes need not branch around it */
+            if (eps2_pos || nd.body != self) {
                 drSq += eps2;
                 const double drab = sqrt(drSq);
                 const double phii = nd.m / drab;
@@ -147,11 +150,10 @@ __global__ __launch_bounds__(256) void k_walk64(const NodeW* __restrict__ node,
                     az -= dr5inv * qz;
                 }
                 if (COUNT) {
-                    if (nd.body >= 0) ++nbb; else ++nbc;
+                    if (nd.body >= 0) nbb += nd.body != self; else ++nbc;
                 }
-            } else {
-                met = true;
             }
+            met |= nd.body == self;
             lp = nd.next;
         } else if (open) {
             lp = q + 1;
@@ -210,6 +212,7 @@ __global__ __launch_bounds__(256) void k_walk32(const NodeF* __restrict__ node,
     const int self = (valid && s < M) ? s : -2;
     int lp = valid ? 0 : nn;
     const float eps2 = (float) o.eps2;
+    const bool eps2_pos = o.eps2 > 0.0;
     float sx = 0.0f, sy = 0.0f, sz = 0.0f;
     double ax = 0.0, ay = 0.0, az = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
     bool met = false;
@@ -236,7 +239,7 @@ __global__ __launch_bounds__(256) void k_walk32(const NodeF* __restrict__ node,
         }
         const NodeF nxt = node[min(qn, nn - 1)];
         if (accept) {
-            if (nd.body != self) {
+            if (eps2_pos || nd.body != self) {
                 drSq += eps2;
                 const float rinv = __builtin_amdgcn_rsqf(drSq);
                 const float rinv2 = rinv * rinv;
@@ -257,11 +260,10 @@ __global__ __launch_bounds__(256) void k_walk32(const NodeF* __restrict__ node,
                 sy += fy;
                 sz += fz;
                 if (COUNT) {
-                    if (nd.body >= 0) ++nbb; else ++nbc;
+                    if (nd.body >= 0) nbb += nd.body != self; else ++nbc;
                 }
-            } else {
-                met = true;
             }
+            met |= nd.body == self;
             lp = nd.next;
         } else if (open) {
             lp = q + 1;
