@@ -28,10 +28,10 @@ struct __attribute__((aligned(16))) NodeW {
 };
 
 /* Walk record of the mixed-precision walk (64 B, one scalar load).
- * Position = origin + (hx + lx, hy + ly, hz + lz): the FP64 offset from the
- * root centre of mass split into two floats, so that a float difference of
- * two records keeps ~1e-7 relative accuracy at any distance from the
- * origin.  q[6] is the quadrupole in FP32. */
+ * Position = root centre of mass + (hx + lx, hy + ly, hz + lz): the FP64
+ * offset split into two floats, so that a float difference of two records
+ * keeps ~1e-7 relative accuracy at any distance from the origin.  rcrit2 is
+ * -1 for bodies (always accepted); q[6] is the quadrupole in FP32. */
 struct __attribute__((aligned(16))) NodeF {
     float hx, hy, hz, m;
     float lx, ly, lz, rcrit2;

@@ -106,6 +106,7 @@ __global__ __launch_bounds__(256) void k_walk64(const NodeW* __restrict__ node,
     NodeW nd = node[0];
     while (q < nn) {
         if (COUNT) ++steps;
+        const int nd_next = __builtin_amdgcn_readfirstlane(nd.next);
         const bool active = lp == q;
         const double dx = nd.x - p.x;
         const double dy = nd.y - p.y;
@@ -113,8 +114,8 @@ __global__ __launch_bounds__(256) void k_walk64(const NodeW* __restrict__ node,
         double drSq = dz * dz + (dy * dy + dx * dx);
         const bool accept = active && (nd.body >= 0 || drSq >= nd.rcrit2);
         const bool open = active && !accept;
-        const bool any_open = __ballot(open) != 0ull;
-        const int qn = __builtin_amdgcn_readfirstlane(any_open ? q + 1 : nd.next);
+        const bool any_open = __builtin_amdgcn_ballot_w64(open) != 0ull;
+        const int qn = any_open ? q + 1 : nd_next;
         if (qn <= q) {
             /* a Next() link that does not advance: corrupt table, stop */
             bad = true;
@@ -154,7 +155,7 @@ __global__ __launch_bounds__(256) void k_walk64(const NodeW* __restrict__ node,
                 }
             }
             met |= nd.body == self;
-            lp = nd.next;
+            lp = nd_next;
         } else if (open) {
             lp = q + 1;
             if (COUNT) ++nop;
@@ -224,15 +225,16 @@ __global__ __launch_bounds__(256) void k_walk32(const NodeF* __restrict__ node,
     NodeF nd = node[0];
     while (q < nn) {
         if (COUNT) ++steps;
+        const int nd_next = __builtin_amdgcn_readfirstlane(nd.next);
         const bool active = lp == q;
         const float dx = (nd.hx - phx) + (nd.lx - plx);
         const float dy = (nd.hy - phy) + (nd.ly - ply);
         const float dz = (nd.hz - phz) + (nd.lz - plz);
         float drSq = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-        const bool accept = active && (nd.body >= 0 || drSq >= nd.rcrit2);
+        const bool accept = active && drSq >= nd.rcrit2;
         const bool open = active && !accept;
-        const bool any_open = __ballot(open) != 0ull;
-        const int qn = __builtin_amdgcn_readfirstlane(any_open ? q + 1 : nd.next);
+        const bool any_open = __builtin_amdgcn_ballot_w64(open) != 0ull;
+        const int qn = any_open ? q + 1 : nd_next;
         if (qn <= q) {
             bad = true;
             break;
@@ -244,32 +246,34 @@ __global__ __launch_bounds__(256) void k_walk32(const NodeF* __restrict__ node,
                 const float rinv = __builtin_amdgcn_rsqf(drSq);
                 const float rinv2 = rinv * rinv;
                 const float mor3 = nd.m * rinv * rinv2;
-                float fx = mor3 * dx, fy = mor3 * dy, fz = mor3 * dz;
                 if (QUAD && nd.body < 0) {
+                    /* acc += (mor3 + 2.5 dQd / r^7) d - Qd / r^5 */
                     const float qx = __builtin_fmaf(nd.q[0], dx, __builtin_fmaf(nd.q[1], dy, nd.q[2] * dz));
                     const float qy = __builtin_fmaf(nd.q[1], dx, __builtin_fmaf(nd.q[3], dy, nd.q[4] * dz));
                     const float qz = __builtin_fmaf(nd.q[2], dx, __builtin_fmaf(nd.q[4], dy, nd.q[5] * dz));
-                    const float drQdr = __builtin_fmaf(qx, dx, __builtin_fmaf(qy, dy, qz * dz));
-                    const float dr5inv = rinv2 * rinv2 * rinv;
-                    const float phiQ = 2.5f * dr5inv * drQdr * rinv2;
-                    fx = __builtin_fmaf(phiQ, dx, __builtin_fmaf(-dr5inv, qx, fx));
-                    fy = __builtin_fmaf(phiQ, dy, __builtin_fmaf(-dr5inv, qy, fy));
-                    fz = __builtin_fmaf(phiQ, dz, __builtin_fmaf(-dr5inv, qz, fz));
+                    const float dQd = __builtin_fmaf(qx, dx, __builtin_fmaf(qy, dy, qz * dz));
+                    const float dr5 = rinv2 * rinv2 * rinv;
+                    const float s1 = __builtin_fmaf(dr5 * 2.5f, dQd * rinv2, mor3);
+                    sx = __builtin_fmaf(s1, dx, __builtin_fmaf(-dr5, qx, sx));
+                    sy = __builtin_fmaf(s1, dy, __builtin_fmaf(-dr5, qy, sy));
+                    sz = __builtin_fmaf(s1, dz, __builtin_fmaf(-dr5, qz, sz));
+                } else {
+                    sx = __builtin_fmaf(mor3, dx, sx);
+                    sy = __builtin_fmaf(mor3, dy, sy);
+                    sz = __builtin_fmaf(mor3, dz, sz);
                 }
-                sx += fx;
-                sy += fy;
-                sz += fz;
                 if (COUNT) {
                     if (nd.body >= 0) nbb += nd.body != self; else ++nbc;
                 }
             }
             met |= nd.body == self;
-            lp = nd.next;
+            lp = nd_next;
         } else if (open) {
             lp = q + 1;
             if (COUNT) ++nop;
         }
-        if (++batch == NBH_MIXED_BATCH) {
+        batch = __builtin_amdgcn_readfirstlane(batch + 1);
+        if (batch == NBH_MIXED_BATCH) {
             batch = 0;
             nbh_kahan_add(ax, cx, (double) sx);
             nbh_kahan_add(ay, cy, (double) sy);
@@ -290,8 +294,8 @@ __global__ __launch_bounds__(256) void k_walk32(const NodeF* __restrict__ node,
     nbh_walk_finish(o, p.x, p.y, p.z, ax, ay, az, perm[s], oax, oay, oaz, vx, vy, vz);
 }
 
-/* FP32 walk records from the FP64 tree, offsets from the root centre of
- * mass. */
+/* Mixed-precision walk records from the FP64 tree: offsets from the root
+ * centre of mass, split hi + lo. */
 __global__ __launch_bounds__(256) void k_pack32(const NodeW* __restrict__ node,
                                                 const double* __restrict__ nquad,
                                                 NodeF* __restrict__ nodef,
@@ -311,7 +315,7 @@ __global__ __launch_bounds__(256) void k_pack32(const NodeW* __restrict__ node,
     f.ly = (float) (dy - (double) f.hy);
     f.lz = (float) (dz - (double) f.hz);
     f.m = (float) w.m;
-    f.rcrit2 = (float) w.rcrit2;
+    f.rcrit2 = w.body >= 0 ? -1.0f : (float) w.rcrit2;
     f.next = w.next;
     f.body = w.body;
     if (w.body < 0) {
