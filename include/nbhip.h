@@ -159,6 +159,47 @@ unsigned nbhip_walk_counters(nbhip_state* st, unsigned long long* counts4);
 unsigned nbhip_set_profiling(nbhip_state* st, int enable);
 unsigned nbhip_phase_times(nbhip_state* st, double* ms5);
 
+/* ---- batches of independent simulations ----------------------------------
+ * Many dwarfs of uneven body count, as a likelihood search evaluates them,
+ * share one state: every phase of a step is one launch over all of them and
+ * each simulation's results are bit-identical to its own single run.  No
+ * reference counterpart: the reference runs one simulation per process
+ * (nbody/src/nbody.c:399-560); these calls serve the batched search.
+ * nbhip_gravity / nbhip_step / nbhip_steps / nbhip_run act on every
+ * simulation and return the OR of their statuses. */
+unsigned nbhip_batch_create(const nbhip_ctx* ctx, int device, int nsim, const int* nbody,
+                            const double* eps2, nbhip_state** out);
+unsigned nbhip_batch_upload(nbhip_state* st, int sim, const double* pos, const double* vel,
+                            const double* mass);
+unsigned nbhip_batch_download(nbhip_state* st, int sim, double* pos, double* vel,
+                              double* acc);
+/* Sticky status of each simulation (nsim entries). */
+unsigned nbhip_batch_status(nbhip_state* st, unsigned* status);
+unsigned nbhip_batch_tree_info(nbhip_state* st, int sim, nbhip_tree_info* info);
+
+/* enable != 0: nbhip_steps captures one step into a hipGraph on first use and
+ * replays it (no host synchronisation inside a step; the status is read
+ * once per call). */
+unsigned nbhip_set_graph(nbhip_state* st, int enable);
+
+/* Walk launch parameters: workgroup size (64, 128, 256 threads) and, for the
+ * mixed walk, the node block (a power of two, in pre-order node indices)
+ * over which a body's FP32 partial sum runs before the FP64 Kahan add.
+ * The block depends only on a body's own interaction list, so results do not
+ * change with the workgroup size or with batching. */
+typedef struct nbhip_tune {
+    int block;
+    int batch;
+    double ms;              /* measured walk time of this setting */
+} nbhip_tune;
+
+/* Times every setting on the current tree (call after nbhip_gravity), keeps
+ * the fastest for later steps and reports it in *best; trials (may be NULL)
+ * receives up to max_trials measured settings, *ntrials their count. */
+unsigned nbhip_autotune(nbhip_state* st, nbhip_tune* best, nbhip_tune* trials, int max_trials,
+                        int* ntrials);
+unsigned nbhip_set_tune(nbhip_state* st, const nbhip_tune* tune);
+
 /* Waits for all work queued by the library. */
 unsigned nbhip_sync(nbhip_state* st);
 

@@ -78,22 +78,45 @@ static hipError_t nbh_alloc(T** p, size_t count)
 static void nbh_free_all(nbhip_state* st)
 {
     void* ptrs[] = { st->px,   st->py,     st->pz,    st->vx,      st->vy,     st->vz,
-                     st->m,    st->ax,     st->ay,    st->az,      st->key,    st->key_alt,
-                     st->perm, st->perm_alt, st->hist, st->dtot, st->spos,   st->su,     st->dadj,
-                     st->cincl, st->scan_part, st->blo, st->bhi,   st->bdelta, st->bparent,
-                     st->node, st->nquad,  st->nmid,  st->nlevel,  st->nparent,
-                     st->level_list, st->nodef, st->scal };
+                     st->m,    st->ax,     st->ay,    st->az,      st->simid,  st->key,
+                     st->key_alt, st->perm, st->perm_alt, st->hist, st->dtot,  st->spos,
+                     st->simid_s, st->su,  st->dadj,  st->cincl,   st->scan_part, st->blo,
+                     st->bhi,  st->bdelta, st->bparent, st->node,  st->nquad,  st->nmid,
+                     st->nlevel, st->nparent, st->nsim_node, st->level_list, st->nodef,
+                     st->scal, st->sims };
     for (void* p : ptrs)
         if (p) (void) hipFree(p);
+    if (st->graph_exec) (void) hipGraphExecDestroy(st->graph_exec);
+    if (st->graph) (void) hipGraphDestroy(st->graph);
+    free(st->h_body_off);
+    free(st->h_body_n);
+    free(st->h_sim_status);
 }
 
-extern "C" unsigned nbhip_create(const nbhip_ctx* ctx, int device, int nbody, nbhip_state** out)
+static unsigned nbh_create(const nbhip_ctx* ctx, int device, int nsim, const int* nbody,
+                           const double* eps2, nbhip_state** out)
 {
-    if (!ctx || !out || nbody < 0) {
+    if (!ctx || !out || nsim < 1 || !nbody) {
         nbh_set_error("nbhip_create: bad arguments");
         return NBHIP_USER_ERROR;
     }
     *out = nullptr;
+    long long total = 0;
+    for (int k = 0; k < nsim; ++k) {
+        if (nbody[k] < 0) {
+            nbh_set_error("negative body count for simulation %d", k);
+            return NBHIP_USER_ERROR;
+        }
+        total += nbody[k];
+    }
+    if (total > (1ll << 24)) {
+        nbh_set_error("%lld bodies: at most 2^24 per state", total);
+        return NBHIP_USER_ERROR;
+    }
+    if (nsim > 4096) {
+        nbh_set_error("%d simulations: at most 4096 per state", nsim);
+        return NBHIP_USER_ERROR;
+    }
     unsigned rc = nbh_validate(ctx);
     if (rc) return rc;
     int ndev = 0;
@@ -110,44 +133,88 @@ extern "C" unsigned nbhip_create(const nbhip_ctx* ctx, int device, int nbody, nb
     nbhip_state* st = (nbhip_state*) calloc(1, sizeof(nbhip_state));
     st->ctx = *ctx;
     st->device = device;
-    st->n = nbody;
-    st->node_cap = 5 * nbody + 1024;
-    st->sort_blocks = nbody > 0 ? (nbody + NBH_SORT_TILE - 1) / NBH_SORT_TILE : 1;
-    const size_t n = (size_t) nbody;
+    st->n = (int) total;
+    st->nsim = nsim;
+    int sb = 0;
+    while ((1 << sb) < nsim) ++sb;
+    st->top_levels = (63 - sb) / 3;
+    st->sim_shift = nsim > 1 ? 3 * st->top_levels : 64;
+    st->node_cap = 5 * st->n + 1024 * nsim;
+    st->sort_blocks = st->n > 0 ? (st->n + NBH_SORT_TILE - 1) / NBH_SORT_TILE : 1;
+    st->tune.block = 256;
+    st->tune.batch = 1024;
+    st->h_body_off = (int*) calloc(nsim, sizeof(int));
+    st->h_body_n = (int*) calloc(nsim, sizeof(int));
+    st->h_sim_status = (unsigned*) calloc(nsim, sizeof(unsigned));
+    st->all_eps2_pos = 1;
+    std::vector<SimTab> tab(nsim);
+    std::vector<int> simid(st->n > 0 ? st->n : 1);
+    int off = 0;
+    for (int k = 0; k < nsim; ++k) {
+        SimTab& t = tab[k];
+        memset(&t, 0, sizeof(t));
+        t.rsize = ctx->tree_rsize;
+        t.key_scale = 9223372036854775808.0 / ctx->tree_rsize;
+        t.eps2 = eps2 ? eps2[k] : ctx->eps2;
+        t.body_off = off;
+        t.body_n = nbody[k];
+        t.first = t.root = t.end = -1;
+        if (!(t.eps2 > 0.0)) st->all_eps2_pos = 0;
+        st->h_body_off[k] = off;
+        st->h_body_n[k] = nbody[k];
+        for (int i = 0; i < nbody[k]; ++i) simid[off + i] = k;
+        off += nbody[k];
+    }
+    const size_t n = (size_t) st->n;
     const size_t nc = (size_t) st->node_cap;
     hipError_t e = hipSuccess;
 #define A(ptr, cnt) if (e == hipSuccess) e = nbh_alloc(&st->ptr, (cnt))
     A(px, n); A(py, n); A(pz, n); A(vx, n); A(vy, n); A(vz, n); A(m, n);
-    A(ax, n); A(ay, n); A(az, n);
+    A(ax, n); A(ay, n); A(az, n); A(simid, n);
     A(key, n); A(key_alt, n); A(perm, n); A(perm_alt, n);
     A(hist, 256 * (size_t) st->sort_blocks);
     A(dtot, 256);
-    A(spos, n); A(su, 3 * n); A(dadj, n); A(cincl, n);
+    A(spos, n); A(simid_s, n); A(su, 3 * n); A(dadj, n); A(cincl, n);
     A(scan_part, 4096);
     A(blo, n); A(bhi, n); A(bdelta, n); A(bparent, 2 * n);
     A(node, nc); A(nquad, 6 * nc); A(nmid, 3 * nc); A(nlevel, nc); A(nparent, nc);
-    A(level_list, nc);
+    A(nsim_node, nc); A(level_list, nc);
     if (ctx->precision == NBHIP_PRECISION_MIXED) A(nodef, nc);
     A(scal, 1);
+    A(sims, (size_t) nsim);
 #undef A
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&st->stream, hipStreamNonBlocking);
     for (int k = 0; k < 6 && e == hipSuccess; ++k) e = hipEventCreate(&st->ev[k]);
     if (e != hipSuccess) {
-        nbh_set_error("allocation for %d bodies failed: %s", nbody, hipGetErrorString(e));
+        nbh_set_error("allocation for %d bodies failed: %s", st->n, hipGetErrorString(e));
         nbh_free_all(st);
         free(st);
         return NBHIP_GPU_ERROR;
     }
     DevScalars s0;
     memset(&s0, 0, sizeof(s0));
-    s0.rsize = ctx->tree_rsize;
-    s0.key_scale = 9223372036854775808.0 / ctx->tree_rsize;
+    s0.sim_bits = sb;
     NBH_CHECK(hipMemcpy(st->scal, &s0, sizeof(s0), hipMemcpyHostToDevice));
-    NBH_CHECK(hipMemset(st->ax, 0, n * sizeof(double) + (n ? 0 : 8)));
-    NBH_CHECK(hipMemset(st->ay, 0, n * sizeof(double) + (n ? 0 : 8)));
-    NBH_CHECK(hipMemset(st->az, 0, n * sizeof(double) + (n ? 0 : 8)));
+    NBH_CHECK(hipMemcpy(st->sims, tab.data(), nsim * sizeof(SimTab), hipMemcpyHostToDevice));
+    if (n) {
+        NBH_CHECK(hipMemcpy(st->simid, simid.data(), n * sizeof(int), hipMemcpyHostToDevice));
+        NBH_CHECK(hipMemset(st->ax, 0, n * sizeof(double)));
+        NBH_CHECK(hipMemset(st->ay, 0, n * sizeof(double)));
+        NBH_CHECK(hipMemset(st->az, 0, n * sizeof(double)));
+    }
     *out = st;
     return NBHIP_SUCCESS;
+}
+
+extern "C" unsigned nbhip_create(const nbhip_ctx* ctx, int device, int nbody, nbhip_state** out)
+{
+    return nbh_create(ctx, device, 1, &nbody, nullptr, out);
+}
+
+extern "C" unsigned nbhip_batch_create(const nbhip_ctx* ctx, int device, int nsim,
+                                       const int* nbody, const double* eps2, nbhip_state** out)
+{
+    return nbh_create(ctx, device, nsim, nbody, eps2, out);
 }
 
 extern "C" void nbhip_release(nbhip_state* st)
@@ -162,15 +229,11 @@ extern "C" void nbhip_release(nbhip_state* st)
     free(st);
 }
 
-extern "C" unsigned nbhip_upload(nbhip_state* st, const double* pos, const double* vel,
-                                 const double* mass)
+static unsigned nbh_upload_range(nbhip_state* st, int off, int cnt, const double* pos,
+                                 const double* vel, const double* mass)
 {
-    if (!st || (st->n > 0 && (!pos || !vel || !mass))) {
-        nbh_set_error("nbhip_upload: bad arguments");
-        return NBHIP_USER_ERROR;
-    }
     NBH_CHECK(hipSetDevice(st->device));
-    const size_t n = (size_t) st->n;
+    const size_t n = (size_t) cnt;
     if (n == 0) return NBHIP_SUCCESS;
     std::vector<double> tmp(n);
     double* dst[6] = { st->px, st->py, st->pz, st->vx, st->vy, st->vz };
@@ -178,24 +241,22 @@ extern "C" unsigned nbhip_upload(nbhip_state* st, const double* pos, const doubl
         const double* src = k < 3 ? pos : vel;
         const int d = k % 3;
         for (size_t i = 0; i < n; ++i) tmp[i] = src[3 * i + d];
-        NBH_CHECK(hipMemcpyAsync(dst[k], tmp.data(), n * sizeof(double), hipMemcpyHostToDevice,
-                                 st->stream));
+        NBH_CHECK(hipMemcpyAsync(dst[k] + off, tmp.data(), n * sizeof(double),
+                                 hipMemcpyHostToDevice, st->stream));
         NBH_CHECK(hipStreamSynchronize(st->stream));
     }
-    NBH_CHECK(hipMemcpyAsync(st->m, mass, n * sizeof(double), hipMemcpyHostToDevice, st->stream));
+    NBH_CHECK(hipMemcpyAsync(st->m + off, mass, n * sizeof(double), hipMemcpyHostToDevice,
+                             st->stream));
     NBH_CHECK(hipStreamSynchronize(st->stream));
     st->have_tree = 0;
     return NBHIP_SUCCESS;
 }
 
-extern "C" unsigned nbhip_download(nbhip_state* st, double* pos, double* vel, double* acc)
+static unsigned nbh_download_range(nbhip_state* st, int off, int cnt, double* pos, double* vel,
+                                   double* acc)
 {
-    if (!st) {
-        nbh_set_error("nbhip_download: null state");
-        return NBHIP_USER_ERROR;
-    }
     NBH_CHECK(hipSetDevice(st->device));
-    const size_t n = (size_t) st->n;
+    const size_t n = (size_t) cnt;
     if (n == 0) return NBHIP_SUCCESS;
     std::vector<double> tmp(n);
     double* outs[3] = { pos, vel, acc };
@@ -205,7 +266,7 @@ extern "C" unsigned nbhip_download(nbhip_state* st, double* pos, double* vel, do
     for (int o = 0; o < 3; ++o) {
         if (!outs[o]) continue;
         for (int d = 0; d < 3; ++d) {
-            NBH_CHECK(hipMemcpyAsync(tmp.data(), srcs[o][d], n * sizeof(double),
+            NBH_CHECK(hipMemcpyAsync(tmp.data(), srcs[o][d] + off, n * sizeof(double),
                                      hipMemcpyDeviceToHost, st->stream));
             NBH_CHECK(hipStreamSynchronize(st->stream));
             for (size_t i = 0; i < n; ++i) outs[o][3 * i + d] = tmp[i];
@@ -214,22 +275,81 @@ extern "C" unsigned nbhip_download(nbhip_state* st, double* pos, double* vel, do
     return NBHIP_SUCCESS;
 }
 
-/* Read back the device checks and fold them into the sticky status,
- * mapping a missed self-interaction onto the incest codes
- * (nbIncestStatusCheck, nbody/src/nbody_grav.c:295-303). */
+extern "C" unsigned nbhip_upload(nbhip_state* st, const double* pos, const double* vel,
+                                 const double* mass)
+{
+    if (!st || (st->n > 0 && (!pos || !vel || !mass))) {
+        nbh_set_error("nbhip_upload: bad arguments");
+        return NBHIP_USER_ERROR;
+    }
+    return nbh_upload_range(st, 0, st->n, pos, vel, mass);
+}
+
+extern "C" unsigned nbhip_download(nbhip_state* st, double* pos, double* vel, double* acc)
+{
+    if (!st) {
+        nbh_set_error("nbhip_download: null state");
+        return NBHIP_USER_ERROR;
+    }
+    return nbh_download_range(st, 0, st->n, pos, vel, acc);
+}
+
+extern "C" unsigned nbhip_batch_upload(nbhip_state* st, int sim, const double* pos,
+                                       const double* vel, const double* mass)
+{
+    if (!st || sim < 0 || sim >= st->nsim ||
+        (st->h_body_n[sim] > 0 && (!pos || !vel || !mass))) {
+        nbh_set_error("nbhip_batch_upload: bad arguments");
+        return NBHIP_USER_ERROR;
+    }
+    return nbh_upload_range(st, st->h_body_off[sim], st->h_body_n[sim], pos, vel, mass);
+}
+
+extern "C" unsigned nbhip_batch_download(nbhip_state* st, int sim, double* pos, double* vel,
+                                         double* acc)
+{
+    if (!st || sim < 0 || sim >= st->nsim) {
+        nbh_set_error("nbhip_batch_download: bad arguments");
+        return NBHIP_USER_ERROR;
+    }
+    return nbh_download_range(st, st->h_body_off[sim], st->h_body_n[sim], pos, vel, acc);
+}
+
+/* Read back the device checks and fold them into the sticky statuses, mapping
+ * a missed self-interaction onto the incest codes (nbIncestStatusCheck,
+ * nbody/src/nbody_grav.c:295-303). */
 static unsigned nbh_collect(nbhip_state* st)
 {
     DevScalars s;
+    std::vector<SimTab> tab(st->nsim);
     NBH_CHECK(hipMemcpyAsync(&s, st->scal, sizeof(s), hipMemcpyDeviceToHost, st->stream));
+    NBH_CHECK(hipMemcpyAsync(tab.data(), st->sims, st->nsim * sizeof(SimTab),
+                             hipMemcpyDeviceToHost, st->stream));
     NBH_CHECK(hipStreamSynchronize(st->stream));
-    unsigned rc = st->status | s.status;
-    if (s.incest)
-        rc |= st->ctx.allow_incest ? NBHIP_TREE_INCEST_NONFATAL : NBHIP_TREE_INCEST_FATAL;
-    st->status = rc;
+    /* st->status keeps what concerns the whole state (node capacity, launch
+     * errors); h_sim_status[k] what one simulation's tree or walk reported. */
+    st->status |= s.status;
+    unsigned rc = st->status;
+    for (int k = 0; k < st->nsim; ++k) {
+        unsigned f = tab[k].flags;
+        if (tab[k].incest)
+            f |= st->ctx.allow_incest ? NBHIP_TREE_INCEST_NONFATAL : NBHIP_TREE_INCEST_FATAL;
+        st->h_sim_status[k] |= f;
+        rc |= st->h_sim_status[k];
+    }
     if (rc & NBHIP_CELL_OVERFLOW_ERROR)
         nbh_set_error("node table overflow: %d nodes for capacity %d", s.nnode, st->node_cap);
     else if (rc & NBHIP_TREE_STRUCTURE_ERROR)
-        nbh_set_error("tree structure error (max depth %d, rsize %.17g)", s.maxdepth, s.rsize);
+        nbh_set_error("tree structure error (max depth %d)", s.maxdepth);
+    else if (rc & NBHIP_ERROR)
+        nbh_set_error("internal consistency error in the tree or walk");
+    return rc;
+}
+
+static unsigned nbh_status_all(const nbhip_state* st)
+{
+    unsigned rc = st->status;
+    for (int k = 0; k < st->nsim; ++k) rc |= st->h_sim_status[k];
     return rc;
 }
 
@@ -252,9 +372,9 @@ static unsigned nbh_force(nbhip_state* st, int drift, int kick)
     } else {
         nbh_bbox(st, drift);
         nbh_mark(st, 1);
-        nbh_build(st);
+        nbh_build(st, 0);
     }
-    nbh_walk(st, kick, 0, st->ax, st->ay, st->az, 0);
+    nbh_walk(st, kick, 0, st->ax, st->ay, st->az, 0, nullptr);
     nbh_mark(st, 5);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
@@ -299,16 +419,59 @@ extern "C" unsigned nbhip_step(nbhip_state* st)
     return rc;
 }
 
+/* One KDK step as a captured hipGraph: no host synchronisation inside. */
+static unsigned nbh_capture_step(nbhip_state* st)
+{
+    if (st->graph_exec) (void) hipGraphExecDestroy(st->graph_exec);
+    if (st->graph) (void) hipGraphDestroy(st->graph);
+    st->graph_exec = nullptr;
+    st->graph = nullptr;
+    const int prof = st->profiling;
+    st->profiling = 0;
+    NBH_CHECK(hipStreamBeginCapture(st->stream, hipStreamCaptureModeThreadLocal));
+    nbh_bbox(st, 1);
+    if (st->ctx.criterion != NBHIP_EXACT) nbh_build(st, 1);
+    nbh_walk(st, 1, 0, st->ax, st->ay, st->az, 0, nullptr);
+    hipError_t e = hipStreamEndCapture(st->stream, &st->graph);
+    st->profiling = prof;
+    if (e != hipSuccess) {
+        nbh_set_error("graph capture failed: %s", hipGetErrorString(e));
+        return NBHIP_GPU_ERROR;
+    }
+    NBH_CHECK(hipGraphInstantiate(&st->graph_exec, st->graph, nullptr, nullptr, 0));
+    st->graph_ready = 1;
+    return NBHIP_SUCCESS;
+}
+
 extern "C" unsigned nbhip_steps(nbhip_state* st, unsigned nsteps)
 {
     if (!st) return NBHIP_USER_ERROR;
-    unsigned rc = st->status;
+    unsigned rc = nbh_status_all(st);
     if (NBHIP_STATUS_IS_FATAL(rc)) return rc;
+    if (st->use_graph && nsteps > 0) {
+        NBH_CHECK(hipSetDevice(st->device));
+        if (!st->graph_ready) {
+            rc = nbh_capture_step(st);
+            if (rc) return rc;
+        }
+        for (unsigned k = 0; k < nsteps; ++k) NBH_CHECK(hipGraphLaunch(st->graph_exec, st->stream));
+        st->step += nsteps;
+        st->have_tree = st->ctx.criterion != NBHIP_EXACT;
+        return nbh_collect(st);
+    }
     for (unsigned k = 0; k < nsteps; ++k) {
         rc |= nbhip_step(st);
         if (NBHIP_STATUS_IS_FATAL(rc)) return rc;
     }
     return rc;
+}
+
+extern "C" unsigned nbhip_set_graph(nbhip_state* st, int enable)
+{
+    if (!st) return NBHIP_USER_ERROR;
+    st->use_graph = enable ? 1 : 0;
+    st->graph_ready = 0;
+    return NBHIP_SUCCESS;
 }
 
 extern "C" unsigned nbhip_run(nbhip_state* st, unsigned nsteps)
@@ -327,20 +490,54 @@ extern "C" unsigned nbhip_sync(nbhip_state* st)
     return NBHIP_SUCCESS;
 }
 
+static unsigned nbh_info(nbhip_state* st, int sim, nbhip_tree_info* info)
+{
+    NBH_CHECK(hipSetDevice(st->device));
+    DevScalars s;
+    std::vector<SimTab> tab(st->nsim);
+    NBH_CHECK(hipMemcpyAsync(&s, st->scal, sizeof(s), hipMemcpyDeviceToHost, st->stream));
+    NBH_CHECK(hipMemcpyAsync(tab.data(), st->sims, st->nsim * sizeof(SimTab),
+                             hipMemcpyDeviceToHost, st->stream));
+    NBH_CHECK(hipStreamSynchronize(st->stream));
+    const SimTab& t = tab[sim];
+    info->rsize = t.rsize;
+    info->max_depth = t.maxdepth;
+    info->step = st->step;
+    info->status = st->h_sim_status[sim] | st->status;
+    if (st->nsim == 1) {
+        info->ncell = s.ncell;
+        info->nnode = s.nnode;
+        info->nmassive = s.nmassive;
+        return NBHIP_SUCCESS;
+    }
+    int next_first = s.nmassive;
+    for (int k = sim + 1; k < st->nsim; ++k)
+        if (tab[k].first >= 0) {
+            next_first = tab[k].first;
+            break;
+        }
+    info->nmassive = t.first >= 0 ? next_first - t.first : 0;
+    info->nnode = t.root >= 0 ? t.end - t.root : 0;
+    info->ncell = info->nnode - info->nmassive;
+    return NBHIP_SUCCESS;
+}
+
 extern "C" unsigned nbhip_tree_info_get(nbhip_state* st, nbhip_tree_info* info)
 {
     if (!st || !info) return NBHIP_USER_ERROR;
-    NBH_CHECK(hipSetDevice(st->device));
-    DevScalars s;
-    NBH_CHECK(hipMemcpyAsync(&s, st->scal, sizeof(s), hipMemcpyDeviceToHost, st->stream));
-    NBH_CHECK(hipStreamSynchronize(st->stream));
-    info->rsize = s.rsize;
-    info->ncell = s.ncell;
-    info->max_depth = s.maxdepth;
-    info->nnode = s.nnode;
-    info->nmassive = s.nmassive;
-    info->step = st->step;
-    info->status = st->status;
+    return nbh_info(st, 0, info);
+}
+
+extern "C" unsigned nbhip_batch_tree_info(nbhip_state* st, int sim, nbhip_tree_info* info)
+{
+    if (!st || !info || sim < 0 || sim >= st->nsim) return NBHIP_USER_ERROR;
+    return nbh_info(st, sim, info);
+}
+
+extern "C" unsigned nbhip_batch_status(nbhip_state* st, unsigned* status)
+{
+    if (!st || !status) return NBHIP_USER_ERROR;
+    for (int k = 0; k < st->nsim; ++k) status[k] = st->h_sim_status[k] | st->status;
     return NBHIP_SUCCESS;
 }
 
@@ -434,7 +631,7 @@ extern "C" unsigned nbhip_walk_accel(nbhip_state* st, double* acc)
     if (n == 0) return NBHIP_SUCCESS;
     double* t = nullptr;
     NBH_CHECK(hipMalloc(&t, 3 * n * sizeof(double)));
-    nbh_walk(st, 0, 0, t, t + n, t + 2 * n, 1);
+    nbh_walk(st, 0, 0, t, t + n, t + 2 * n, 1, nullptr);
     std::vector<double> h(3 * n);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess)
@@ -465,7 +662,7 @@ extern "C" unsigned nbhip_walk_counters(nbhip_state* st, unsigned long long* cou
     hipError_t e = hipMemcpyAsync(&st->scal->counters[0], zero, sizeof(zero),
                                   hipMemcpyHostToDevice, st->stream);
     if (e == hipSuccess) {
-        nbh_walk(st, 0, 1, t, t + n, t + 2 * n, 1);
+        nbh_walk(st, 0, 1, t, t + n, t + 2 * n, 1, nullptr);
         e = hipGetLastError();
     }
     if (e == hipSuccess)
@@ -491,5 +688,81 @@ extern "C" unsigned nbhip_phase_times(nbhip_state* st, double* ms5)
 {
     if (!st || !ms5) return NBHIP_USER_ERROR;
     for (int k = 0; k < 5; ++k) ms5[k] = st->phase_ms[k];
+    return NBHIP_SUCCESS;
+}
+
+extern "C" unsigned nbhip_set_tune(nbhip_state* st, const nbhip_tune* tune)
+{
+    if (!st || !tune) return NBHIP_USER_ERROR;
+    if (tune->block != 64 && tune->block != 128 && tune->block != 256) return NBHIP_USER_ERROR;
+    if (tune->batch < 1 || (tune->batch & (tune->batch - 1))) {
+        nbh_set_error("nbhip_set_tune: batch must be a power of two");
+        return NBHIP_USER_ERROR;
+    }
+    st->tune.block = tune->block;
+    st->tune.batch = tune->batch;
+    st->graph_ready = 0;
+    return NBHIP_SUCCESS;
+}
+
+/* Project-written autotuner: each candidate walk setting runs three times
+ * on the current tree (outputs to scratch), timed with events on the
+ * library's stream; the median decides. */
+extern "C" unsigned nbhip_autotune(nbhip_state* st, nbhip_tune* best, nbhip_tune* trials,
+                                   int max_trials, int* ntrials)
+{
+    if (!st) return NBHIP_USER_ERROR;
+    if (!st->have_tree) {
+        nbh_set_error("no tree: call nbhip_gravity first");
+        return NBHIP_USER_ERROR;
+    }
+    NBH_CHECK(hipSetDevice(st->device));
+    const size_t n = (size_t) (st->n > 0 ? st->n : 1);
+    double* t = nullptr;
+    NBH_CHECK(hipMalloc(&t, 3 * n * sizeof(double)));
+    const int blocks[3] = { 64, 128, 256 };
+    const int batches_mixed[4] = { 256, 1024, 4096, 16384 };
+    const bool mixed = st->ctx.precision == NBHIP_PRECISION_MIXED;
+    const int nb = mixed ? 4 : 1;
+    nbhip_tune bestt = { st->tune.block, st->tune.batch, 1e300 };
+    int cnt = 0;
+    hipError_t e = hipSuccess;
+    for (int bi = 0; bi < 3 && e == hipSuccess; ++bi) {
+        for (int ci = 0; ci < nb && e == hipSuccess; ++ci) {
+            WalkTune w = { blocks[bi], mixed ? batches_mixed[ci] : st->tune.batch };
+            float ms[3];
+            for (int r = 0; r < 3 && e == hipSuccess; ++r) {
+                e = hipEventRecord(st->ev[0], st->stream);
+                nbh_walk(st, 0, 0, t, t + n, t + 2 * n, 0, &w);
+                if (e == hipSuccess) e = hipEventRecord(st->ev[1], st->stream);
+                if (e == hipSuccess) e = hipEventSynchronize(st->ev[1]);
+                if (e == hipSuccess) e = hipEventElapsedTime(&ms[r], st->ev[0], st->ev[1]);
+            }
+            if (e != hipSuccess) break;
+            float lo = fminf(ms[0], fminf(ms[1], ms[2])), hi = fmaxf(ms[0], fmaxf(ms[1], ms[2]));
+            const double med = (double) (ms[0] + ms[1] + ms[2] - lo - hi);
+            if (trials && cnt < max_trials) {
+                trials[cnt].block = w.block;
+                trials[cnt].batch = w.batch;
+                trials[cnt].ms = med;
+            }
+            ++cnt;
+            if (med < bestt.ms) {
+                bestt.block = w.block;
+                bestt.batch = w.batch;
+                bestt.ms = med;
+            }
+        }
+    }
+    (void) hipFree(t);
+    if (e != hipSuccess) {
+        nbh_set_error("autotune failed: %s", hipGetErrorString(e));
+        return NBHIP_GPU_ERROR;
+    }
+    st->tune.block = bestt.block;
+    st->tune.batch = bestt.batch;
+    st->graph_ready = 0;
+    if (best) *best = bestt;
+    if (ntrials) *ntrials = cnt;
     return NBHIP_SUCCESS;
 }

@@ -64,10 +64,12 @@ __device__ __forceinline__ int nbh_prefix_bits(unsigned long long ax, unsigned l
     return 3 * m + (ex == m ? 0 : (ey == m ? 1 : 2));
 }
 
-/* Number of octree levels shared by a prefix of `bits` bits; -1 for none. */
-__device__ __forceinline__ int nbh_levels(int bits)
+/* Number of octree levels shared by a key prefix of `bits` bits when the
+ * top `sb` key bits hold the simulation id; -1 for none (different
+ * simulations, or outside the array). */
+__device__ __forceinline__ int nbh_levels(int bits, int sb)
 {
-    return bits < 0 ? -1 : min(bits, 189) / 3;
+    return bits < sb ? -1 : min(bits - sb, 189) / 3;
 }
 
 /* ---------------------------------------------------------------------- */

@@ -39,21 +39,39 @@ struct __attribute__((aligned(16))) NodeF {
     float q[6];
 };
 
-/* Device-resident scalars of one simulation. */
-struct DevScalars {
+/* Per-simulation device record (one per dwarf of a batch; one for a single
+ * simulation). */
+struct SimTab {
     double rsize;                  /* st->tree.rsize, persists, grows only */
     double key_scale;              /* 2^63 / rsize */
+    double eps2;
     unsigned long long xyzmax_bits;/* max |x_i| of the last bbox pass */
-    int nmassive;                  /* bodies with mass != 0 */
+    int body_off, body_n;          /* bodies, original order */
+    int first;                     /* first sorted massive body, -1 if none */
+    int root, end;                 /* pre-order segment [root, end), -1 if none */
+    int maxdepth;
+    unsigned flags;                /* sticky NBHIP_* bits of this simulation */
+    unsigned incest;               /* a body of this simulation missed itself */
+};
+
+/* Device-resident scalars shared by all simulations of a state. */
+struct DevScalars {
+    int nmassive;                  /* massive bodies, all simulations */
     int ncell;
     int nnode;
     int maxdepth;
-    unsigned status;               /* sticky NBHIP_* bits of device checks */
-    unsigned incest;               /* a body missed itself in the walk */
+    unsigned status;               /* sticky global bits (overflow, internal error) */
+    int sim_bits;                  /* key bits holding the simulation id */
     unsigned long long counters[4];/* body-body, body-cell, opened, wave node steps */
     int level_count[64];           /* cells per level */
     int level_fill[64];
     int level_off[65];             /* exclusive scan of level_count */
+};
+
+/* Walk tuning picked by the autotuner (nbhip_autotune). */
+struct WalkTune {
+    int block;                     /* threads per workgroup: 64, 128 or 256 */
+    int batch;                     /* mixed walk: node block per FP32 partial sum (power of 2) */
 };
 
 #define NBH_SORT_TILE 4096   /* keys per radix-sort block: 256 threads x 16 */
@@ -62,12 +80,16 @@ struct DevScalars {
 struct nbhip_state {
     nbhip_ctx ctx;
     int device;
-    int n;
+    int n;                        /* bodies, all simulations */
+    int nsim;
+    int sim_shift;                /* key bit where the simulation id starts */
+    int top_levels;               /* Morton levels in the sort word */
     int node_cap;
     int sort_blocks;
     hipStream_t stream;
 
     double *px, *py, *pz, *vx, *vy, *vz, *m, *ax, *ay, *az;
+    int *simid;                   /* simulation of each body, original order */
 
     unsigned long long *key, *key_alt;
     unsigned *perm, *perm_alt;
@@ -75,6 +97,7 @@ struct nbhip_state {
     unsigned *dtot;               /* per-digit totals of one radix pass */
 
     double4 *spos;
+    int *simid_s;                 /* simulation of each sorted body */
     unsigned long long *su;       /* 3 * n: ux | uy | uz */
     int *dadj;
     unsigned *cincl;              /* inclusive scan of cells-per-first-body */
@@ -85,26 +108,36 @@ struct nbhip_state {
     NodeW *node;
     double *nquad;                /* 6 per node */
     double *nmid;                 /* 3 per node (cells) */
-    int *nlevel, *nparent;
+    int *nlevel, *nparent, *nsim_node;
     int *level_list;              /* cell pre-order indices grouped by level */
     NodeF *nodef;                 /* mixed-precision walk records (precision MIXED) */
 
     DevScalars *scal;
+    SimTab *sims;
+    int *h_body_off, *h_body_n;   /* host copies */
+    int all_eps2_pos;
 
     unsigned step;
-    unsigned status;              /* host-side sticky status */
+    unsigned status;              /* host-side sticky status (all simulations) */
+    unsigned *h_sim_status;       /* host-side sticky status per simulation */
     int have_tree;
     int profiling;
     hipEvent_t ev[6];
     float phase_ms[5];
+
+    WalkTune tune;
+    int use_graph;
+    hipGraph_t graph;
+    hipGraphExec_t graph_exec;
+    int graph_ready;
 };
 
 /* launchers (tree.hip) */
 void nbh_bbox(nbhip_state* st, int drift);
-void nbh_build(nbhip_state* st);
+void nbh_build(nbhip_state* st, int graph_safe);
 /* launchers (walk.hip) */
 void nbh_walk(nbhip_state* st, int kick, int count, double* out_ax, double* out_ay,
-              double* out_az, int self_only);
+              double* out_az, int self_only, const WalkTune* tune);
 void nbh_direct(nbhip_state* st, int nsample, const int* d_idx, double* d_acc);
 
 #define NBH_CHECK(x)                                                         \

@@ -1,6 +1,6 @@
 /*
  * Tree construction on gfx950: root-size update, Morton keys, LSD radix sort,
- * Karras binary radix tree, octree emission in pre-order and bottom-up
+ * Karras binary radix tree, octree emission in pre-order and level-by-level
  * summarisation (centre of mass, critical radius, quadrupole).
  *
  * The octree is exactly the reference's (nbody/src/nbody_tree.c): root at the
@@ -14,6 +14,12 @@
  * each sorted body a, its new cells by level, then the body itself; so
  * pre(body a) = a + #cells with first body <= a and Next(cell [lo,hi]) =
  * pre(body hi) + 1.
+ *
+ * Several independent simulations can share one state: the simulation id
+ * sits above the Morton bits of the sort key, prefixes across simulations
+ * are -1, so every phase is one launch for all simulations, the binary radix
+ * tree is a forest joined by prefix -1 nodes (which own no cell) and each
+ * simulation's pre-order table is one contiguous segment.
  */
 #include "nbhip_device.h"
 
@@ -23,7 +29,7 @@
 /* bounding box (+ optional opening half-kick and drift)                  */
 /* ---------------------------------------------------------------------- */
 /* advancePosVel (nbody/src/nbody_plain.c:268-284) fused with expandBox's
- * max-|x| scan (nbody/src/nbody_tree.c:156-176). */
+ * max-|x| scan (nbody/src/nbody_tree.c:156-176), per simulation. */
 __global__ __launch_bounds__(256) void k_drift_bbox(int n, int drift, double dt, double dtHalf,
                                                     double* __restrict__ px, double* __restrict__ py,
                                                     double* __restrict__ pz, double* __restrict__ vx,
@@ -31,11 +37,14 @@ __global__ __launch_bounds__(256) void k_drift_bbox(int n, int drift, double dt,
                                                     const double* __restrict__ ax,
                                                     const double* __restrict__ ay,
                                                     const double* __restrict__ az,
-                                                    DevScalars* __restrict__ scal)
+                                                    const int* __restrict__ simid,
+                                                    SimTab* __restrict__ sims)
 {
     const int i = blockIdx.x * 256 + threadIdx.x;
+    const bool valid = i < n;
     double mx = 0.0;
-    if (i < n) {
+    int sim = valid ? simid[i] : 0;
+    if (valid) {
         double x = px[i], y = py[i], z = pz[i];
         if (drift) {
             double ux = vx[i] + dtHalf * (ax[i] + 0.0);
@@ -51,63 +60,79 @@ __global__ __launch_bounds__(256) void k_drift_bbox(int n, int drift, double dt,
         mx = fmax(mx, fabs(y));
         mx = fmax(mx, fabs(z));
     }
-    mx = nbh_wave_max(mx);
-    __shared__ double wmax[4];
-    if (__lane_id() == 0) wmax[threadIdx.x >> 6] = mx;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double m = fmax(fmax(wmax[0], wmax[1]), fmax(wmax[2], wmax[3]));
-        atomicMax(&scal->xyzmax_bits, (unsigned long long) __double_as_longlong(m));
+    const int sim0 = __builtin_amdgcn_readfirstlane(sim);
+    if (!valid) sim = sim0;
+    if (__ballot(sim != sim0) == 0ull) {
+        mx = nbh_wave_max(mx);
+        if (__lane_id() == 0 && __ballot(valid) != 0ull)
+            atomicMax(&sims[sim0].xyzmax_bits, (unsigned long long) __double_as_longlong(mx));
+    } else if (valid) {
+        atomicMax(&sims[sim].xyzmax_bits, (unsigned long long) __double_as_longlong(mx));
     }
 }
 
-/* Root doubling rule (expandBox, nbody/src/nbody_tree.c:173-175) and per-build
- * resets. */
-__global__ void k_rsize(DevScalars* scal)
+/* Root doubling rule (expandBox, nbody/src/nbody_tree.c:173-175) per
+ * simulation, and per-build resets. */
+__global__ __launch_bounds__(256) void k_rsize(int nsim, SimTab* __restrict__ sims,
+                                               DevScalars* __restrict__ scal)
 {
-    double xm = __longlong_as_double((long long) scal->xyzmax_bits);
-    double r = scal->rsize;
-    if (!(xm <= DBL_MAX)) {
-        scal->status |= NBHIP_ERROR;
-    } else {
-        while (r < 2.0 * xm) r *= 2.0;
+    for (int k = threadIdx.x; k < nsim; k += 256) {
+        SimTab& t = sims[k];
+        const double xm = __longlong_as_double((long long) t.xyzmax_bits);
+        double r = t.rsize;
+        if (!(xm <= DBL_MAX)) {
+            t.flags |= NBHIP_ERROR;
+        } else {
+            while (r < 2.0 * xm) r *= 2.0;
+        }
+        t.rsize = r;
+        t.key_scale = 9223372036854775808.0 / r;
+        t.xyzmax_bits = 0ull;
+        t.first = -1;
+        t.root = -1;
+        t.end = -1;
+        t.maxdepth = 0;
     }
-    scal->rsize = r;
-    scal->key_scale = 9223372036854775808.0 / r;
-    scal->xyzmax_bits = 0ull;
-    scal->nmassive = 0;
-    scal->ncell = 0;
-    scal->nnode = 0;
-    scal->maxdepth = 0;
-    scal->counters[0] = scal->counters[1] = scal->counters[2] = scal->counters[3] = 0ull;
-    for (int l = 0; l < 64; ++l) {
-        scal->level_count[l] = 0;
-        scal->level_fill[l] = 0;
+    if (threadIdx.x == 0) {
+        scal->nmassive = 0;
+        scal->ncell = 0;
+        scal->nnode = 0;
+        scal->maxdepth = 0;
+        scal->counters[0] = scal->counters[1] = scal->counters[2] = scal->counters[3] = 0ull;
+    }
+    if (threadIdx.x < 64) {
+        scal->level_count[threadIdx.x] = 0;
+        scal->level_fill[threadIdx.x] = 0;
     }
 }
 
 /* ---------------------------------------------------------------------- */
 /* keys                                                                   */
 /* ---------------------------------------------------------------------- */
-/* Massless bodies are not inserted (nbMakeTree, nbody/src/nbody_tree.c:438-442):
+/* Sort word: simulation id above the top `top` Morton levels.  Massless
+ * bodies are not inserted (nbMakeTree, nbody/src/nbody_tree.c:438-442):
  * their key sorts after every massive key. */
 __global__ __launch_bounds__(256) void k_keys(int n, const double* __restrict__ px,
                                               const double* __restrict__ py,
                                               const double* __restrict__ pz,
                                               const double* __restrict__ m,
-                                              unsigned long long* __restrict__ key,
-                                              unsigned* __restrict__ perm,
-                                              DevScalars* __restrict__ scal)
+                                              const int* __restrict__ simid,
+                                              const SimTab* __restrict__ sims, int sim_shift,
+                                              int top, unsigned long long* __restrict__ key,
+                                              unsigned* __restrict__ perm)
 {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
-    const double ks = scal->key_scale;
     unsigned long long k = ~0ull;
     if (m[i] != 0.0) {
+        const int sim = simid[i];
+        const double ks = sims[sim].key_scale;
+        const int sh = 63 - top;
         unsigned long long ux = nbh_fixed(px[i], ks);
         unsigned long long uy = nbh_fixed(py[i], ks);
         unsigned long long uz = nbh_fixed(pz[i], ks);
-        k = nbh_morton21(ux >> 42, uy >> 42, uz >> 42);
+        k = nbh_morton21(ux >> sh, uy >> sh, uz >> sh);
+        if (sim_shift < 64) k |= (unsigned long long) sim << sim_shift;
     }
     key[i] = k;
     perm[i] = (unsigned) i;
@@ -243,17 +268,35 @@ __global__ __launch_bounds__(256) void k_radix_scatter(int n, int shift,
 }
 
 /* ---------------------------------------------------------------------- */
-/* ties in the first 21 levels: order by levels 22-63                      */
+/* ties in the sort word: order by the remaining levels                    */
 /* ---------------------------------------------------------------------- */
+/* Morton word of the 21 levels that start at level a (1-based); levels past
+ * 63 read as zero. */
+__device__ __forceinline__ unsigned long long nbh_word_at(unsigned long long ux,
+                                                          unsigned long long uy,
+                                                          unsigned long long uz, int a)
+{
+    const int s = a - 1;
+    return nbh_morton21(((ux << 1) << s) >> 43, ((uy << 1) << s) >> 43, ((uz << 1) << s) >> 43);
+}
+
 __device__ __forceinline__ void nbh_deep_words(const double* px, const double* py,
-                                               const double* pz, double ks, unsigned j,
-                                               unsigned long long& w1, unsigned long long& w2)
+                                               const double* pz, double ks, unsigned j, int top,
+                                               unsigned long long* w)
 {
     unsigned long long ux = nbh_fixed(px[j], ks), uy = nbh_fixed(py[j], ks),
                        uz = nbh_fixed(pz[j], ks);
-    const unsigned long long m21 = 0x1FFFFFull;
-    w1 = nbh_morton21((ux >> 21) & m21, (uy >> 21) & m21, (uz >> 21) & m21);
-    w2 = nbh_morton21(ux & m21, uy & m21, uz & m21);
+    w[0] = nbh_word_at(ux, uy, uz, top + 1);
+    w[1] = nbh_word_at(ux, uy, uz, top + 22);
+    w[2] = top + 43 <= 63 ? nbh_word_at(ux, uy, uz, top + 43) : 0ull;
+}
+
+__device__ __forceinline__ bool nbh_words_gt(const unsigned long long* a,
+                                             const unsigned long long* b)
+{
+    if (a[0] != b[0]) return a[0] > b[0];
+    if (a[1] != b[1]) return a[1] > b[1];
+    return a[2] > b[2];
 }
 
 __global__ __launch_bounds__(256) void k_tiefix(const unsigned long long* __restrict__ key,
@@ -261,6 +304,8 @@ __global__ __launch_bounds__(256) void k_tiefix(const unsigned long long* __rest
                                                 const double* __restrict__ px,
                                                 const double* __restrict__ py,
                                                 const double* __restrict__ pz,
+                                                const int* __restrict__ simid,
+                                                const SimTab* __restrict__ sims, int top,
                                                 const DevScalars* __restrict__ scal)
 {
     const int i = blockIdx.x * 256 + threadIdx.x;
@@ -271,16 +316,16 @@ __global__ __launch_bounds__(256) void k_tiefix(const unsigned long long* __rest
     if (i + 1 >= M || key[i + 1] != k) return;
     int e = i + 1;
     while (e < M && key[e] == k) ++e;
-    const double ks = scal->key_scale;
+    const double ks = sims[simid[perm[i]]].key_scale;
     for (int a = i + 1; a < e; ++a) {
-        unsigned v = perm[a];
-        unsigned long long v1, v2;
-        nbh_deep_words(px, py, pz, ks, v, v1, v2);
+        const unsigned v = perm[a];
+        unsigned long long wv[3];
+        nbh_deep_words(px, py, pz, ks, v, top, wv);
         int b = a - 1;
         while (b >= i) {
-            unsigned long long b1, b2;
-            nbh_deep_words(px, py, pz, ks, perm[b], b1, b2);
-            if (b1 > v1 || (b1 == v1 && b2 > v2)) {
+            unsigned long long wb[3];
+            nbh_deep_words(px, py, pz, ks, perm[b], top, wb);
+            if (nbh_words_gt(wb, wv)) {
                 perm[b + 1] = perm[b];
                 --b;
             } else {
@@ -299,52 +344,87 @@ __global__ __launch_bounds__(256) void k_gather(int n, const unsigned* __restric
                                                 const double* __restrict__ py,
                                                 const double* __restrict__ pz,
                                                 const double* __restrict__ m,
+                                                const int* __restrict__ simid,
+                                                const SimTab* __restrict__ sims,
                                                 double4* __restrict__ spos,
+                                                int* __restrict__ simid_s,
                                                 unsigned long long* __restrict__ su,
                                                 const DevScalars* __restrict__ scal)
 {
     const int s = blockIdx.x * 256 + threadIdx.x;
     if (s >= n) return;
     const unsigned j = perm[s];
-    double x = px[j], y = py[j], z = pz[j];
+    const double x = px[j], y = py[j], z = pz[j];
+    const int sim = simid[j];
     spos[s] = make_double4(x, y, z, m[j]);
+    simid_s[s] = sim;
     if (s < scal->nmassive) {
-        const double ks = scal->key_scale;
+        const double ks = sims[sim].key_scale;
         su[s] = nbh_fixed(x, ks);
         su[n + s] = nbh_fixed(y, ks);
         su[2 * n + s] = nbh_fixed(z, ks);
     }
 }
 
-/* Prefix length between sorted massive bodies i and j, with equal keys
- * ordered by index (Karras 2012, section 4). */
-__device__ __forceinline__ int nbh_kdelta(const unsigned long long* __restrict__ su, int n, int M,
+/* Common prefix, in bits, of the full keys (simulation id in the top sb
+ * bits, then the 189-bit Morton key) of sorted massive bodies i and j; -1
+ * outside [0, M); equal keys ordered by index (Karras 2012, section 4). */
+__device__ __forceinline__ int nbh_kdelta(const unsigned long long* __restrict__ su,
+                                          const int* __restrict__ simid_s, int n, int M, int sb,
                                           int i, int j)
 {
     if (j < 0 || j >= M) return -1;
+    const int si = simid_s[i], sj = simid_s[j];
+    if (si != sj) return __clz((unsigned) (si ^ sj)) - (32 - sb);
     int d = nbh_prefix_bits(su[i], su[n + i], su[2 * n + i], su[j], su[n + j], su[2 * n + j]);
     if (d == 189) d = 189 + __clz((unsigned) (i ^ j));
-    return d;
+    return sb + d;
+}
+
+__device__ __forceinline__ bool nbh_first_of_sim(const int* __restrict__ simid_s, int a)
+{
+    return a == 0 || simid_s[a - 1] != simid_s[a];
 }
 
 __global__ __launch_bounds__(256) void k_delta_count(int n, const unsigned long long* __restrict__ su,
+                                                     const int* __restrict__ simid_s,
                                                      int* __restrict__ dadj,
                                                      unsigned* __restrict__ ccount,
+                                                     SimTab* __restrict__ sims,
                                                      const DevScalars* __restrict__ scal)
 {
+    __shared__ int bsim, bmax, mixed;
+    if (threadIdx.x == 0) {
+        bsim = -1;
+        bmax = -1;
+        mixed = 0;
+    }
+    __syncthreads();
     const int a = blockIdx.x * 256 + threadIdx.x;
     const int M = scal->nmassive;
-    if (a >= M) return;
-    int d = nbh_kdelta(su, n, M, a, a + 1);
-    dadj[a] = d;
-    int c;
-    if (a == 0) {
-        c = max(nbh_levels(d), 0) + 1;
-    } else {
-        int dp = nbh_kdelta(su, n, M, a - 1, a);
-        c = max(0, nbh_levels(d) - nbh_levels(dp));
+    const int sb = scal->sim_bits;
+    int depth = -1, sim = -1;
+    if (a < M) {
+        const int d = nbh_kdelta(su, simid_s, n, M, sb, a, a + 1);
+        dadj[a] = d;
+        int c;
+        if (nbh_first_of_sim(simid_s, a)) {
+            c = max(nbh_levels(d, sb), 0) + 1;
+        } else {
+            const int dp = nbh_kdelta(su, simid_s, n, M, sb, a - 1, a);
+            c = max(0, nbh_levels(d, sb) - nbh_levels(dp, sb));
+        }
+        ccount[a] = (unsigned) c;
+        depth = max(nbh_levels(d, sb), 0);
+        sim = simid_s[a];
+        if (threadIdx.x == 0) bsim = sim;
     }
-    ccount[a] = (unsigned) c;
+    __syncthreads();
+    if (sim >= 0 && sim != bsim) mixed = 1;
+    if (sim >= 0) atomicMax(&bmax, depth);
+    __syncthreads();
+    if (sim >= 0 && mixed) atomicMax(&sims[sim].maxdepth, depth);
+    if (threadIdx.x == 0 && !mixed && bsim >= 0) atomicMax(&sims[bsim].maxdepth, bmax);
 }
 
 /* ---------------------------------------------------------------------- */
@@ -430,8 +510,50 @@ __global__ __launch_bounds__(1024) void k_scan_add(unsigned* __restrict__ data,
 }
 
 /* ---------------------------------------------------------------------- */
-/* node counts, trivial trees                                              */
+/* node counts, per-simulation segments                                    */
 /* ---------------------------------------------------------------------- */
+__global__ void k_node_counts(const unsigned* __restrict__ cincl, int node_cap,
+                              DevScalars* __restrict__ scal)
+{
+    const int M = scal->nmassive;
+    const int C = M > 0 ? (int) cincl[M - 1] : 0;
+    const int nn = M + C;
+    scal->ncell = C;
+    scal->nnode = nn;
+    if (nn > node_cap) scal->status |= NBHIP_CELL_OVERFLOW_ERROR;
+}
+
+/* The first massive body of each simulation owns its root cell. */
+__global__ __launch_bounds__(256) void k_sim_roots(const int* __restrict__ simid_s,
+                                                   const unsigned* __restrict__ cincl,
+                                                   const unsigned* __restrict__ ccount_unused,
+                                                   SimTab* __restrict__ sims,
+                                                   const DevScalars* __restrict__ scal)
+{
+    const int a = blockIdx.x * 256 + threadIdx.x;
+    if (a >= scal->nmassive) return;
+    if (!nbh_first_of_sim(simid_s, a)) return;
+    const int k = simid_s[a];
+    sims[k].first = a;
+    sims[k].root = a + (a > 0 ? (int) cincl[a - 1] : 0);
+}
+
+__global__ __launch_bounds__(256) void k_sim_ends(int nsim, SimTab* __restrict__ sims,
+                                                  DevScalars* __restrict__ scal)
+{
+    for (int k = threadIdx.x; k < nsim; k += 256) {
+        if (sims[k].root < 0) continue;
+        int end = scal->nnode;
+        for (int j = k + 1; j < nsim; ++j)
+            if (sims[j].root >= 0) {
+                end = sims[j].root;
+                break;
+            }
+        sims[k].end = end;
+        atomicMax(&scal->maxdepth, sims[k].maxdepth);
+    }
+}
+
 __device__ __forceinline__ double nbh_rcrit2(const nbhip_ctx& c, double mx, double my, double mz,
                                              double rsize, double cx, double cy, double cz,
                                              double psize)
@@ -463,41 +585,11 @@ __device__ __forceinline__ double nbh_rcrit2(const nbhip_ctx& c, double mx, doub
     return rc * rc;
 }
 
-__global__ void k_node_counts(const unsigned* __restrict__ cincl, NodeW* __restrict__ node,
-                              double* __restrict__ nquad, double* __restrict__ nmid,
-                              int* __restrict__ nlevel, int* __restrict__ nparent,
-                              int node_cap, nbhip_ctx ctx, DevScalars* __restrict__ scal)
-{
-    const int M = scal->nmassive;
-    const int C = M > 0 ? (int) cincl[M - 1] : 1;
-    const int nn = M + C;
-    scal->ncell = C;
-    scal->nnode = nn;
-    if (nn > node_cap) {
-        scal->status |= NBHIP_CELL_OVERFLOW_ERROR;
-        return;
-    }
-    if (M <= 1) {
-        /* root only (M == 0) or root + one body (M == 1) */
-        NodeW r;
-        r.x = r.y = r.z = 0.0;
-        r.m = 0.0;
-        r.rcrit2 = nbh_rcrit2(ctx, 0.0, 0.0, 0.0, scal->rsize, 0.0, 0.0, 0.0, scal->rsize);
-        r.next = M == 0 ? 1 : 2;
-        r.body = -1;
-        node[0] = r;
-        for (int k = 0; k < 6; ++k) nquad[k] = 0.0;
-        nmid[0] = nmid[1] = nmid[2] = 0.0;
-        nlevel[0] = 0;
-        nparent[0] = -1;
-        scal->level_count[0] = 1;
-    }
-}
-
 /* ---------------------------------------------------------------------- */
-/* Karras binary radix tree                                                */
+/* Karras binary radix tree (a forest across simulations)                  */
 /* ---------------------------------------------------------------------- */
 __global__ __launch_bounds__(256) void k_karras(int n, const unsigned long long* __restrict__ su,
+                                                const int* __restrict__ simid_s,
                                                 int* __restrict__ blo, int* __restrict__ bhi,
                                                 int* __restrict__ bdelta, int* __restrict__ bparent,
                                                 const DevScalars* __restrict__ scal)
@@ -506,23 +598,26 @@ __global__ __launch_bounds__(256) void k_karras(int n, const unsigned long long*
     const int M = scal->nmassive;
     if (i >= M - 1) return;
     if (scal->status & NBHIP_CELL_OVERFLOW_ERROR) return;
-    const int dr = nbh_kdelta(su, n, M, i, i + 1);
-    const int dl = nbh_kdelta(su, n, M, i, i - 1);
+    const int sb = scal->sim_bits;
+#define KD(a, b) nbh_kdelta(su, simid_s, n, M, sb, (a), (b))
+    const int dr = KD(i, i + 1);
+    const int dl = KD(i, i - 1);
     const int d = dr > dl ? 1 : -1;
     const int dmin = d > 0 ? dl : dr;
     int lmax = 2;
-    while (nbh_kdelta(su, n, M, i, i + lmax * d) > dmin) lmax <<= 1;
+    while (KD(i, i + lmax * d) > dmin) lmax <<= 1;
     int l = 0;
     for (int t = lmax >> 1; t >= 1; t >>= 1)
-        if (nbh_kdelta(su, n, M, i, i + (l + t) * d) > dmin) l += t;
+        if (KD(i, i + (l + t) * d) > dmin) l += t;
     const int j = i + l * d;
-    const int dnode = nbh_kdelta(su, n, M, i, j);
+    const int dnode = KD(i, j);
     int s = 0;
     for (int div = 2;; div <<= 1) {
         int t = (l + div - 1) / div;
-        if (nbh_kdelta(su, n, M, i, i + (s + t) * d) > dnode) s += t;
+        if (KD(i, i + (s + t) * d) > dnode) s += t;
         if (t <= 1) break;
     }
+#undef KD
     const int gamma = i + s * d + min(d, 0);
     const int lo = min(i, j), hi = max(i, j);
     blo[i] = lo;
@@ -544,11 +639,12 @@ struct EmitView {
     const int* bparent;
     const int* dadj;
     const unsigned* cincl;
+    int sb;
 };
 
 __device__ __forceinline__ int nbh_first_level(const EmitView& v, int lo)
 {
-    return lo == 0 ? 0 : nbh_levels(v.dadj[lo - 1]) + 1;
+    return lo == 0 ? 0 : nbh_levels(v.dadj[lo - 1], v.sb) + 1;
 }
 
 __device__ __forceinline__ int nbh_excl(const EmitView& v, int lo)
@@ -558,45 +654,82 @@ __device__ __forceinline__ int nbh_excl(const EmitView& v, int lo)
 
 __device__ __forceinline__ int nbh_parent_levels(const EmitView& v, int k)
 {
-    return k == 0 ? -1 : nbh_levels(v.bdelta[v.bparent[k]]);
+    return k == 0 ? -1 : nbh_levels(v.bdelta[v.bparent[k]], v.sb);
 }
 
 /* Pre-order index of the deepest octree cell owned by the nearest binary
- * ancestor-or-self of k that owns any cell. */
+ * ancestor-or-self of k that owns any cell (-1 on a corrupt tree). */
 __device__ __forceinline__ int nbh_owner_pre(const EmitView& v, int k, int M)
 {
-    for (int guard = 0; k > 0 && nbh_levels(v.bdelta[k]) <= nbh_parent_levels(v, k); ++guard) {
+    for (int guard = 0; k > 0 && nbh_levels(v.bdelta[k], v.sb) <= nbh_parent_levels(v, k);
+         ++guard) {
         k = v.bparent[k];
-        if (guard > 4 * 192 || k < 0 || k >= M - 1) return -1;
+        if (guard > 4 * 256 || k < 0 || k >= M - 1) return -1;
     }
-    if (k < 0 || k >= M - 1) return -1;
+    if (k < 0 || k >= M - 1 || nbh_levels(v.bdelta[k], v.sb) < 0) return -1;
     const int lo = v.blo[k];
-    return lo + nbh_excl(v, lo) + (nbh_levels(v.bdelta[k]) - nbh_first_level(v, lo));
+    return lo + nbh_excl(v, lo) + (nbh_levels(v.bdelta[k], v.sb) - nbh_first_level(v, lo));
+}
+
+__device__ __forceinline__ void nbh_emit_cell(int pre, int l, int next, int parent, int sim,
+                                              double R, unsigned long long ux,
+                                              unsigned long long uy, unsigned long long uz,
+                                              NodeW* __restrict__ node, double* __restrict__ nmid,
+                                              int* __restrict__ nlevel, int* __restrict__ nparent,
+                                              int* __restrict__ nsim, SimTab* __restrict__ sims)
+{
+    NodeW c;
+    c.x = c.y = c.z = c.m = 0.0;
+    c.rcrit2 = 0.0;
+    c.next = next;
+    c.body = -1;
+    node[pre] = c;
+    nlevel[pre] = l;
+    nparent[pre] = parent;
+    nsim[pre] = sim;
+    double mx = 0.0, my = 0.0, mz = 0.0;
+    if (l > 0) {
+        const double h = ldexp(R, -(l + 1));
+        const int sh = 63 - l;
+        mx = (double) (2ull * (ux >> sh) + 1ull) * h - 0.5 * R;
+        my = (double) (2ull * (uy >> sh) + 1ull) * h - 0.5 * R;
+        mz = (double) (2ull * (uz >> sh) + 1ull) * h - 0.5 * R;
+    }
+    nmid[3 * pre + 0] = mx;
+    nmid[3 * pre + 1] = my;
+    nmid[3 * pre + 2] = mz;
+    /* Structure-error rule of nbLoadBody (nbody/src/nbody_tree.c:256-264):
+     * a body descends through a cell of side <= DBL_EPSILON with an occupied
+     * slot, i.e. a cell exists at level l with rsize * 2^-(l-1) <= DBL_EPSILON. */
+    if (l >= 1 && ldexp(R, -(l - 1)) <= DBL_EPSILON) atomicOr(&sims[sim].flags, NBHIP_TREE_STRUCTURE_ERROR);
+    else if (l >= 63) atomicOr(&sims[sim].flags, NBHIP_MAX_DEPTH_ERROR);
 }
 
 __device__ __forceinline__ void nbh_emit_cells_body(
     const int* __restrict__ blo, const int* __restrict__ bhi, const int* __restrict__ bdelta,
     const int* __restrict__ bparent, const int* __restrict__ dadj,
-    const unsigned* __restrict__ cincl, const unsigned long long* __restrict__ su, int n,
-    NodeW* __restrict__ node, double* __restrict__ nmid, int* __restrict__ nlevel,
-    int* __restrict__ nparent, DevScalars* __restrict__ scal, int* lh)
+    const unsigned* __restrict__ cincl, const unsigned long long* __restrict__ su,
+    const int* __restrict__ simid_s, int n, NodeW* __restrict__ node,
+    double* __restrict__ nmid, int* __restrict__ nlevel, int* __restrict__ nparent,
+    int* __restrict__ nsim, SimTab* __restrict__ sims, DevScalars* __restrict__ scal, int* lh)
 {
     const int k = blockIdx.x * 256 + threadIdx.x;
     const int M = scal->nmassive;
     if (k >= M - 1) return;
     if (scal->status & NBHIP_CELL_OVERFLOW_ERROR) return;
-    EmitView v = { blo, bdelta, bparent, dadj, cincl };
-    const int Lk = nbh_levels(bdelta[k]);
+    EmitView v = { blo, bdelta, bparent, dadj, cincl, scal->sim_bits };
+    const int Lk = nbh_levels(bdelta[k], v.sb);
     const int Lp = nbh_parent_levels(v, k);
     if (Lk <= Lp) return;
     const int lo = blo[k], hi = bhi[k];
     const int base = lo + nbh_excl(v, lo) - nbh_first_level(v, lo);
     const int next = hi + 1 + (int) cincl[hi];
-    const double R = scal->rsize;
+    const int sim = simid_s[lo];
+    const double R = sims[sim].rsize;
     const unsigned long long ux = su[lo], uy = su[n + lo], uz = su[2 * n + lo];
     const int nn = scal->nnode;
     int top_parent = -1;
-    if (k != 0) {
+    if (Lp + 1 > 0) {
         top_parent = nbh_owner_pre(v, bparent[k], M);
         if (top_parent < 0) {
             atomicOr(&scal->status, NBHIP_ERROR);
@@ -609,25 +742,8 @@ __device__ __forceinline__ void nbh_emit_cells_body(
             atomicOr(&scal->status, NBHIP_ERROR);
             return;
         }
-        NodeW c;
-        c.x = c.y = c.z = c.m = 0.0;
-        c.rcrit2 = 0.0;
-        c.next = next;
-        c.body = -1;
-        node[pre] = c;
-        nlevel[pre] = l;
-        nparent[pre] = (l == Lp + 1) ? top_parent : pre - 1;
-        double mx = 0.0, my = 0.0, mz = 0.0;
-        if (l > 0) {
-            const double h = ldexp(R, -(l + 1));
-            const int sh = 63 - l;
-            mx = (double) (2ull * (ux >> sh) + 1ull) * h - 0.5 * R;
-            my = (double) (2ull * (uy >> sh) + 1ull) * h - 0.5 * R;
-            mz = (double) (2ull * (uz >> sh) + 1ull) * h - 0.5 * R;
-        }
-        nmid[3 * pre + 0] = mx;
-        nmid[3 * pre + 1] = my;
-        nmid[3 * pre + 2] = mz;
+        nbh_emit_cell(pre, l, next, (l == Lp + 1) ? top_parent : pre - 1, sim, R, ux, uy, uz,
+                      node, nmid, nlevel, nparent, nsim, sims);
         atomicAdd(&lh[l], 1);
     }
 }
@@ -639,47 +755,60 @@ __global__ __launch_bounds__(256) void k_emit_cells(const int* __restrict__ blo,
                                                     const int* __restrict__ dadj,
                                                     const unsigned* __restrict__ cincl,
                                                     const unsigned long long* __restrict__ su,
-                                                    int n, NodeW* __restrict__ node,
+                                                    const int* __restrict__ simid_s, int n,
+                                                    NodeW* __restrict__ node,
                                                     double* __restrict__ nmid,
                                                     int* __restrict__ nlevel,
                                                     int* __restrict__ nparent,
+                                                    int* __restrict__ nsim,
+                                                    SimTab* __restrict__ sims,
                                                     DevScalars* __restrict__ scal)
 {
     __shared__ int lh[64];
     if (threadIdx.x < 64) lh[threadIdx.x] = 0;
     __syncthreads();
-    nbh_emit_cells_body(blo, bhi, bdelta, bparent, dadj, cincl, su, n, node, nmid, nlevel,
-                        nparent, scal, lh);
+    nbh_emit_cells_body(blo, bhi, bdelta, bparent, dadj, cincl, su, simid_s, n, node, nmid,
+                        nlevel, nparent, nsim, sims, scal, lh);
     __syncthreads();
-    if (threadIdx.x < 64 && lh[threadIdx.x]) atomicAdd(&scal->level_count[threadIdx.x],
-                                                       lh[threadIdx.x]);
-    /* deepest level of this block = highest non-empty bin */
-    if (threadIdx.x == 0) {
-        int d = -1;
-        for (int l = 63; l >= 0; --l)
-            if (lh[l]) { d = l; break; }
-        if (d >= 0) atomicMax(&scal->maxdepth, d);
-    }
+    if (threadIdx.x < 64 && lh[threadIdx.x])
+        atomicAdd(&scal->level_count[threadIdx.x], lh[threadIdx.x]);
 }
 
+/* Body leaves; a simulation with a single massive body gets its root cell
+ * here (it owns no binary node). */
 __global__ __launch_bounds__(256) void k_emit_leaves(const int* __restrict__ blo,
                                                      const int* __restrict__ bdelta,
                                                      const int* __restrict__ bparent,
                                                      const int* __restrict__ dadj,
                                                      const unsigned* __restrict__ cincl,
                                                      const double4* __restrict__ spos,
+                                                     const int* __restrict__ simid_s,
                                                      NodeW* __restrict__ node,
+                                                     double* __restrict__ nmid,
                                                      int* __restrict__ nlevel,
                                                      int* __restrict__ nparent,
+                                                     int* __restrict__ nsim,
+                                                     SimTab* __restrict__ sims,
                                                      DevScalars* __restrict__ scal)
 {
     const int s = blockIdx.x * 256 + threadIdx.x;
     const int M = scal->nmassive;
     if (s >= M) return;
     if (scal->status & NBHIP_CELL_OVERFLOW_ERROR) return;
-    EmitView v = { blo, bdelta, bparent, dadj, cincl };
+    EmitView v = { blo, bdelta, bparent, dadj, cincl, scal->sim_bits };
     const int pre = s + (int) cincl[s];
-    const int parent = (M == 1) ? 0 : nbh_owner_pre(v, bparent[(M - 1) + s], M);
+    const int sim = simid_s[s];
+    const bool singleton = nbh_first_of_sim(simid_s, s) && (s + 1 == M || simid_s[s + 1] != sim);
+    int parent;
+    if (singleton) {
+        parent = pre - 1;
+        if (parent >= 0)
+            nbh_emit_cell(parent, 0, pre + 1, -1, sim, sims[sim].rsize, 0ull, 0ull, 0ull, node,
+                          nmid, nlevel, nparent, nsim, sims);
+        atomicAdd(&scal->level_count[0], 1);
+    } else {
+        parent = nbh_owner_pre(v, bparent[(M - 1) + s], M);
+    }
     if (pre < 0 || pre >= scal->nnode || parent < 0 || parent >= pre) {
         atomicOr(&scal->status, NBHIP_ERROR);
         return;
@@ -693,6 +822,7 @@ __global__ __launch_bounds__(256) void k_emit_leaves(const int* __restrict__ blo
     node[pre] = b;
     nlevel[pre] = -1;
     nparent[pre] = parent;
+    nsim[pre] = sim;
 }
 
 /* Cells grouped by level: exclusive scan of the per-level counts, then
@@ -735,99 +865,93 @@ __global__ __launch_bounds__(256) void k_level_scatter(const int* __restrict__ n
 /* hackCofM + findRCrit + nbCheckTreeStructure (nbody/src/nbody_tree.c:
  * 345-424) and hackQuad (:70-121), children in octant (pre-)order.  All
  * children of a level-l cell are bodies or level-(l+1) cells, finished by an
- * earlier launch. */
+ * earlier launch.  Grid-stride over the level's cells, so a fixed launch
+ * shape serves any count (graph capture). */
 __global__ __launch_bounds__(256) void k_summarize_level(const int* __restrict__ level_list,
                                                          int level, NodeW* __restrict__ node,
                                                          double* __restrict__ nquad,
                                                          const double* __restrict__ nmid,
+                                                         const int* __restrict__ nsim,
+                                                         SimTab* __restrict__ sims,
                                                          nbhip_ctx ctx, DevScalars* scal)
 {
     if (scal->status & (NBHIP_CELL_OVERFLOW_ERROR | NBHIP_ERROR)) return;
     const int cnt = scal->level_count[level];
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= cnt) return;
-    const int p = level_list[scal->level_off[level] + i];
     const int nn = scal->nnode;
-    const double rsize = scal->rsize;
-
-    const int end = node[p].next;
-    if (p < 0 || end <= p || end > nn) {
-        atomicOr(&scal->status, NBHIP_ERROR);
-        return;
-    }
-    double mass = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
-    for (int c = p + 1; c < end;) {
-        const NodeW ch = node[c];
-        mass += ch.m;
-        cx += ch.m * ch.x;
-        cy += ch.m * ch.y;
-        cz += ch.m * ch.z;
-        if (ch.next <= c || ch.next > end) {
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < cnt; i += gridDim.x * 256) {
+        const int p = level_list[scal->level_off[level] + i];
+        const int sim = nsim[p];
+        const double rsize = sims[sim].rsize;
+        const int end = node[p].next;
+        if (p < 0 || end <= p || end > nn) {
             atomicOr(&scal->status, NBHIP_ERROR);
             return;
         }
-        c = ch.next;
-    }
-    const double mx = nmid[3 * p + 0], my = nmid[3 * p + 1], mz = nmid[3 * p + 2];
-    if (mass > 0.0) {
-        cx /= mass;
-        cy /= mass;
-        cz /= mass;
-    } else {
-        cx = mx; cy = my; cz = mz;
-    }
-    const double psize = ldexp(rsize, -level);
-    const double half = 0.5 * psize;
-    if (cx < mx - half || cx > mx + half || cy < my - half || cy > my + half ||
-        cz < mz - half || cz > mz + half)
-        atomicOr(&scal->status, NBHIP_TREE_STRUCTURE_ERROR);
-    const double rc2 = nbh_rcrit2(ctx, mx, my, mz, rsize, cx, cy, cz, psize);
-
-    if (ctx.use_quad) {
-        double q0 = 0.0, q1 = 0.0, q2 = 0.0, q3 = 0.0, q4 = 0.0, q5 = 0.0;
+        double mass = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
         for (int c = p + 1; c < end;) {
             const NodeW ch = node[c];
-            const double dx = ch.x - cx, dy = ch.y - cy, dz = ch.z - cz;
-            const double drsq = dz * dz + (dy * dy + dx * dx);
-            const double m = ch.m;
-            double e0 = m * (3.0 * (dx * dx) - drsq);
-            double e1 = m * (3.0 * (dx * dy));
-            double e2 = m * (3.0 * (dx * dz));
-            double e3 = m * (3.0 * (dy * dy) - drsq);
-            double e4 = m * (3.0 * (dy * dz));
-            double e5 = m * (3.0 * (dz * dz) - drsq);
-            if (ch.body < 0) {
-                const double* cq = &nquad[6 * c];
-                e0 += cq[0]; e1 += cq[1]; e2 += cq[2];
-                e3 += cq[3]; e4 += cq[4]; e5 += cq[5];
+            mass += ch.m;
+            cx += ch.m * ch.x;
+            cy += ch.m * ch.y;
+            cz += ch.m * ch.z;
+            if (ch.next <= c || ch.next > end) {
+                atomicOr(&scal->status, NBHIP_ERROR);
+                return;
             }
-            q0 += e0; q1 += e1; q2 += e2; q3 += e3; q4 += e4; q5 += e5;
             c = ch.next;
         }
-        double* pq = &nquad[6 * p];
-        pq[0] = q0; pq[1] = q1; pq[2] = q2; pq[3] = q3; pq[4] = q4; pq[5] = q5;
-    }
-    NodeW w;
-    w.x = cx;
-    w.y = cy;
-    w.z = cz;
-    w.m = mass;
-    w.rcrit2 = rc2;
-    w.next = end;
-    w.body = -1;
-    node[p] = w;
-}
+        const double mx = nmid[3 * p + 0], my = nmid[3 * p + 1], mz = nmid[3 * p + 2];
+        if (mass > 0.0) {
+            cx /= mass;
+            cy /= mass;
+            cz /= mass;
+        } else {
+            cx = mx; cy = my; cz = mz;
+        }
+        const double psize = ldexp(rsize, -level);
+        const double half = 0.5 * psize;
+        if (cx < mx - half || cx > mx + half || cy < my - half || cy > my + half ||
+            cz < mz - half || cz > mz + half)
+            atomicOr(&sims[sim].flags, NBHIP_TREE_STRUCTURE_ERROR);
+        const double rc2 = nbh_rcrit2(ctx, mx, my, mz, rsize, cx, cy, cz, psize);
 
-/* Structure-error rule of nbLoadBody (nbody/src/nbody_tree.c:256-264):
- * descending through a cell of side <= DBL_EPSILON with an occupied slot,
- * i.e. a cell exists at level D with rsize * 2^-(D-1) <= DBL_EPSILON. */
-__global__ void k_depth_check(DevScalars* scal)
-{
-    const int D = scal->maxdepth;
-    if (D >= 1 && ldexp(scal->rsize, -(D - 1)) <= DBL_EPSILON)
-        scal->status |= NBHIP_TREE_STRUCTURE_ERROR;
-    else if (D >= 63)
-        scal->status |= NBHIP_MAX_DEPTH_ERROR;
+        if (ctx.use_quad) {
+            double q0 = 0.0, q1 = 0.0, q2 = 0.0, q3 = 0.0, q4 = 0.0, q5 = 0.0;
+            for (int c = p + 1; c < end;) {
+                const NodeW ch = node[c];
+                const double dx = ch.x - cx, dy = ch.y - cy, dz = ch.z - cz;
+                const double drsq = dz * dz + (dy * dy + dx * dx);
+                const double m = ch.m;
+                double e0 = m * (3.0 * (dx * dx) - drsq);
+                double e1 = m * (3.0 * (dx * dy));
+                double e2 = m * (3.0 * (dx * dz));
+                double e3 = m * (3.0 * (dy * dy) - drsq);
+                double e4 = m * (3.0 * (dy * dz));
+                double e5 = m * (3.0 * (dz * dz) - drsq);
+                if (ch.body < 0) {
+                    const double* cq = &nquad[6 * c];
+                    e0 += cq[0]; e1 += cq[1]; e2 += cq[2];
+                    e3 += cq[3]; e4 += cq[4]; e5 += cq[5];
+                }
+                q0 += e0; q1 += e1; q2 += e2; q3 += e3; q4 += e4; q5 += e5;
+                c = ch.next;
+            }
+            double* pq = &nquad[6 * p];
+            pq[0] = q0; pq[1] = q1; pq[2] = q2; pq[3] = q3; pq[4] = q4; pq[5] = q5;
+        } else {
+            double* pq = &nquad[6 * p];
+            pq[0] = pq[1] = pq[2] = pq[3] = pq[4] = pq[5] = 0.0;
+        }
+        NodeW w;
+        w.x = cx;
+        w.y = cy;
+        w.z = cz;
+        w.m = mass;
+        w.rcrit2 = rc2;
+        w.next = end;
+        w.body = -1;
+        node[p] = w;
+    }
 }
 
 /* ---------------------------------------------------------------------- */
@@ -840,8 +964,8 @@ void nbh_bbox(nbhip_state* st, int drift)
     const double dt = st->ctx.timestep;
     hipLaunchKernelGGL(k_drift_bbox, dim3(nbh_blocks(st->n, 256)), dim3(256), 0, st->stream,
                        st->n, drift, dt, 0.5 * dt, st->px, st->py, st->pz, st->vx, st->vy, st->vz,
-                       st->ax, st->ay, st->az, st->scal);
-    hipLaunchKernelGGL(k_rsize, dim3(1), dim3(1), 0, st->stream, st->scal);
+                       st->ax, st->ay, st->az, st->simid, st->sims);
+    hipLaunchKernelGGL(k_rsize, dim3(1), dim3(256), 0, st->stream, st->nsim, st->sims, st->scal);
 }
 
 static void nbh_mark(nbhip_state* st, int k)
@@ -849,14 +973,15 @@ static void nbh_mark(nbhip_state* st, int k)
     if (st->profiling) (void) hipEventRecord(st->ev[k], st->stream);
 }
 
-void nbh_build(nbhip_state* st)
+/* graph_safe: no host synchronisation (all 64 level launches, grid-stride) */
+void nbh_build(nbhip_state* st, int graph_safe)
 {
     const int n = st->n;
     const int g256 = nbh_blocks(n, 256);
     hipStream_t s = st->stream;
 
     hipLaunchKernelGGL(k_keys, dim3(g256), dim3(256), 0, s, n, st->px, st->py, st->pz, st->m,
-                       st->key, st->perm, st->scal);
+                       st->simid, st->sims, st->sim_shift, st->top_levels, st->key, st->perm);
     unsigned long long* kin = st->key;
     unsigned long long* kout = st->key_alt;
     unsigned* vin = st->perm;
@@ -874,51 +999,55 @@ void nbh_build(nbhip_state* st)
     /* 8 passes: sorted data is back in st->key / st->perm */
     hipLaunchKernelGGL(k_count_massive, dim3(g256), dim3(256), 0, s, n, st->key, st->scal);
     hipLaunchKernelGGL(k_tiefix, dim3(g256), dim3(256), 0, s, st->key, st->perm, st->px, st->py,
-                       st->pz, st->scal);
+                       st->pz, st->simid, st->sims, st->top_levels, st->scal);
     nbh_mark(st, 2);
 
     hipLaunchKernelGGL(k_gather, dim3(g256), dim3(256), 0, s, n, st->perm, st->px, st->py, st->pz,
-                       st->m, st->spos, st->su, st->scal);
-    hipLaunchKernelGGL(k_delta_count, dim3(g256), dim3(256), 0, s, n, st->su, st->dadj, st->cincl,
-                       st->scal);
+                       st->m, st->simid, st->sims, st->spos, st->simid_s, st->su, st->scal);
+    hipLaunchKernelGGL(k_delta_count, dim3(g256), dim3(256), 0, s, n, st->su, st->simid_s,
+                       st->dadj, st->cincl, st->sims, st->scal);
     const int nscan = nbh_blocks(n, NBH_SCAN_TILE);
     hipLaunchKernelGGL(k_scan_tiles, dim3(nscan), dim3(1024), 0, s, st->cincl, st->scan_part,
                        st->scal);
     hipLaunchKernelGGL(k_scan_part, dim3(1), dim3(1024), 0, s, st->scan_part, st->scal);
     hipLaunchKernelGGL(k_scan_add, dim3(nscan), dim3(1024), 0, s, st->cincl, st->scan_part,
                        st->scal);
-    hipLaunchKernelGGL(k_node_counts, dim3(1), dim3(1), 0, s, st->cincl, st->node, st->nquad,
-                       st->nmid, st->nlevel, st->nparent, st->node_cap,
-                       st->ctx, st->scal);
-    hipLaunchKernelGGL(k_karras, dim3(g256), dim3(256), 0, s, n, st->su, st->blo, st->bhi,
-                       st->bdelta, st->bparent, st->scal);
+    hipLaunchKernelGGL(k_node_counts, dim3(1), dim3(1), 0, s, st->cincl, st->node_cap, st->scal);
+    hipLaunchKernelGGL(k_sim_roots, dim3(g256), dim3(256), 0, s, st->simid_s, st->cincl,
+                       st->cincl, st->sims, st->scal);
+    hipLaunchKernelGGL(k_sim_ends, dim3(1), dim3(256), 0, s, st->nsim, st->sims, st->scal);
+    hipLaunchKernelGGL(k_karras, dim3(g256), dim3(256), 0, s, n, st->su, st->simid_s, st->blo,
+                       st->bhi, st->bdelta, st->bparent, st->scal);
     hipLaunchKernelGGL(k_emit_cells, dim3(g256), dim3(256), 0, s, st->blo, st->bhi, st->bdelta,
-                       st->bparent, st->dadj, st->cincl, st->su, n, st->node, st->nmid,
-                       st->nlevel, st->nparent, st->scal);
+                       st->bparent, st->dadj, st->cincl, st->su, st->simid_s, n, st->node,
+                       st->nmid, st->nlevel, st->nparent, st->nsim_node, st->sims, st->scal);
     hipLaunchKernelGGL(k_emit_leaves, dim3(g256), dim3(256), 0, s, st->blo, st->bdelta,
-                       st->bparent, st->dadj, st->cincl, st->spos, st->node, st->nlevel,
-                       st->nparent, st->scal);
+                       st->bparent, st->dadj, st->cincl, st->spos, st->simid_s, st->node,
+                       st->nmid, st->nlevel, st->nparent, st->nsim_node, st->sims, st->scal);
     hipLaunchKernelGGL(k_level_scan, dim3(1), dim3(1), 0, s, st->scal);
     hipLaunchKernelGGL(k_level_scatter, dim3(nbh_blocks(st->node_cap, 256)), dim3(256), 0, s,
                        st->nlevel, st->level_list, st->scal);
-    hipLaunchKernelGGL(k_depth_check, dim3(1), dim3(1), 0, s, st->scal);
     nbh_mark(st, 3);
-    /* per-level launches sized from the device counts (one small readback) */
     int counts[64];
-    int maxdepth = 63;
-    const bool have = hipMemcpyAsync(counts, st->scal->level_count, sizeof(counts),
-                                     hipMemcpyDeviceToHost, s) == hipSuccess &&
-                      hipStreamSynchronize(s) == hipSuccess;
-    if (have) {
-        maxdepth = 0;
-        for (int l = 0; l < 64; ++l)
-            if (counts[l] > 0) maxdepth = l;
+    int top = 63;
+    bool have = false;
+    if (!graph_safe) {
+        /* per-level launches sized from the device counts (one small readback) */
+        have = hipMemcpyAsync(counts, st->scal->level_count, sizeof(counts),
+                              hipMemcpyDeviceToHost, s) == hipSuccess &&
+               hipStreamSynchronize(s) == hipSuccess;
+        if (have) {
+            top = 0;
+            for (int l = 0; l < 64; ++l)
+                if (counts[l] > 0) top = l;
+        }
     }
-    for (int l = maxdepth; l >= 0; --l) {
-        const int g = have ? nbh_blocks(counts[l], 256) : nbh_blocks(st->node_cap, 256);
+    const int gmax = nbh_blocks(n, 256 * 8) + 1;
+    for (int l = top; l >= 0; --l) {
         if (have && counts[l] == 0) continue;
+        const int g = have ? nbh_blocks(counts[l], 256) : gmax;
         hipLaunchKernelGGL(k_summarize_level, dim3(g), dim3(256), 0, s, st->level_list, l,
-                           st->node, st->nquad, st->nmid, st->ctx, st->scal);
+                           st->node, st->nquad, st->nmid, st->nsim_node, st->sims, st->ctx, st->scal);
     }
     nbh_mark(st, 4);
 }

@@ -20,9 +20,10 @@ struct WalkOut {
     int n;
     int kick;
     int use_pot;
-    int pad;
+    int multi;          /* several simulations: lanes park at their own end */
+    int all_eps2_pos;   /* every simulation has eps2 > 0 */
+    int batch_shift;    /* mixed walk: log2 of the node block per FP32 partial sum */
     double dtHalf;
-    double eps2;
     nbhip_potential pot;
 };
 
@@ -69,42 +70,80 @@ __device__ __forceinline__ void nbh_walk_counters(DevScalars* scal, unsigned nbb
     }
 }
 
-/* Plain form: FP64 end to end, reference operation order. */
-template <bool QUAD, bool COUNT>
-__global__ __launch_bounds__(256) void k_walk64(const NodeW* __restrict__ node,
-                                                const double* __restrict__ nquad,
-                                                const double4* __restrict__ spos,
-                                                const unsigned* __restrict__ perm,
-                                                double* __restrict__ oax,
-                                                double* __restrict__ oay,
-                                                double* __restrict__ oaz,
-                                                double* __restrict__ vx,
-                                                double* __restrict__ vy,
-                                                double* __restrict__ vz,
-                                                DevScalars* __restrict__ scal, WalkOut o)
+__device__ __forceinline__ int nbh_wave_min_i(int v)
 {
-    const int s = blockIdx.x * 256 + threadIdx.x;
+    for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+__device__ __forceinline__ int nbh_wave_max_i(int v)
+{
+    for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+/* Lane set-up shared by both walks: the lane's simulation, its segment of
+ * the node table and its start position; the wave's first and last node. */
+struct LaneStart {
+    int sim, self, lp, end, q0, qend;
+    double eps2;
+};
+
+__device__ __forceinline__ LaneStart nbh_lane_start(int s, bool valid, int M,
+                                                    const int* __restrict__ simid_s,
+                                                    const SimTab* __restrict__ sims)
+{
+    LaneStart r;
+    r.sim = valid ? simid_s[s] : 0;
+    const SimTab& t = sims[r.sim];
+    const bool live = valid && t.root >= 0;
+    r.self = (valid && s < M) ? s : -2;
+    r.lp = live ? t.root : INT_MAX;
+    r.end = live ? t.end : 0;
+    r.eps2 = t.eps2;
+    r.q0 = __builtin_amdgcn_readfirstlane(nbh_wave_min_i(r.lp));
+    r.qend = __builtin_amdgcn_readfirstlane(nbh_wave_max_i(r.end));
+    return r;
+}
+
+/* Plain form: FP64 end to end, reference operation order. */
+template <int BS, bool QUAD, bool COUNT>
+__global__ __launch_bounds__(BS) void k_walk64(const NodeW* __restrict__ node,
+                                               const double* __restrict__ nquad,
+                                               const double4* __restrict__ spos,
+                                               const unsigned* __restrict__ perm,
+                                               const int* __restrict__ simid_s,
+                                               SimTab* __restrict__ sims,
+                                               double* __restrict__ oax,
+                                               double* __restrict__ oay,
+                                               double* __restrict__ oaz,
+                                               double* __restrict__ vx,
+                                               double* __restrict__ vy,
+                                               double* __restrict__ vz,
+                                               DevScalars* __restrict__ scal, WalkOut o)
+{
+    const int s = blockIdx.x * BS + threadIdx.x;
     if (scal->status & (NBHIP_CELL_OVERFLOW_ERROR | NBHIP_ERROR)) return;
     const int M = scal->nmassive;
-    const int nn = scal->nnode;
     const bool valid = s < o.n;
     if (__ballot(valid) == 0ull) return;
     const double4 p = valid ? spos[s] : make_double4(0.0, 0.0, 0.0, 0.0);
-    const int self = (valid && s < M) ? s : -2;
-    int lp = valid ? 0 : nn;
+    const LaneStart L = nbh_lane_start(s, valid, M, simid_s, sims);
+    const int self = L.self, my_end = L.end;
+    int lp = L.lp;
+    const double eps2 = L.eps2;
+    const bool eps2_pos = o.all_eps2_pos;
     double ax = 0.0, ay = 0.0, az = 0.0;
     bool met = false;
     unsigned nbb = 0, nbc = 0, nop = 0, steps = 0;
-    const double eps2 = o.eps2;
-    const bool eps2_pos = eps2 > 0.0;
 
     /* The opening tests come first: once the wave's vote is known the
      * successor node is fetched, and its scalar-load latency overlaps the
      * interaction math of the current node. */
     bool bad = false;
-    int q = 0;
-    NodeW nd = node[0];
-    while (q < nn) {
+    int q = L.q0;
+    NodeW nd = node[max(0, min(q, scal->nnode - 1))];
+    while (q < L.qend) {
         if (COUNT) ++steps;
         const int nd_next = __builtin_amdgcn_readfirstlane(nd.next);
         const bool active = lp == q;
@@ -121,7 +160,7 @@ __global__ __launch_bounds__(256) void k_walk64(const NodeW* __restrict__ node,
             bad = true;
             break;
         }
-        const NodeW nxt = node[min(qn, nn - 1)];
+        const NodeW nxt = node[min(qn, L.qend - 1)];
         if (accept) {
             /* with eps2 > 0 the self term adds exactly +0 (dx = dy = dz = 0),
              * so lanes need not branch around it */
@@ -160,6 +199,7 @@ __global__ __launch_bounds__(256) void k_walk64(const NodeW* __restrict__ node,
             lp = q + 1;
             if (COUNT) ++nop;
         }
+        if (o.multi && lp >= my_end) lp = INT_MAX;
         q = qn;
         nd = nxt;
     }
@@ -167,15 +207,13 @@ __global__ __launch_bounds__(256) void k_walk64(const NodeW* __restrict__ node,
     if (bad && __lane_id() == 0) atomicOr(&scal->status, NBHIP_ERROR);
     if (COUNT) nbh_walk_counters(scal, nbb, nbc, nop, steps);
     if (!valid) return;
-    if (!met) atomicOr(&scal->incest, 1u);
+    if (!met) atomicOr(&sims[L.sim].incest, 1u);
     nbh_walk_finish(o, p.x, p.y, p.z, ax, ay, az, perm[s], oax, oay, oaz, vx, vy, vz);
 }
 
 /* Mixed form: displacements and interactions in FP32, FP32 partial sums
- * over batches of NBH_MIXED_BATCH node steps, batches added into FP64 with
- * Kahan compensation.  Positions, potential and kicks stay FP64. */
-#define NBH_MIXED_BATCH 32
-
+ * over batches of o.batch node steps, batches added into FP64 with Kahan
+ * compensation.  Positions, potential and kicks stay FP64. */
 __device__ __forceinline__ void nbh_kahan_add(double& sum, double& comp, double v)
 {
     const double y = v - comp;
@@ -184,46 +222,50 @@ __device__ __forceinline__ void nbh_kahan_add(double& sum, double& comp, double 
     sum = t;
 }
 
-template <bool QUAD, bool COUNT>
-__global__ __launch_bounds__(256) void k_walk32(const NodeF* __restrict__ node,
-                                                const NodeW* __restrict__ root,
-                                                const double4* __restrict__ spos,
-                                                const unsigned* __restrict__ perm,
-                                                double* __restrict__ oax,
-                                                double* __restrict__ oay,
-                                                double* __restrict__ oaz,
-                                                double* __restrict__ vx,
-                                                double* __restrict__ vy,
-                                                double* __restrict__ vz,
-                                                DevScalars* __restrict__ scal, WalkOut o)
+template <int BS, bool QUAD, bool COUNT>
+__global__ __launch_bounds__(BS) void k_walk32(const NodeF* __restrict__ node,
+                                               const NodeW* __restrict__ nodew,
+                                               const double4* __restrict__ spos,
+                                               const unsigned* __restrict__ perm,
+                                               const int* __restrict__ simid_s,
+                                               SimTab* __restrict__ sims,
+                                               double* __restrict__ oax,
+                                               double* __restrict__ oay,
+                                               double* __restrict__ oaz,
+                                               double* __restrict__ vx,
+                                               double* __restrict__ vy,
+                                               double* __restrict__ vz,
+                                               DevScalars* __restrict__ scal, WalkOut o)
 {
-    const int s = blockIdx.x * 256 + threadIdx.x;
+    const int s = blockIdx.x * BS + threadIdx.x;
     if (scal->status & (NBHIP_CELL_OVERFLOW_ERROR | NBHIP_ERROR)) return;
     const int M = scal->nmassive;
-    const int nn = scal->nnode;
     const bool valid = s < o.n;
     if (__ballot(valid) == 0ull) return;
     const double4 p = valid ? spos[s] : make_double4(0.0, 0.0, 0.0, 0.0);
-    const NodeW r0 = root[0];
+    const LaneStart L = nbh_lane_start(s, valid, M, simid_s, sims);
+    const int self = L.self, my_end = L.end;
+    int lp = L.lp;
+    /* offsets from the simulation's root centre of mass, split hi + lo */
+    const int root = sims[L.sim].root;
+    const NodeW r0 = nodew[root >= 0 ? root : 0];
     const double ox = p.x - r0.x, oy = p.y - r0.y, oz = p.z - r0.z;
     const float phx = (float) ox, phy = (float) oy, phz = (float) oz;
     const float plx = (float) (ox - (double) phx);
     const float ply = (float) (oy - (double) phy);
     const float plz = (float) (oz - (double) phz);
-    const int self = (valid && s < M) ? s : -2;
-    int lp = valid ? 0 : nn;
-    const float eps2 = (float) o.eps2;
-    const bool eps2_pos = o.eps2 > 0.0;
+    const float eps2 = (float) L.eps2;
+    const bool eps2_pos = o.all_eps2_pos;
+    const int bshift = o.batch_shift;
     float sx = 0.0f, sy = 0.0f, sz = 0.0f;
     double ax = 0.0, ay = 0.0, az = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
-    bool met = false;
+    bool met = false, pending = false;
     unsigned nbb = 0, nbc = 0, nop = 0, steps = 0;
-    int batch = 0;
 
     bool bad = false;
-    int q = 0;
-    NodeF nd = node[0];
-    while (q < nn) {
+    int q = L.q0;
+    NodeF nd = node[max(0, min(q, scal->nnode - 1))];
+    while (q < L.qend) {
         if (COUNT) ++steps;
         const int nd_next = __builtin_amdgcn_readfirstlane(nd.next);
         const bool active = lp == q;
@@ -239,7 +281,7 @@ __global__ __launch_bounds__(256) void k_walk32(const NodeF* __restrict__ node,
             bad = true;
             break;
         }
-        const NodeF nxt = node[min(qn, nn - 1)];
+        const NodeF nxt = node[min(qn, L.qend - 1)];
         if (accept) {
             if (eps2_pos || nd.body != self) {
                 drSq += eps2;
@@ -267,18 +309,25 @@ __global__ __launch_bounds__(256) void k_walk32(const NodeF* __restrict__ node,
                 }
             }
             met |= nd.body == self;
+            pending = true;
             lp = nd_next;
         } else if (open) {
             lp = q + 1;
             if (COUNT) ++nop;
         }
-        batch = __builtin_amdgcn_readfirstlane(batch + 1);
-        if (batch == NBH_MIXED_BATCH) {
-            batch = 0;
+        if (o.multi && lp >= my_end) lp = INT_MAX;
+        /* A lane's FP32 partial sum covers its interactions inside one
+         * block of 2^bshift node indices counted from its simulation's
+         * root.  q increases monotonically, so the grouping depends only on
+         * the lane's own interaction list, not on which lanes share its
+         * wave: results do not change with the batching of simulations or
+         * the workgroup size. */
+        if (((qn - root) >> bshift) != ((q - root) >> bshift) && pending) {
             nbh_kahan_add(ax, cx, (double) sx);
             nbh_kahan_add(ay, cy, (double) sy);
             nbh_kahan_add(az, cz, (double) sz);
             sx = sy = sz = 0.0f;
+            pending = false;
         }
         q = qn;
         nd = nxt;
@@ -290,21 +339,23 @@ __global__ __launch_bounds__(256) void k_walk32(const NodeF* __restrict__ node,
     if (bad && __lane_id() == 0) atomicOr(&scal->status, NBHIP_ERROR);
     if (COUNT) nbh_walk_counters(scal, nbb, nbc, nop, steps);
     if (!valid) return;
-    if (!met) atomicOr(&scal->incest, 1u);
+    if (!met) atomicOr(&sims[L.sim].incest, 1u);
     nbh_walk_finish(o, p.x, p.y, p.z, ax, ay, az, perm[s], oax, oay, oaz, vx, vy, vz);
 }
 
-/* Mixed-precision walk records from the FP64 tree: offsets from the root
- * centre of mass, split hi + lo. */
+/* Mixed-precision walk records from the FP64 tree: offsets from the
+ * simulation's root centre of mass, split hi + lo. */
 __global__ __launch_bounds__(256) void k_pack32(const NodeW* __restrict__ node,
                                                 const double* __restrict__ nquad,
+                                                const int* __restrict__ nsim,
+                                                const SimTab* __restrict__ sims,
                                                 NodeF* __restrict__ nodef,
                                                 const DevScalars* __restrict__ scal)
 {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (scal->status & (NBHIP_CELL_OVERFLOW_ERROR | NBHIP_ERROR)) return;
     if (i >= scal->nnode) return;
-    const NodeW r0 = node[0];
+    const NodeW r0 = node[sims[nsim[i]].root];
     const NodeW w = node[i];
     const double dx = w.x - r0.x, dy = w.y - r0.y, dz = w.z - r0.z;
     NodeF f;
@@ -331,7 +382,9 @@ __global__ __launch_bounds__(256) void k_pack32(const NodeW* __restrict__ node,
 __global__ __launch_bounds__(256) void k_exact_all(int n, const double* __restrict__ px,
                                                    const double* __restrict__ py,
                                                    const double* __restrict__ pz,
-                                                   const double* __restrict__ m, double eps2,
+                                                   const double* __restrict__ m,
+                                                   const int* __restrict__ simid,
+                                                   const SimTab* __restrict__ sims,
                                                    int use_pot, nbhip_potential pot, int kick,
                                                    double dtHalf, double* __restrict__ oax,
                                                    double* __restrict__ oay,
@@ -343,8 +396,10 @@ __global__ __launch_bounds__(256) void k_exact_all(int n, const double* __restri
     const int i = blockIdx.x * 256 + threadIdx.x;
     const bool valid = i < n;
     const double x0 = valid ? px[i] : 0.0, y0 = valid ? py[i] : 0.0, z0 = valid ? pz[i] : 0.0;
+    const SimTab& t = sims[valid ? simid[i] : 0];
+    const double eps2 = t.eps2;
     double ax = 0.0, ay = 0.0, az = 0.0;
-    for (int k = 0; k < n; ++k) {
+    for (int k = t.body_off; k < t.body_off + t.body_n; ++k) {
         const double dx = px[k] - x0;
         const double dy = py[k] - y0;
         const double dz = pz[k] - z0;
@@ -376,17 +431,21 @@ __global__ __launch_bounds__(256) void k_exact_all(int n, const double* __restri
 
 /* FP64 direct-sum checker on sampled bodies: one block per sample, the
  * sum over sources split across 256 lanes and reduced in a fixed order. */
-__global__ __launch_bounds__(256) void k_direct_sample(int n, const double* __restrict__ px,
+__global__ __launch_bounds__(256) void k_direct_sample(const double* __restrict__ px,
                                                        const double* __restrict__ py,
                                                        const double* __restrict__ pz,
-                                                       const double* __restrict__ m, double eps2,
+                                                       const double* __restrict__ m,
+                                                       const int* __restrict__ simid,
+                                                       const SimTab* __restrict__ sims,
                                                        const int* __restrict__ idx,
                                                        double* __restrict__ acc)
 {
     const int b = idx[blockIdx.x];
     const double x0 = px[b], y0 = py[b], z0 = pz[b];
+    const SimTab& t = sims[simid[b]];
+    const double eps2 = t.eps2;
     double ax = 0.0, ay = 0.0, az = 0.0;
-    for (int k = threadIdx.x; k < n; k += 256) {
+    for (int k = t.body_off + (int) threadIdx.x; k < t.body_off + t.body_n; k += 256) {
         const double dx = px[k] - x0;
         const double dy = py[k] - y0;
         const double dz = pz[k] - z0;
@@ -420,48 +479,61 @@ __global__ __launch_bounds__(256) void k_direct_sample(int n, const double* __re
 static inline int nbh_blocks(int n, int per) { return n > 0 ? (n + per - 1) / per : 1; }
 
 void nbh_walk(nbhip_state* st, int kick, int count, double* oax, double* oay, double* oaz,
-              int self_only)
+              int self_only, const WalkTune* tune)
 {
     const nbhip_ctx& c = st->ctx;
+    hipStream_t s = st->stream;
     if (c.criterion == NBHIP_EXACT) {
-        hipLaunchKernelGGL(k_exact_all, dim3(nbh_blocks(st->n, 256)), dim3(256), 0, st->stream,
-                           st->n, st->px, st->py, st->pz, st->m, c.eps2,
+        hipLaunchKernelGGL(k_exact_all, dim3(nbh_blocks(st->n, 256)), dim3(256), 0, s, st->n,
+                           st->px, st->py, st->pz, st->m, st->simid, st->sims,
                            self_only ? 0 : c.use_potential, c.pot, kick, 0.5 * c.timestep, oax,
                            oay, oaz, st->vx, st->vy, st->vz);
         return;
     }
+    const WalkTune t = tune ? *tune : st->tune;
     WalkOut o;
     o.n = st->n;
     o.kick = kick;
     o.use_pot = self_only ? 0 : c.use_potential;
-    o.pad = 0;
+    o.multi = st->nsim > 1;
+    o.all_eps2_pos = st->all_eps2_pos;
+    int bsh = 0;
+    while (bsh < 30 && (1 << (bsh + 1)) <= (t.batch > 0 ? t.batch : 1024)) ++bsh;
+    o.batch_shift = bsh;
     o.dtHalf = 0.5 * c.timestep;
-    o.eps2 = c.eps2;
     o.pot = c.pot;
-    dim3 grid(nbh_blocks(st->n, 256)), block(256);
-    hipStream_t s = st->stream;
+    const int bs = (t.block == 64 || t.block == 128) ? t.block : 256;
+    dim3 grid(nbh_blocks(st->n, bs)), block(bs);
     if (c.precision == NBHIP_PRECISION_MIXED) {
-        hipLaunchKernelGGL(k_pack32, dim3(nbh_blocks(st->node_cap, 256)), block, 0, s, st->node,
-                           st->nquad, st->nodef, st->scal);
-#define NBH_W32(Q, C)                                                                        \
-    hipLaunchKernelGGL((k_walk32<Q, C>), grid, block, 0, s, st->nodef, st->node, st->spos,    \
-                       st->perm, oax, oay, oaz, st->vx, st->vy, st->vz, st->scal, o)
-        if (c.use_quad) { if (count) NBH_W32(true, true); else NBH_W32(true, false); }
-        else { if (count) NBH_W32(false, true); else NBH_W32(false, false); }
+        hipLaunchKernelGGL(k_pack32, dim3(nbh_blocks(st->node_cap, 256)), dim3(256), 0, s,
+                           st->node, st->nquad, st->nsim_node, st->sims, st->nodef, st->scal);
+#define NBH_W32(B, Q, C)                                                                     \
+    hipLaunchKernelGGL((k_walk32<B, Q, C>), grid, block, 0, s, st->nodef, st->node, st->spos, \
+                       st->perm, st->simid_s, st->sims, oax, oay, oaz, st->vx, st->vy, st->vz, \
+                       st->scal, o)
+#define NBH_W32B(B)                                                                          \
+    if (c.use_quad) { if (count) NBH_W32(B, true, true); else NBH_W32(B, true, false); }     \
+    else { if (count) NBH_W32(B, false, true); else NBH_W32(B, false, false); }
+        if (bs == 64) { NBH_W32B(64) } else if (bs == 128) { NBH_W32B(128) } else { NBH_W32B(256) }
+#undef NBH_W32B
 #undef NBH_W32
         return;
     }
-#define NBH_W64(Q, C)                                                                        \
-    hipLaunchKernelGGL((k_walk64<Q, C>), grid, block, 0, s, st->node, st->nquad, st->spos,    \
-                       st->perm, oax, oay, oaz, st->vx, st->vy, st->vz, st->scal, o)
-    if (c.use_quad) { if (count) NBH_W64(true, true); else NBH_W64(true, false); }
-    else { if (count) NBH_W64(false, true); else NBH_W64(false, false); }
+#define NBH_W64(B, Q, C)                                                                     \
+    hipLaunchKernelGGL((k_walk64<B, Q, C>), grid, block, 0, s, st->node, st->nquad, st->spos, \
+                       st->perm, st->simid_s, st->sims, oax, oay, oaz, st->vx, st->vy, st->vz, \
+                       st->scal, o)
+#define NBH_W64B(B)                                                                          \
+    if (c.use_quad) { if (count) NBH_W64(B, true, true); else NBH_W64(B, true, false); }     \
+    else { if (count) NBH_W64(B, false, true); else NBH_W64(B, false, false); }
+    if (bs == 64) { NBH_W64B(64) } else if (bs == 128) { NBH_W64B(128) } else { NBH_W64B(256) }
+#undef NBH_W64B
 #undef NBH_W64
 }
 
 void nbh_direct(nbhip_state* st, int nsample, const int* d_idx, double* d_acc)
 {
     if (nsample <= 0) return;
-    hipLaunchKernelGGL(k_direct_sample, dim3(nsample), dim3(256), 0, st->stream, st->n, st->px,
-                       st->py, st->pz, st->m, st->ctx.eps2, d_idx, d_acc);
+    hipLaunchKernelGGL(k_direct_sample, dim3(nsample), dim3(256), 0, st->stream, st->px, st->py,
+                       st->pz, st->m, st->simid, st->sims, d_idx, d_acc);
 }

@@ -75,11 +75,17 @@ class _TreeInfo(C.Structure):
     ]
 
 
+class _Tune(C.Structure):
+    _fields_ = [("block", C.c_int), ("batch", C.c_int), ("ms", C.c_double)]
+
+
 EXPORTED = [
     "nbhip_create", "nbhip_upload", "nbhip_download", "nbhip_gravity", "nbhip_step",
     "nbhip_run", "nbhip_steps", "nbhip_tree_info_get", "nbhip_tree_download",
     "nbhip_direct_accel", "nbhip_walk_accel", "nbhip_walk_counters", "nbhip_set_profiling",
     "nbhip_phase_times", "nbhip_sync", "nbhip_release", "nbhip_last_error",
+    "nbhip_batch_create", "nbhip_batch_upload", "nbhip_batch_download", "nbhip_batch_status",
+    "nbhip_batch_tree_info", "nbhip_set_graph", "nbhip_autotune", "nbhip_set_tune",
 ]
 
 _LIB: Optional[C.CDLL] = None
@@ -117,6 +123,16 @@ def load_library() -> C.CDLL:
         "nbhip_sync": (u, [st]),
         "nbhip_release": (None, [st]),
         "nbhip_last_error": (C.c_char_p, []),
+        "nbhip_batch_create": (u, [C.POINTER(_Ctx), C.c_int, C.c_int, _ip, _dp,
+                                   C.POINTER(C.c_void_p)]),
+        "nbhip_batch_upload": (u, [st, C.c_int, _dp, _dp, _dp]),
+        "nbhip_batch_download": (u, [st, C.c_int, _dp, _dp, _dp]),
+        "nbhip_batch_status": (u, [st, C.POINTER(C.c_uint)]),
+        "nbhip_batch_tree_info": (u, [st, C.c_int, C.POINTER(_TreeInfo)]),
+        "nbhip_set_graph": (u, [st, C.c_int]),
+        "nbhip_autotune": (u, [st, C.POINTER(_Tune), C.POINTER(_Tune), C.c_int,
+                               C.POINTER(C.c_int)]),
+        "nbhip_set_tune": (u, [st, C.POINTER(_Tune)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -387,6 +403,24 @@ class NBodyState:
         self._check(self.lib.nbhip_walk_counters(self._h, c))
         return int(c[0]), int(c[1]), int(c[2]), int(c[3])
 
+    def set_graph(self, enable: bool = True) -> None:
+        """Replay steps as a captured hipGraph (no host sync inside a step)."""
+        self._check(self.lib.nbhip_set_graph(self._h, int(enable)))
+
+    def autotune(self):
+        """Time every walk setting on the current tree; keep the fastest.
+        Returns (best (block, batch, ms), [all trials])."""
+        best = _Tune()
+        trials = (_Tune * 32)()
+        cnt = C.c_int(0)
+        self._check(self.lib.nbhip_autotune(self._h, C.byref(best), trials, 32, C.byref(cnt)))
+        return ((best.block, best.batch, best.ms),
+                [(trials[i].block, trials[i].batch, trials[i].ms) for i in range(cnt.value)])
+
+    def set_tune(self, block: int, batch: int = 1024) -> None:
+        t = _Tune(block, batch, 0.0)
+        self._check(self.lib.nbhip_set_tune(self._h, C.byref(t)))
+
     def set_profiling(self, enable: bool = True) -> None:
         self._check(self.lib.nbhip_set_profiling(self._h, int(enable)))
 
@@ -414,3 +448,53 @@ class NBodyState:
 
     def __exit__(self, *exc):
         self.close()
+
+
+class NBodyBatch(NBodyState):
+    """Many independent simulations in one state (same context and potential,
+    per-simulation softening): every phase of a step is one launch."""
+
+    def __init__(self, ctx: NBodyCtx, potential: Optional[Potential], bodies, eps2=None,
+                 device: int = 0, precision: str = "fp64", prestep: bool = True):
+        self.lib = load_library()
+        self.ctx = ctx
+        self.potential = potential
+        self.precision = precision
+        self.counts = [int(np.asarray(b[2]).shape[0]) for b in bodies]
+        self.nsim = len(bodies)
+        self.n = sum(self.counts)
+        cnt = np.asarray(self.counts, dtype=np.int32)
+        e2 = None if eps2 is None else np.ascontiguousarray(eps2, dtype=np.float64)
+        h = C.c_void_p()
+        cctx = ctx.to_c(potential, precision)
+        rc = self.lib.nbhip_batch_create(C.byref(cctx), device, self.nsim,
+                                         _ptr(cnt, C.c_int), _ptr(e2), C.byref(h))
+        if rc:
+            raise NBodyError(rc, self.lib.nbhip_last_error().decode())
+        self._h = h
+        for k, (pos, vel, mass) in enumerate(bodies):
+            n = self.counts[k]
+            pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(n, 3)
+            vel = np.ascontiguousarray(vel, dtype=np.float64).reshape(n, 3)
+            mass = np.ascontiguousarray(mass, dtype=np.float64).reshape(n)
+            self._check(self.lib.nbhip_batch_upload(h, k, _ptr(pos), _ptr(vel), _ptr(mass)))
+        self.last_status = 0
+        if prestep:
+            self.last_status = self._check(self.lib.nbhip_gravity(h), allow_warning=True)
+
+    def sim_bodies(self, k: int):
+        n = self.counts[k]
+        pos, vel, acc = np.empty((n, 3)), np.empty((n, 3)), np.empty((n, 3))
+        self._check(self.lib.nbhip_batch_download(self._h, k, _ptr(pos), _ptr(vel), _ptr(acc)))
+        return pos, vel, acc
+
+    def statuses(self):
+        out = (C.c_uint * self.nsim)()
+        self._check(self.lib.nbhip_batch_status(self._h, out))
+        return [int(x) for x in out]
+
+    def sim_tree_info(self, k: int) -> TreeInfo:
+        ti = _TreeInfo()
+        self._check(self.lib.nbhip_batch_tree_info(self._h, k, C.byref(ti)))
+        return TreeInfo(ti.rsize, ti.ncell, ti.max_depth, ti.nnode, ti.nmassive, ti.step,
+                        ti.status)
