@@ -51,12 +51,23 @@ def walk_flops(counts, use_quad=True):
     return 18.0 * nbb + (56.0 if use_quad else 18.0) * nbc + 9.0 * nop
 
 
-def run_gpu(precision, pot, pos, vel, m, eps2, dt, steps, warmup, theta):
+def run_gpu(precision, pot, pos, vel, m, eps2, dt, steps, warmup, theta, autotune=True,
+            graph_steps=0):
+    """Timed region: `steps` KDK steps with per-phase HIP events on the
+    library's stream (the walk time of the roofline comes from these).
+    Optionally afterwards, `graph_steps` more steps replayed as a hipGraph
+    (no host synchronisation between steps) for the graph-mode rate."""
     from milkywayathome_client_amd import nbody as nb
     ctx = nb.NBodyCtx.create(timestep=dt, timeEvolve=dt * (steps + warmup), eps2=eps2,
                              theta=theta, criterion="BH86", useQuad=True, treeRSize=4.0)
     st = nb.NBodyState(ctx, pot, pos, vel, m, precision=precision)
     st.sync()
+    tune = None
+    if autotune:
+        best, trials = st.autotune()
+        tune = {"block": best[0], "batch": best[1], "walk_ms": best[2],
+                "trials": [[b, c, round(t, 4)] for b, c, t in trials]}
+        log(f"[bench] {precision} autotune: best {best}, trials {trials}")
     if warmup:
         st.steps(warmup)
     st.sync()
@@ -70,7 +81,17 @@ def run_gpu(precision, pot, pos, vel, m, eps2, dt, steps, warmup, theta):
     wall = time.perf_counter() - t0
     st.set_profiling(False)
     counts = st.walk_counters()
-    return st, wall, phases / steps, counts
+    graph_ms = None
+    if graph_steps:
+        st.set_graph(True)
+        st.steps(1)                # capture + first replay
+        st.sync()
+        t0 = time.perf_counter()
+        st.steps(graph_steps)
+        st.sync()
+        graph_ms = 1e3 * (time.perf_counter() - t0) / graph_steps
+        st.set_graph(False)
+    return st, wall, phases / steps, counts, tune, graph_ms
 
 
 def accel_errors(st, pot, n, nsample=4096, seed=1):
@@ -136,6 +157,11 @@ def main():
     ap.add_argument("--theta", type=float, default=0.5)
     ap.add_argument("--precision", choices=["both", "fp64", "mixed"], default="both")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default walk settings instead of timing the candidates")
+    ap.add_argument("--graph-steps", type=int, default=20,
+                    help="extra steps replayed as a hipGraph after the timed region "
+                         "(reported as graph_ms_per_step; 0 to skip)")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run a plain device-to-device copy of BYTES (profiler "
@@ -166,14 +192,16 @@ def main():
     for prec in precs:
         log(f"[bench] {prec}: warmup {args.warmup}, timed {args.steps}")
         try:
-            st, wall, phases, counts = run_gpu(prec, pot, pos, vel, m, eps2, dt, args.steps,
-                                               args.warmup, args.theta)
+            st, wall, phases, counts, tune, graph_ms = run_gpu(
+                prec, pot, pos, vel, m, eps2, dt, args.steps, args.warmup, args.theta,
+                autotune=not args.no_autotune, graph_steps=args.graph_steps)
         except nb.NBodyError as e:
             log(f"[bench] {prec} failed: {e}")
             continue
         errs = accel_errors(st, pot, args.n)
         results[prec] = {"wall": wall, "phases_ms": phases, "counts": counts, "errors": errs,
-                         "body_steps_per_s": args.n * args.steps / wall}
+                         "body_steps_per_s": args.n * args.steps / wall, "tune": tune,
+                         "graph_ms": graph_ms}
         log(f"[bench] {prec}: {results[prec]['body_steps_per_s']:.4g} body-steps/s, "
             f"ms/step {1e3 * wall / args.steps:.3f}, phases {np.round(phases, 3)}, errs {errs}")
         st.close()
@@ -215,8 +243,15 @@ def main():
             ["drift_bbox", "keys_sort", "build", "summarize", "walk"], r["phases_ms"])},
         "accel_error_vs_fp64_direct": r["errors"],
     }
+    line["walk_tune"] = r["tune"]
+    line["graph_ms_per_step"] = r["graph_ms"]
     if "fp64" in results:
         line["fp64_body_steps_per_s"] = results["fp64"]["body_steps_per_s"]
+        line["fp64_phases_ms"] = {k: float(v) for k, v in zip(
+            ["drift_bbox", "keys_sort", "build", "summarize", "walk"],
+            results["fp64"]["phases_ms"])}
+        line["fp64_accel_error_vs_fp64_direct"] = results["fp64"]["errors"]
+        line["fp64_graph_ms_per_step"] = results["fp64"]["graph_ms"]
     if "mixed" in results and "fp64" in results:
         line["mixed_over_fp64"] = (results["mixed"]["body_steps_per_s"]
                                    / results["fp64"]["body_steps_per_s"])

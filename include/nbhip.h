@@ -200,6 +200,25 @@ unsigned nbhip_autotune(nbhip_state* st, nbhip_tune* best, nbhip_tune* trials, i
                         int* ntrials);
 unsigned nbhip_set_tune(nbhip_state* st, const nbhip_tune* tune);
 
+/* ---- adjoint leapfrog ------------------------------------------------------
+ * No counterpart in the reference.  Gradients of a loss on the final state
+ * with respect to the initial positions, velocities, the masses and the
+ * timestep, through nbRunSystem's KDK loop (nbody/src/nbody_plain.c:366-461),
+ * with the tree of each step (cells, membership, opening decisions) held
+ * fixed.  FP64 states only (NBODY_UNSUPPORTED otherwise).
+ *
+ * nbhip_adjoint_forward: from the uploaded state, one force evaluation and
+ * nsteps KDK steps (the same results as nbhip_run), keeping a checkpoint of
+ * the state every `every` steps in device memory.
+ * nbhip_adjoint_backward: given dL/dpos and dL/dvel of the final state
+ * (n x 3 each), recomputes each segment from its checkpoint and runs the
+ * adjoint steps in reverse; writes dL/dpos0, dL/dvel0 (n x 3), dL/dmass (n)
+ * and dL/dtimestep per simulation (nsim); any output may be NULL.  The state
+ * is left as the forward run ended. */
+unsigned nbhip_adjoint_forward(nbhip_state* st, unsigned nsteps, unsigned every);
+unsigned nbhip_adjoint_backward(nbhip_state* st, const double* gpos, const double* gvel,
+                                double* gpos0, double* gvel0, double* gmass, double* gdt);
+
 /* Waits for all work queued by the library. */
 unsigned nbhip_sync(nbhip_state* st);
 

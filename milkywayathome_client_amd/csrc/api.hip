@@ -75,6 +75,18 @@ static hipError_t nbh_alloc(T** p, size_t count)
     return hipMalloc((void**) p, (count > 0 ? count : 1) * sizeof(T));
 }
 
+static void nbh_adj_free(nbhip_state* st)
+{
+    NbhAdjoint* A = st->adj;
+    if (!A) return;
+    void* ptrs[] = { A->ckpt, A->ckpt_tab, A->seg, A->seg_tab, A->fin, A->fin_tab, A->g,
+                     A->gdt, A->gnode };
+    for (void* p : ptrs)
+        if (p) (void) hipFree(p);
+    free(A);
+    st->adj = nullptr;
+}
+
 static void nbh_free_all(nbhip_state* st)
 {
     void* ptrs[] = { st->px,   st->py,     st->pz,    st->vx,      st->vy,     st->vz,
@@ -88,6 +100,7 @@ static void nbh_free_all(nbhip_state* st)
         if (p) (void) hipFree(p);
     if (st->graph_exec) (void) hipGraphExecDestroy(st->graph_exec);
     if (st->graph) (void) hipGraphDestroy(st->graph);
+    nbh_adj_free(st);
     free(st->h_body_off);
     free(st->h_body_n);
     free(st->h_sim_status);
@@ -765,4 +778,192 @@ extern "C" unsigned nbhip_autotune(nbhip_state* st, nbhip_tune* best, nbhip_tune
     if (best) *best = bestt;
     if (ntrials) *ntrials = cnt;
     return NBHIP_SUCCESS;
+}
+
+/* ---------------------------------------------------------------------- */
+/* Adjoint leapfrog with checkpointing                                    */
+/* ---------------------------------------------------------------------- */
+/* state slot <-> live arrays: x y z vx vy vz ax ay az, then the SimTab rows
+ * (root sizes persist across steps, so a recomputed step needs them) */
+static hipError_t nbh_slot_save(nbhip_state* st, double* slot, SimTab* tab)
+{
+    const size_t N = (size_t) st->n;
+    double* src[9] = { st->px, st->py, st->pz, st->vx, st->vy, st->vz, st->ax, st->ay, st->az };
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < 9 && e == hipSuccess && N; ++k)
+        e = hipMemcpyAsync(slot + k * N, src[k], N * sizeof(double), hipMemcpyDeviceToDevice,
+                           st->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(tab, st->sims, st->nsim * sizeof(SimTab), hipMemcpyDeviceToDevice,
+                           st->stream);
+    return e;
+}
+
+static hipError_t nbh_slot_load(nbhip_state* st, const double* slot, const SimTab* tab,
+                                int positions_only)
+{
+    const size_t N = (size_t) st->n;
+    double* dst[9] = { st->px, st->py, st->pz, st->vx, st->vy, st->vz, st->ax, st->ay, st->az };
+    hipError_t e = hipSuccess;
+    const int cnt = positions_only ? 3 : 9;
+    for (int k = 0; k < cnt && e == hipSuccess && N; ++k)
+        e = hipMemcpyAsync(dst[k], slot + k * N, N * sizeof(double), hipMemcpyDeviceToDevice,
+                           st->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(st->sims, tab, st->nsim * sizeof(SimTab), hipMemcpyDeviceToDevice,
+                           st->stream);
+    return e;
+}
+
+/* rebuilds the tree for the positions now in px, py, pz (nothing for Exact) */
+static void nbh_adj_rebuild(nbhip_state* st)
+{
+    if (st->ctx.criterion == NBHIP_EXACT) return;
+    nbh_bbox(st, 0);
+    nbh_build(st, 0);
+}
+
+extern "C" unsigned nbhip_adjoint_forward(nbhip_state* st, unsigned nsteps, unsigned every)
+{
+    if (!st) return NBHIP_USER_ERROR;
+    if (st->ctx.precision != NBHIP_PRECISION_FP64) {
+        nbh_set_error("nbhip_adjoint_forward: the adjoint needs an FP64 state");
+        return NBHIP_UNSUPPORTED;
+    }
+    if (every < 1 || nsteps < 1) {
+        nbh_set_error("nbhip_adjoint_forward: nsteps and every must be >= 1");
+        return NBHIP_USER_ERROR;
+    }
+    if (every > nsteps) every = nsteps;
+    NBH_CHECK(hipSetDevice(st->device));
+    nbh_adj_free(st);
+    NbhAdjoint* A = (NbhAdjoint*) calloc(1, sizeof(NbhAdjoint));
+    st->adj = A;
+    A->nsteps = nsteps;
+    A->every = every;
+    A->nckpt = nsteps / every + 1;
+    const size_t N = (size_t) st->n, ns = (size_t) st->nsim;
+    hipError_t e = hipSuccess;
+#define A_(ptr, cnt) if (e == hipSuccess) e = nbh_alloc(&A->ptr, (cnt))
+    A_(ckpt, 9 * N * A->nckpt);
+    A_(ckpt_tab, ns * A->nckpt);
+    A_(seg, 9 * N * (every + 1));
+    A_(seg_tab, ns * (every + 1));
+    A_(fin, 9 * N);
+    A_(fin_tab, ns);
+    A_(g, 10 * N);
+    A_(gdt, ns);
+    A_(gnode, 10 * (size_t) st->node_cap);
+#undef A_
+    if (e != hipSuccess) {
+        nbh_set_error("adjoint storage for %u checkpoints failed: %s", A->nckpt,
+                      hipGetErrorString(e));
+        nbh_adj_free(st);
+        return NBHIP_GPU_ERROR;
+    }
+    unsigned rc = nbhip_gravity(st);
+    if (NBHIP_STATUS_IS_FATAL(rc)) return rc;
+    NBH_CHECK(nbh_slot_save(st, A->ckpt, A->ckpt_tab));
+    for (unsigned k = 1; k <= nsteps; ++k) {
+        rc |= nbhip_step(st);
+        if (NBHIP_STATUS_IS_FATAL(rc)) return rc;
+        if (k % every == 0)
+            NBH_CHECK(nbh_slot_save(st, A->ckpt + 9 * N * (k / every),
+                                    A->ckpt_tab + ns * (k / every)));
+    }
+    NBH_CHECK(hipStreamSynchronize(st->stream));
+    A->ready = 1;
+    return rc;
+}
+
+extern "C" unsigned nbhip_adjoint_backward(nbhip_state* st, const double* gpos, const double* gvel,
+                                           double* gpos0, double* gvel0, double* gmass,
+                                           double* gdt)
+{
+    if (!st || !gpos || !gvel) return NBHIP_USER_ERROR;
+    NbhAdjoint* A = st->adj;
+    if (!A || !A->ready) {
+        nbh_set_error("nbhip_adjoint_backward: run nbhip_adjoint_forward first");
+        return NBHIP_USER_ERROR;
+    }
+    NBH_CHECK(hipSetDevice(st->device));
+    const size_t N = (size_t) st->n, ns = (size_t) st->nsim;
+    const unsigned every = A->every;
+    const double dt = st->ctx.timestep;
+    /* adjoint of the final state: AoS host -> SoA device */
+    std::vector<double> h(10 * N, 0.0);
+    for (size_t i = 0; i < N; ++i)
+        for (int d = 0; d < 3; ++d) {
+            h[d * N + i] = gpos[3 * i + d];
+            h[(3 + d) * N + i] = gvel[3 * i + d];
+        }
+    NBH_CHECK(hipMemcpyAsync(A->g, h.data(), 10 * N * sizeof(double), hipMemcpyHostToDevice,
+                             st->stream));
+    NBH_CHECK(hipMemsetAsync(A->gdt, 0, ns * sizeof(double), st->stream));
+    NBH_CHECK(nbh_slot_save(st, A->fin, A->fin_tab));
+    const unsigned step_fin = st->step;
+    const int prof = st->profiling;
+    st->profiling = 0;
+
+    unsigned rc = 0;
+    const int nseg = (int) ((A->nsteps + every - 1) / every);
+    for (int sg = nseg - 1; sg >= 0; --sg) {
+        const unsigned s0 = (unsigned) sg * every;
+        const unsigned len = (A->nsteps - s0 < every) ? A->nsteps - s0 : every;
+        /* recompute the segment from its checkpoint, keeping every state */
+        NBH_CHECK(nbh_slot_load(st, A->ckpt + 9 * N * sg, A->ckpt_tab + ns * sg, 0));
+        NBH_CHECK(nbh_slot_save(st, A->seg, A->seg_tab));
+        for (unsigned k = 1; k <= len; ++k) {
+            rc |= nbh_force(st, 1, 1);
+            NBH_CHECK(nbh_slot_save(st, A->seg + 9 * N * k, A->seg_tab + ns * k));
+        }
+        rc |= nbh_collect(st);
+        if (NBHIP_STATUS_IS_FATAL(rc)) break;
+        /* reverse through the segment: state k = x_{n+1}, state k-1 = x_n */
+        for (unsigned k = len; k >= 1; --k) {
+            const double* cur = A->seg + 9 * N * k;
+            const double* prev = A->seg + 9 * N * (k - 1);
+            nbh_adj_kick_end(st, cur + 6 * N);
+            NBH_CHECK(nbh_slot_load(st, cur, A->seg_tab + ns * k, 1));
+            nbh_adj_rebuild(st);
+            nbh_adj_force(st);
+            nbh_adj_body(st, 1, prev + 3 * N, prev + 6 * N);
+        }
+    }
+    if (!NBHIP_STATUS_IS_FATAL(rc)) {
+        /* the force evaluation before the first step */
+        NBH_CHECK(nbh_slot_load(st, A->ckpt, A->ckpt_tab, 1));
+        nbh_adj_rebuild(st);
+        nbh_adj_force(st);
+        nbh_adj_body(st, 0, nullptr, nullptr);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) {
+            nbh_set_error("adjoint launch failed: %s", hipGetErrorString(e));
+            rc |= NBHIP_GPU_ERROR;
+        }
+    }
+    /* the state goes back to the end of the forward run, tree included */
+    NBH_CHECK(nbh_slot_load(st, A->fin, A->fin_tab, 0));
+    nbh_adj_rebuild(st);
+    st->step = step_fin;
+    st->profiling = prof;
+    rc |= nbh_collect(st);
+    if (NBHIP_STATUS_IS_FATAL(rc)) return rc;
+
+    NBH_CHECK(hipMemcpyAsync(h.data(), A->g, 10 * N * sizeof(double), hipMemcpyDeviceToHost,
+                             st->stream));
+    std::vector<double> hdt(ns);
+    NBH_CHECK(hipMemcpyAsync(hdt.data(), A->gdt, ns * sizeof(double), hipMemcpyDeviceToHost,
+                             st->stream));
+    NBH_CHECK(hipStreamSynchronize(st->stream));
+    for (size_t i = 0; i < N; ++i)
+        for (int d = 0; d < 3; ++d) {
+            if (gpos0) gpos0[3 * i + d] = h[d * N + i];
+            if (gvel0) gvel0[3 * i + d] = h[(3 + d) * N + i];
+        }
+    if (gmass)
+        for (size_t i = 0; i < N; ++i) gmass[i] = h[9 * N + i];
+    if (gdt)
+        for (size_t k = 0; k < ns; ++k) gdt[k] = hdt[k];
+    return rc;
 }

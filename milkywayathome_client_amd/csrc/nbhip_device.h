@@ -170,6 +170,90 @@ __device__ __forceinline__ void nbh_ext_accel(const nbhip_potential& pot, double
     oaz = a2 + t2;
 }
 
+
+/* J g with J = d(nbh_ext_accel)/dx at (x, y, z), for the adjoint.  J is
+ * symmetric (the acceleration is minus the gradient of a potential), so
+ * this is also J^T g.  Same components and the same r clamp as
+ * nbh_ext_accel: inside the clamp radius r is a constant.
+ *   Miyamoto-Nagai:  a = -M u / D^1.5, u = (x, y, z (a + zeta) / zeta),
+ *                    D = x^2 + y^2 + (a + zeta)^2, zeta = sqrt(z^2 + b^2),
+ *                    J = -M (diag(1, 1, 1 + a b^2 / zeta^3) / D^1.5 - 3 u u^T / D^2.5)
+ *   logarithmic:     a = k w, w = (x, y, z / q^2), k = -2 v0^2 / D,
+ *                    D = d^2 + x^2 + y^2 + (z/q)^2, J = k diag(1, 1, 1/q^2) - 2 (k/D) w w^T
+ *   radial f(r) x:   J = f I + f'(r) x x^T / r  (NFW, Hernquist, Plummer) */
+__device__ __forceinline__ void nbh_mn_jac(double M, double da, double db, double x, double y,
+                                           double z, double gx, double gy, double gz,
+                                           double& ox, double& oy, double& oz)
+{
+    const double zeta = sqrt(z * z + db * db);
+    const double azp = da + zeta;
+    const double D = x * x + y * y + azp * azp;
+    const double D15 = D * sqrt(D);
+    const double D25 = D15 * D;
+    const double uz = z * azp / zeta;
+    const double ug = x * gx + y * gy + uz * gz;
+    const double czz = 1.0 + da * db * db / (zeta * zeta * zeta);
+    ox += -M * (gx / D15 - 3.0 * x * ug / D25);
+    oy += -M * (gy / D15 - 3.0 * y * ug / D25);
+    oz += -M * (czz * gz / D15 - 3.0 * uz * ug / D25);
+}
+
+__device__ __forceinline__ void nbh_ext_jac(const nbhip_potential& pot, double x, double y,
+                                            double z, double gx, double gy, double gz,
+                                            double& ox, double& oy, double& oz)
+{
+    const double limit = 0.00390625;
+    const double ab = sqrt(z * z + (y * y + x * x));
+    const bool clamp = ab <= limit;
+    const double r = clamp ? limit : ab;
+    const double xg = x * gx + y * gy + z * gz;
+    ox = oy = oz = 0.0;
+    if (pot.disk_type == NBHIP_DISK_MIYAMOTO_NAGAI)
+        nbh_mn_jac(pot.disk_mass, pot.disk_scale_length, pot.disk_scale_height, x, y, z, gx, gy,
+                   gz, ox, oy, oz);
+    if (pot.disk2_type == NBHIP_DISK_MIYAMOTO_NAGAI)
+        nbh_mn_jac(pot.disk2_mass, pot.disk2_scale_length, pot.disk2_scale_height, x, y, z, gx,
+                   gy, gz, ox, oy, oz);
+    double f = 0.0, fp = 0.0;          /* radial terms: f(r) and f'(r) */
+    if (pot.halo_type == NBHIP_HALO_LOGARITHMIC) {
+        const double v0 = pot.halo_vhalo, q = pot.halo_flatten_z, d = pot.halo_scale_length;
+        const double q2 = q * q;
+        const double zq = z / q;
+        const double D = d * d + x * x + y * y + zq * zq;
+        const double k = -2.0 * v0 * v0 / D;
+        const double wz = z / q2;
+        const double wg = x * gx + y * gy + wz * gz;
+        const double c = 2.0 * k / D * wg;
+        ox += k * gx - c * x;
+        oy += k * gy - c * y;
+        oz += k * gz / q2 - c * wz;
+    } else if (pot.halo_type == NBHIP_HALO_NFW) {
+        const double h = pot.halo_scale_length;
+        const double M = pot.halo_vhalo * pot.halo_vhalo * h / 0.2162165954;
+        const double hr = h + r;
+        const double L = log(hr / h);
+        const double r2 = r * r, r3 = r2 * r;
+        f += -M * (L / r3 - 1.0 / (r2 * hr));
+        fp += -M * (1.0 / (r3 * hr) - 3.0 * L / (r3 * r) + (2.0 * h + 3.0 * r) / (r3 * hr * hr));
+    }
+    if (pot.sphere_type == NBHIP_SPHERICAL_HERNQUIST) {
+        const double a = pot.sphere_scale, M = pot.sphere_mass;
+        const double ar = a + r;
+        f += -M / (r * ar * ar);
+        fp += M * (a + 3.0 * r) / (r * r * ar * ar * ar);
+    } else if (pot.sphere_type == NBHIP_SPHERICAL_PLUMMER) {
+        const double b = pot.sphere_scale, M = pot.sphere_mass;
+        const double s2 = b * b + r * r;
+        const double s = sqrt(s2);
+        f += -M / (s2 * s);
+        fp += 3.0 * M * r / (s2 * s2 * s);
+    }
+    const double c = clamp ? 0.0 : fp / r * xg;
+    ox += f * gx + c * x;
+    oy += f * gy + c * y;
+    oz += f * gz + c * z;
+}
+
 /* ---------------------------------------------------------------------- */
 /* wave64 helpers                                                          */
 /* ---------------------------------------------------------------------- */
@@ -192,6 +276,48 @@ __device__ __forceinline__ unsigned long long nbh_wave_sum_u64(unsigned long lon
 }
 
 __device__ __forceinline__ unsigned nbh_wave_sum_u32(unsigned v)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__device__ __forceinline__ int nbh_wave_min_i(int v)
+{
+    for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+__device__ __forceinline__ int nbh_wave_max_i(int v)
+{
+    for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+/* Lane set-up shared by both walks: the lane's simulation, its segment of
+ * the node table and its start position; the wave's first and last node. */
+struct LaneStart {
+    int sim, self, lp, end, q0, qend;
+    double eps2;
+};
+
+__device__ __forceinline__ LaneStart nbh_lane_start(int s, bool valid, int M,
+                                                    const int* __restrict__ simid_s,
+                                                    const SimTab* __restrict__ sims)
+{
+    LaneStart r;
+    r.sim = valid ? simid_s[s] : 0;
+    const SimTab& t = sims[r.sim];
+    const bool live = valid && t.root >= 0;
+    r.self = (valid && s < M) ? s : -2;
+    r.lp = live ? t.root : INT_MAX;
+    r.end = live ? t.end : 0;
+    r.eps2 = t.eps2;
+    r.q0 = __builtin_amdgcn_readfirstlane(nbh_wave_min_i(r.lp));
+    r.qend = __builtin_amdgcn_readfirstlane(nbh_wave_max_i(r.end));
+    return r;
+}
+
+__device__ __forceinline__ double nbh_wave_sum_f64(double v)
 {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     return v;

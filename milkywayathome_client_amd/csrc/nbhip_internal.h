@@ -77,6 +77,23 @@ struct WalkTune {
 #define NBH_SORT_TILE 4096   /* keys per radix-sort block: 256 threads x 16 */
 #define NBH_SCAN_TILE 4096   /* items per scan block: 1024 threads x 4 */
 
+/* Adjoint bookkeeping (FP64 states only): checkpoints of the forward run
+ * every `every` steps, one recomputed segment, and the adjoint vectors.
+ * A state slot holds 9 arrays of n doubles: x y z vx vy vz ax ay az. */
+struct NbhAdjoint {
+    unsigned nsteps, every, nckpt;
+    double* ckpt;                 /* nckpt slots */
+    SimTab* ckpt_tab;             /* nckpt * nsim */
+    double* seg;                  /* every + 1 slots */
+    SimTab* seg_tab;              /* (every + 1) * nsim */
+    double* fin;                  /* final forward state, one slot */
+    SimTab* fin_tab;
+    double* g;                    /* gx gy gz | gvx gvy gvz | gax gay gaz | gm: 10 n */
+    double* gdt;                  /* nsim */
+    double* gnode;                /* 10 * node_cap: d/d(cm x y z, mass, quad[6]) */
+    int ready;
+};
+
 struct nbhip_state {
     nbhip_ctx ctx;
     int device;
@@ -125,6 +142,11 @@ struct nbhip_state {
     hipEvent_t ev[6];
     float phase_ms[5];
 
+    int h_level_count[64];        /* cells per level of the last eager build */
+    int h_levels_valid;
+
+    NbhAdjoint* adj;
+
     WalkTune tune;
     int use_graph;
     hipGraph_t graph;
@@ -139,6 +161,10 @@ void nbh_build(nbhip_state* st, int graph_safe);
 void nbh_walk(nbhip_state* st, int kick, int count, double* out_ax, double* out_ay,
               double* out_az, int self_only, const WalkTune* tune);
 void nbh_direct(nbhip_state* st, int nsample, const int* d_idx, double* d_acc);
+/* launchers (adjoint.hip) */
+void nbh_adj_kick_end(nbhip_state* st, const double* a_next);
+void nbh_adj_force(nbhip_state* st);
+void nbh_adj_body(nbhip_state* st, int drift_kick, const double* v_prev, const double* a_prev);
 
 #define NBH_CHECK(x)                                                         \
     do {                                                                     \

@@ -21,6 +21,8 @@
  * tree is a forest joined by prefix -1 nodes (which own no cell) and each
  * simulation's pre-order table is one contiguous segment.
  */
+#include <cstring>
+
 #include "nbhip_device.h"
 
 #include <cfloat>
@@ -1042,6 +1044,8 @@ void nbh_build(nbhip_state* st, int graph_safe)
                 if (counts[l] > 0) top = l;
         }
     }
+    st->h_levels_valid = have ? 1 : 0;
+    if (have) memcpy(st->h_level_count, counts, sizeof(counts));
     const int gmax = nbh_blocks(n, 256 * 8) + 1;
     for (int l = top; l >= 0; --l) {
         if (have && counts[l] == 0) continue;

@@ -70,42 +70,6 @@ __device__ __forceinline__ void nbh_walk_counters(DevScalars* scal, unsigned nbb
     }
 }
 
-__device__ __forceinline__ int nbh_wave_min_i(int v)
-{
-    for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
-    return v;
-}
-
-__device__ __forceinline__ int nbh_wave_max_i(int v)
-{
-    for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
-    return v;
-}
-
-/* Lane set-up shared by both walks: the lane's simulation, its segment of
- * the node table and its start position; the wave's first and last node. */
-struct LaneStart {
-    int sim, self, lp, end, q0, qend;
-    double eps2;
-};
-
-__device__ __forceinline__ LaneStart nbh_lane_start(int s, bool valid, int M,
-                                                    const int* __restrict__ simid_s,
-                                                    const SimTab* __restrict__ sims)
-{
-    LaneStart r;
-    r.sim = valid ? simid_s[s] : 0;
-    const SimTab& t = sims[r.sim];
-    const bool live = valid && t.root >= 0;
-    r.self = (valid && s < M) ? s : -2;
-    r.lp = live ? t.root : INT_MAX;
-    r.end = live ? t.end : 0;
-    r.eps2 = t.eps2;
-    r.q0 = __builtin_amdgcn_readfirstlane(nbh_wave_min_i(r.lp));
-    r.qend = __builtin_amdgcn_readfirstlane(nbh_wave_max_i(r.end));
-    return r;
-}
-
 /* Plain form: FP64 end to end, reference operation order. */
 template <int BS, bool QUAD, bool COUNT>
 __global__ __launch_bounds__(BS) void k_walk64(const NodeW* __restrict__ node,

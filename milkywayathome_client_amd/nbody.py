@@ -86,6 +86,7 @@ EXPORTED = [
     "nbhip_phase_times", "nbhip_sync", "nbhip_release", "nbhip_last_error",
     "nbhip_batch_create", "nbhip_batch_upload", "nbhip_batch_download", "nbhip_batch_status",
     "nbhip_batch_tree_info", "nbhip_set_graph", "nbhip_autotune", "nbhip_set_tune",
+    "nbhip_adjoint_forward", "nbhip_adjoint_backward",
 ]
 
 _LIB: Optional[C.CDLL] = None
@@ -133,6 +134,8 @@ def load_library() -> C.CDLL:
         "nbhip_autotune": (u, [st, C.POINTER(_Tune), C.POINTER(_Tune), C.c_int,
                                C.POINTER(C.c_int)]),
         "nbhip_set_tune": (u, [st, C.POINTER(_Tune)]),
+        "nbhip_adjoint_forward": (u, [st, C.c_uint, C.c_uint]),
+        "nbhip_adjoint_backward": (u, [st, _dp, _dp, _dp, _dp, _dp, _dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -420,6 +423,24 @@ class NBodyState:
     def set_tune(self, block: int, batch: int = 1024) -> None:
         t = _Tune(block, batch, 0.0)
         self._check(self.lib.nbhip_set_tune(self._h, C.byref(t)))
+
+    def adjoint_forward(self, nsteps: int, every: int = 50) -> int:
+        """Force evaluation + nsteps KDK steps (as run()), checkpointing the
+        state every `every` steps for adjoint_backward."""
+        self.last_status = self._check(self.lib.nbhip_adjoint_forward(self._h, nsteps, every))
+        return self.last_status
+
+    def adjoint_backward(self, gpos, gvel):
+        """dL/d(initial pos, vel, mass, timestep) from dL/d(final pos, vel).
+        Returns (gpos0 [n,3], gvel0 [n,3], gmass [n], gdt [nsim])."""
+        gpos = np.ascontiguousarray(gpos, dtype=np.float64).reshape(self.n, 3)
+        gvel = np.ascontiguousarray(gvel, dtype=np.float64).reshape(self.n, 3)
+        gp0, gv0 = np.empty((self.n, 3)), np.empty((self.n, 3))
+        gm = np.empty(self.n)
+        gdt = np.empty(getattr(self, "nsim", 1))
+        self._check(self.lib.nbhip_adjoint_backward(self._h, _ptr(gpos), _ptr(gvel), _ptr(gp0),
+                                                    _ptr(gv0), _ptr(gm), _ptr(gdt)))
+        return gp0, gv0, gm, gdt
 
     def set_profiling(self, enable: bool = True) -> None:
         self._check(self.lib.nbhip_set_profiling(self._h, int(enable)))
