@@ -47,6 +47,7 @@ import torch
 
 from . import models
 from . import nbody as nb
+from .histogram import HistogramParams
 
 F64 = torch.float64
 
@@ -230,38 +231,20 @@ def dwarf_ic(p: torch.Tensor, draws: DwarfDraws, centre_pos, centre_vel):
 # ---------------------------------------------------------------------------
 # Smoothed lambda histogram
 # ---------------------------------------------------------------------------
-@dataclass
-class HistogramSpec:
-    phi: float = 128.79
-    theta: float = 54.39
-    psi: float = 90.70
-    lambda_start: float = -150.0
-    lambda_end: float = 150.0
-    bins: int = 50
-    sun_gc_dist: float = 8.0
-
-    def rotation(self):
-        f, t, s = (math.radians(a) for a in (self.phi, self.theta, self.psi))
-        cf, sf, ct, st, cs, ss = math.cos(f), math.sin(f), math.cos(t), math.sin(t), \
-            math.cos(s), math.sin(s)
-        return ((cs * cf - ct * sf * ss, cs * sf + ct * cf * ss, ss * st),
-                (-ss * cf - ct * sf * cs, -ss * sf + ct * cf * cs, cs * st))
-
-
-def lambda_deg(x: torch.Tensor, spec: HistogramSpec) -> torch.Tensor:
+def lambda_deg(x: torch.Tensor, spec: HistogramParams) -> torch.Tensor:
     """nbXYZToLambdaBeta's lambda (nbody/src/nbody_coordinates.c:129-164)."""
-    (r11, r12, r13), (r21, r22, r23) = spec.rotation()
-    X = x[:, 0] + spec.sun_gc_dist
+    (r11, r12, r13), (r21, r22, r23) = spec.rotation()[:2].tolist()
+    X = x[:, 0] + spec.sunGCDist
     tx = r11 * X + r12 * x[:, 1] + r13 * x[:, 2]
     ty = r21 * X + r22 * x[:, 1] + r23 * x[:, 2]
     return torch.rad2deg(torch.atan2(ty, tx))
 
 
-def smoothed_histogram(lam: torch.Tensor, spec: HistogramSpec, sigma_bins: float = 0.5):
+def smoothed_histogram(lam: torch.Tensor, spec: HistogramParams, sigma_bins: float = 0.5):
     """Each body spreads a Gaussian (sigma = sigma_bins bin widths) over the
-    bins; normalised by the body count."""
-    w = (spec.lambda_end - spec.lambda_start) / spec.bins
-    edges = spec.lambda_start + w * torch.arange(spec.bins + 1, dtype=F64)
+    lambda bins; normalised by the body count."""
+    w = (spec.lambdaEnd - spec.lambdaStart) / spec.lambdaBins
+    edges = spec.lambdaStart + w * torch.arange(spec.lambdaBins + 1, dtype=F64)
     z = (edges[None, :] - lam[:, None]) / (sigma_bins * w)
     cdf = torch.special.ndtr(z)
     return (cdf[:, 1:] - cdf[:, :-1]).sum(0) / lam.shape[0]
@@ -280,7 +263,7 @@ class StreamModel:
     dt_ref: float = 1e-3
     orbit_pos: tuple = (218.0, 53.5, 28.6)      # present-day (l, b, r)
     orbit_vel: tuple = (-156.0, 79.0, 107.0)
-    spec: HistogramSpec = dataclasses.field(default_factory=HistogramSpec)
+    spec: HistogramParams = dataclasses.field(default_factory=HistogramParams)
 
     @property
     def orbit_steps(self) -> int:

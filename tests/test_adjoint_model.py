@@ -40,31 +40,30 @@ def test_torch_ext_accel_equals_c():
 
 
 def test_lambda_frame():
-    spec = adj.HistogramSpec()
-    R = np.array(spec.rotation())
-    np.testing.assert_allclose(R @ R.T, np.eye(2), atol=1e-15)      # orthonormal rows
+    spec = adj.HistogramParams()
+    R = spec.rotation()
+    np.testing.assert_allclose(R @ R.T, np.eye(3), atol=1e-15)      # orthonormal
     lam0 = np.array([-120.0, -30.0, 0.0, 45.0, 170.0])
-    third = np.cross(R[0], R[1])
     pts = []
     for l in np.radians(lam0):
-        helio = 20.0 * (math.cos(l) * R[0] + math.sin(l) * R[1]) + 3.0 * third
-        pts.append(helio - np.array([spec.sun_gc_dist, 0.0, 0.0]))
+        helio = 20.0 * (math.cos(l) * R[0] + math.sin(l) * R[1]) + 3.0 * R[2]
+        pts.append(helio - np.array([spec.sunGCDist, 0.0, 0.0]))
     got = adj.lambda_deg(torch.tensor(np.array(pts), dtype=F64), spec).numpy()
     np.testing.assert_allclose(got, lam0, atol=1e-12)
 
 
 def test_smoothed_histogram_limits():
-    spec = adj.HistogramSpec()
+    spec = adj.HistogramParams()
     rng = np.random.default_rng(2)
     lam = rng.uniform(-140.0, 140.0, size=5000)
-    w = (spec.lambda_end - spec.lambda_start) / spec.bins
+    w = (spec.lambdaEnd - spec.lambdaStart) / spec.lambdaBins
     lt = torch.tensor(lam, dtype=F64)
     sharp = adj.smoothed_histogram(lt, spec, sigma_bins=1e-9).numpy()
-    plain = np.histogram(lam, bins=spec.bins, range=(spec.lambda_start, spec.lambda_end))[0]
+    plain = np.histogram(lam, bins=spec.lambdaBins, range=(spec.lambdaStart, spec.lambdaEnd))[0]
     np.testing.assert_allclose(sharp, plain / lam.size, atol=1e-12)
     smooth = adj.smoothed_histogram(lt, spec).numpy()
-    inside = np.mean([0.5 * (math.erf((spec.lambda_end - l) / (0.5 * w * math.sqrt(2)))
-                             - math.erf((spec.lambda_start - l) / (0.5 * w * math.sqrt(2))))
+    inside = np.mean([0.5 * (math.erf((spec.lambdaEnd - l) / (0.5 * w * math.sqrt(2)))
+                             - math.erf((spec.lambdaStart - l) / (0.5 * w * math.sqrt(2))))
                       for l in lam])
     assert abs(smooth.sum() - inside) < 1e-12
 
