@@ -139,6 +139,174 @@ def cpu_baseline(pot_ctx_args, pos, vel, m, threads, max_bodies):
                       f"{t_pre:.2f} s pre-step force"}
 
 
+def make_batch_workload(ndwarf: int, seed: int = 7, dt: float = 1e-3, t_forward: float = 2.0):
+    """64 independent two-Plummer dwarfs drawn from `seed` with dSFMT:
+    N ~ U(2000, 40000) (half baryons), baryon scale radius ~ U(0.1, 0.5),
+    radius ratio ~ U(0.1, 0.5), baryon mass ~ U(1, 30), mass ratio ~
+    U(0.01, 0.95); dark component M_d = M_b (1 - mr) / mr, a_d = a_b (1 - rr)
+    / rr (nbody/sample_workunits/for_developers.lua:469-479).  Each is placed
+    on the orbit reversed t_forward Gyr; eps2 from calculateEps2 per dwarf."""
+    from milkywayathome_client_amd import dwarf, models, nbody as nb
+    prng = models.DSFMT(seed)
+    u = prng.draw(5 * ndwarf, "close_open").reshape(ndwarf, 5)
+    pot = nb.milky_way_potential()
+    x0 = models.lbr_to_cartesian([218.0, 53.5, 28.6])
+    fpos, fvel = models.reverse_orbit(pot, x0, [-156.0, 79.0, 107.0], t_forward, dt / 10.0)
+    members, eps2, params = [], [], []
+    for k in range(ndwarf):
+        n = int(2000 + u[k, 0] * 38000)
+        a_b = 0.1 + 0.4 * u[k, 1]
+        rr = 0.1 + 0.4 * u[k, 2]
+        m_b = 1.0 + 29.0 * u[k, 3]
+        mr = 0.01 + 0.94 * u[k, 4]
+        m_d = m_b / mr * (1.0 - mr)
+        a_d = a_b / rr * (1.0 - rr)
+        pos, vel, m, typ = dwarf.two_component_dwarf(n, seed=seed + 1 + k, mass_l=m_b,
+                                                     rscale_l=a_b, mass_d=m_d, rscale_d=a_d,
+                                                     dark="plummer", position=fpos,
+                                                     velocity=fvel)
+        members.append((pos, vel, m))
+        eps2.append(models.calculate_eps2(n, a_b, a_d, m_b, m_d))
+        params.append((n, a_b, rr, m_b, mr))
+    return pot, members, eps2, params
+
+
+def run_batch(args):
+    """Batched dwarfs: graph-captured steps of all dwarfs in one state versus
+    the same dwarfs run one after another with the plain path."""
+    from milkywayathome_client_amd import histogram as H, nbody as nb
+    dt = 1e-3
+    log(f"[bench] building {args.dwarfs} dwarfs")
+    pot, members, eps2, params = make_batch_workload(args.dwarfs)
+    ntot = sum(p[0] for p in params)
+    ctx = nb.NBodyCtx.create(timestep=dt, timeEvolve=2.0, eps2=1e-4, theta=args.theta,
+                             criterion="BH86", useQuad=True, treeRSize=4.0)
+    bt = nb.NBodyBatch(ctx, pot, members, eps2=eps2, precision=args.precision_batch)
+    best, trials = bt.autotune() if not args.no_autotune else ((None, None, None), [])
+    bt.set_graph(True)
+    bt.steps(args.warmup)
+    bt.sync()
+    t0 = time.perf_counter()
+    bt.steps(args.steps)
+    bt.sync()
+    wall = time.perf_counter() - t0
+    status = bt.statuses()
+    log(f"[bench] batch: {ntot} bodies, {1e3 * wall / args.steps:.3f} ms/step, status {set(status)}")
+
+    # sequential plain path: every dwarf alone, eager steps
+    seq_steps = max(1, args.seq_steps)
+    seq_time = 0.0
+    hp = H.HistogramParams()
+    emds = []
+    for k, (pos, vel, m) in enumerate(members):
+        ck = nb.NBodyCtx.create(timestep=dt, timeEvolve=2.0, eps2=eps2[k], theta=args.theta,
+                                criterion="BH86", useQuad=True, treeRSize=4.0)
+        st = nb.NBodyState(ck, pot, pos, vel, m, precision=args.precision_batch)
+        if best[0] is not None:
+            st.set_tune(best[0], best[1])     # same partial-sum blocks as the batch
+        st.steps(1)
+        st.sync()
+        t0 = time.perf_counter()
+        st.steps(seq_steps)
+        st.sync()
+        seq_time += (time.perf_counter() - t0) / seq_steps
+        if k < args.emd_dwarfs:
+            st.steps(args.warmup + args.steps - 1 - seq_steps)
+            p1 = st.bodies()[0]
+            pb = bt.sim_bodies(k)[0]
+            nbar = params[k][0] // 2
+            h1, _ = H.create_histogram(p1[:nbar], hp)
+            hb, _ = H.create_histogram(pb[:nbar], hp)
+            emds.append(H.emd(hb, h1, hp))
+        st.close()
+    line = {
+        "metric": "aggregate body-steps/s, 64 independent dwarfs per launch (hipGraph steps)",
+        "value": ntot * args.steps / wall,
+        "unit": "body-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / args.steps,
+        "higher_is_better": True,
+        "dtype": "f64" if args.precision_batch == "fp64" else "f32 walk + f64 state",
+        "data": "synthetic (two-Plummer dwarfs sampled in-process, parameters from dSFMT seed 7)",
+        "config": {"workload": f"{args.dwarfs} two-component dwarfs, N~U(2000,40000), "
+                               "BH86 theta=0.5 + quadrupoles, MW potential, dt=1e-3",
+                   "total_bodies": ntot, "dwarfs": args.dwarfs},
+        "sequential_plain_body_steps_per_s": ntot / seq_time,
+        "batched_over_sequential": (ntot * args.steps / wall) / (ntot / seq_time),
+        "walk_tune": {"block": best[0], "batch": best[1], "walk_ms": best[2]},
+        "emd_batched_vs_single": emds,
+        "status": sorted(set(status)),
+    }
+    bt.close()
+    print(json.dumps(line), flush=True)
+
+
+def run_adjoint(args):
+    """Differentiation workload: two-Plummer dwarf of 20 000 bodies, 1000 KDK
+    steps (dt = 1e-3) in the MW potential, smoothed lambda-histogram loss
+    against the model at p* = (1, 1, 0.2, 0.2, 12, 0.2), evaluated at
+    p* + N(0, 0.02 |p*|); gradient of the 6 parameters by the adjoint
+    (checkpoints every 50 steps) against central differences."""
+    import torch
+    from milkywayathome_client_amd import adjoint as adj
+    p_star = np.array([1.0, 1.0, 0.2, 0.2, 12.0, 0.2])
+    model = adj.make_model(n=args.adj_n, p_ref=p_star, criterion=args.adj_criterion,
+                           every=50, theta=args.theta)
+    with torch.no_grad():
+        target = model.histogram(torch.tensor(p_star), grad=False)
+    rng = np.random.default_rng(11)
+    p = p_star + rng.normal(size=6) * 0.02 * np.abs(p_star)
+    # time the N-body part alone: plain forward vs checkpointed forward + backward
+    x, v, m, dt = (t.detach() for t in model.initial_state(torch.tensor(p)))
+    t0 = time.perf_counter()
+    adj.evolve(x.numpy(), v.numpy(), m.numpy(), float(dt), model.cfg)
+    t_fwd = time.perf_counter() - t0
+    xr, vr = x.clone().requires_grad_(True), v.clone().requires_grad_(True)
+    t0 = time.perf_counter()
+    xN, vN = adj.KDKRun.apply(xr, vr, m, dt, model.cfg)
+    t_afwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    (xN.sum() + vN.sum()).backward()
+    t_bwd = time.perf_counter() - t0
+    log(f"[bench] adjoint: forward {t_fwd:.2f}s, checkpointed forward {t_afwd:.2f}s, "
+        f"backward {t_bwd:.2f}s")
+    t0 = time.perf_counter()
+    L, g = model.value_and_grad(p, target)
+    t_grad = time.perf_counter() - t0
+    fd = np.zeros(6)
+    for k in range(6):
+        h = 1e-5 * abs(p[k])
+        a, b = p.copy(), p.copy()
+        a[k] += h
+        b[k] -= h
+        with torch.no_grad():
+            fd[k] = (float(model.loss(torch.tensor(a), target, grad=False))
+                     - float(model.loss(torch.tensor(b), target, grad=False))) / (2 * h)
+    n, steps = args.adj_n, model.cfg.nsteps
+    line = {
+        "metric": "adjoint body-steps/s (checkpointed forward + reverse sweep)",
+        "value": n * steps / (t_afwd + t_bwd),
+        "unit": "body-steps/s",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "dtype": "f64",
+        "data": "synthetic (two-Plummer dwarf from fixed unit draws)",
+        "config": {"workload": f"two-Plummer dwarf N={n}, {steps} KDK steps dt=1e-3, MW "
+                               f"potential, {args.adj_criterion} "
+                               f"{'theta=%g' % args.theta if args.adj_criterion != 'exact' else ''}"
+                               ", checkpoints every 50 steps",
+                   "p": p.tolist()},
+        "forward_s": t_fwd, "checkpointed_forward_s": t_afwd, "backward_s": t_bwd,
+        "forward_body_steps_per_s": n * steps / t_fwd,
+        "gradient_s_with_model": t_grad,
+        "loss": L, "grad_adjoint": g.tolist(), "grad_central_h1e-5": fd.tolist(),
+        "rel_err": (np.abs(g - fd) / np.abs(fd)).tolist(),
+    }
+    print(json.dumps(line), flush=True)
+
+
 def load_traffic():
     path = os.path.join(ROOT, "profiles", "walk_traffic.json")
     if not os.path.exists(path):
@@ -155,7 +323,9 @@ def main():
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--seed", type=int, default=34086709)
     ap.add_argument("--theta", type=float, default=0.5)
-    ap.add_argument("--precision", choices=["both", "fp64", "mixed"], default="both")
+    ap.add_argument("--precision", choices=["all", "both", "fp64", "mixed", "mixed_coherent"],
+                    default="all",
+                    help="all: fp64, mixed and mixed_coherent; both: fp64 and mixed")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default walk settings instead of timing the candidates")
@@ -166,7 +336,24 @@ def main():
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run a plain device-to-device copy of BYTES (profiler "
                          "calibration of FETCH_SIZE / WRITE_SIZE) and exit")
+    ap.add_argument("--workload", choices=["single", "batch", "adjoint"], default="single",
+                    help="single: the N=1M dwarf (the headline metric); batch: many "
+                         "independent dwarfs per launch; adjoint: the gradient workload")
+    ap.add_argument("--adj-n", type=int, default=20000)
+    ap.add_argument("--adj-criterion", choices=["bh86", "exact"], default="exact")
+    ap.add_argument("--dwarfs", type=int, default=64)
+    ap.add_argument("--seq-steps", type=int, default=5,
+                    help="batch workload: timed steps per dwarf on the sequential plain path")
+    ap.add_argument("--emd-dwarfs", type=int, default=4,
+                    help="batch workload: dwarfs whose histogram is compared with a single run")
+    ap.add_argument("--precision-batch", choices=["fp64", "mixed"], default="fp64")
     args = ap.parse_args()
+    if args.workload != "single":
+        if args.gpus != 1:
+            print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
+            sys.exit(2)
+        (run_batch if args.workload == "batch" else run_adjoint)(args)
+        return
     if args.calibrate_copy:
         import torch
         n = args.calibrate_copy // 8
@@ -188,7 +375,8 @@ def main():
 
     from milkywayathome_client_amd import nbody as nb
     results = {}
-    precs = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
+    precs = {"all": ["fp64", "mixed", "mixed_coherent"],
+             "both": ["fp64", "mixed"]}.get(args.precision, [args.precision])
     for prec in precs:
         log(f"[bench] {prec}: warmup {args.warmup}, timed {args.steps}")
         try:
@@ -209,12 +397,14 @@ def main():
     if not results:
         print(json.dumps({"metric": METRIC, "error": "no GPU result (see stderr)"}))
         sys.exit(1)
-    head = "mixed" if "mixed" in results else "fp64"
+    mixed_modes = [k for k in ("mixed", "mixed_coherent") if k in results]
+    head = (max(mixed_modes, key=lambda k: results[k]["body_steps_per_s"]) if mixed_modes
+            else "fp64")
     r = results[head]
     walk_ms = float(r["phases_ms"][4])
     flops = walk_flops(r["counts"])
     achieved = flops / (walk_ms * 1e-3) / 1e12
-    peak = PEAK["fp32" if head == "mixed" else "fp64"]
+    peak = PEAK["fp64" if head == "fp64" else "fp32"]
     traffic = load_traffic()
     line = {
         "metric": METRIC,
@@ -227,7 +417,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 walk + f64 state" if head == "mixed" else "f64",
+        "dtype": "f64" if head == "fp64" else "f32 walk + f64 state",
         "data": "synthetic (two-component dwarf ICs sampled in-process)",
         "config": {"workload": "two-component dwarf N=1M (500k Plummer 12/0.2 + 500k NFW 60/0.8) "
                                "on its orbit in the MW potential, BH86 theta=0.5 + quadrupoles",
@@ -236,7 +426,8 @@ def main():
         "roofline": {"bound": "valu", "kernel": "k_walk (tree walk + potential + kick)",
                      "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak,
-                     "traffic": None if traffic is None else traffic.get("bytes_per_launch"),
+                     "traffic": None if traffic is None else
+                     traffic.get("walk", {}).get(head, {}).get("bytes_per_launch"),
                      "flops_per_launch": flops, "walk_ms": walk_ms,
                      "lane_utilisation": sum(r["counts"][:3]) / (64.0 * max(r["counts"][3], 1))},
         "phases_ms": {k: float(v) for k, v in zip(
@@ -252,9 +443,13 @@ def main():
             results["fp64"]["phases_ms"])}
         line["fp64_accel_error_vs_fp64_direct"] = results["fp64"]["errors"]
         line["fp64_graph_ms_per_step"] = results["fp64"]["graph_ms"]
-    if "mixed" in results and "fp64" in results:
-        line["mixed_over_fp64"] = (results["mixed"]["body_steps_per_s"]
-                                   / results["fp64"]["body_steps_per_s"])
+    if head != "fp64" and "fp64" in results:
+        line["mixed_over_fp64"] = r["body_steps_per_s"] / results["fp64"]["body_steps_per_s"]
+    for k in mixed_modes:
+        if k != head:
+            line[f"{k}_body_steps_per_s"] = results[k]["body_steps_per_s"]
+            line[f"{k}_accel_error_vs_fp64_direct"] = results[k]["errors"]
+            line[f"{k}_walk_ms"] = float(results[k]["phases_ms"][4])
     if not args.no_cpu:
         log(f"[bench] CPU baseline with {args.cpu_threads} threads")
         line["cpu_baseline"] = cpu_baseline((pot, eps2, dt, args.theta), pos, vel, m,

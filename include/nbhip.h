@@ -57,6 +57,10 @@ extern "C" {
 /* ---- walk arithmetic ---- */
 #define NBHIP_PRECISION_FP64  0   /* plain form: FP64 end to end */
 #define NBHIP_PRECISION_MIXED 1   /* FP32 walk, FP64 cross-batch sums, FP64 state */
+/* as MIXED, with wave-coherent opening: a wavefront of 64 consecutive bodies
+ * opens a cell when any of them must, so each body's interaction list
+ * refines its reference list (more accurate, not the reference's lists) */
+#define NBHIP_PRECISION_MIXED_COHERENT 2
 
 /* The parts of Potential (nbody/include/nbody_potential_types.h) that the
  * supported components read. */

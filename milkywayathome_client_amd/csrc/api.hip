@@ -62,7 +62,8 @@ static unsigned nbh_validate(const nbhip_ctx* c)
             return NBHIP_UNSUPPORTED;
         }
     }
-    if (c->precision != NBHIP_PRECISION_FP64 && c->precision != NBHIP_PRECISION_MIXED) {
+    if (c->precision != NBHIP_PRECISION_FP64 && c->precision != NBHIP_PRECISION_MIXED &&
+        c->precision != NBHIP_PRECISION_MIXED_COHERENT) {
         nbh_set_error("invalid precision %d", c->precision);
         return NBHIP_USER_ERROR;
     }
@@ -192,7 +193,7 @@ static unsigned nbh_create(const nbhip_ctx* ctx, int device, int nsim, const int
     A(blo, n); A(bhi, n); A(bdelta, n); A(bparent, 2 * n);
     A(node, nc); A(nquad, 6 * nc); A(nmid, 3 * nc); A(nlevel, nc); A(nparent, nc);
     A(nsim_node, nc); A(level_list, nc);
-    if (ctx->precision == NBHIP_PRECISION_MIXED) A(nodef, nc);
+    if (ctx->precision != NBHIP_PRECISION_FP64) A(nodef, nc);
     A(scal, 1);
     A(sims, (size_t) nsim);
 #undef A
@@ -735,7 +736,7 @@ extern "C" unsigned nbhip_autotune(nbhip_state* st, nbhip_tune* best, nbhip_tune
     NBH_CHECK(hipMalloc(&t, 3 * n * sizeof(double)));
     const int blocks[3] = { 64, 128, 256 };
     const int batches_mixed[4] = { 256, 1024, 4096, 16384 };
-    const bool mixed = st->ctx.precision == NBHIP_PRECISION_MIXED;
+    const bool mixed = st->ctx.precision != NBHIP_PRECISION_FP64;
     const int nb = mixed ? 4 : 1;
     nbhip_tune bestt = { st->tune.block, st->tune.batch, 1e300 };
     int cnt = 0;

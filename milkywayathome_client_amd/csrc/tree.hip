@@ -31,7 +31,19 @@
 /* bounding box (+ optional opening half-kick and drift)                  */
 /* ---------------------------------------------------------------------- */
 /* advancePosVel (nbody/src/nbody_plain.c:268-284) fused with expandBox's
- * max-|x| scan (nbody/src/nbody_tree.c:156-176), per simulation. */
+ * max-|x| scan (nbody/src/nbody_tree.c:156-176), per simulation.  Each
+ * block owns a contiguous chunk of NBH_BBOX_CHUNK bodies (coalesced, 256
+ * apart per thread); a thread flushes its running maximum only when its
+ * simulation changes, and waves then the block combine theirs, so a launch
+ * issues about one atomic per block and simulation instead of one per wave.
+ * A NaN coordinate counts as infinite so the root-size step reports it. */
+#define NBH_BBOX_CHUNK (256 * 16)
+
+__device__ __forceinline__ void nbh_max_bits(SimTab* sims, int sim, double v)
+{
+    atomicMax(&sims[sim].xyzmax_bits, (unsigned long long) __double_as_longlong(v));
+}
+
 __global__ __launch_bounds__(256) void k_drift_bbox(int n, int drift, double dt, double dtHalf,
                                                     double* __restrict__ px, double* __restrict__ py,
                                                     double* __restrict__ pz, double* __restrict__ vx,
@@ -42,11 +54,13 @@ __global__ __launch_bounds__(256) void k_drift_bbox(int n, int drift, double dt,
                                                     const int* __restrict__ simid,
                                                     SimTab* __restrict__ sims)
 {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    const bool valid = i < n;
+    __shared__ int s_sim[4];
+    __shared__ double s_mx[4];
+    const int lo = blockIdx.x * NBH_BBOX_CHUNK;
+    const int hi = min(n, lo + NBH_BBOX_CHUNK);
+    int cur = -1;
     double mx = 0.0;
-    int sim = valid ? simid[i] : 0;
-    if (valid) {
+    for (int i = lo + (int) threadIdx.x; i < hi; i += 256) {
         double x = px[i], y = py[i], z = pz[i];
         if (drift) {
             double ux = vx[i] + dtHalf * (ax[i] + 0.0);
@@ -58,18 +72,45 @@ __global__ __launch_bounds__(256) void k_drift_bbox(int n, int drift, double dt,
             vx[i] = ux; vy[i] = uy; vz[i] = uz;
             px[i] = x; py[i] = y; pz[i] = z;
         }
-        mx = fmax(mx, fabs(x));
-        mx = fmax(mx, fabs(y));
-        mx = fmax(mx, fabs(z));
+        const int sim = simid[i];
+        if (sim != cur) {
+            if (cur >= 0) nbh_max_bits(sims, cur, mx);
+            cur = sim;
+            mx = 0.0;
+        }
+        double a = fmax(fabs(x), fmax(fabs(y), fabs(z)));
+        if (x != x || y != y || z != z) a = INFINITY;
+        mx = fmax(mx, a);
     }
-    const int sim0 = __builtin_amdgcn_readfirstlane(sim);
-    if (!valid) sim = sim0;
-    if (__ballot(sim != sim0) == 0ull) {
-        mx = nbh_wave_max(mx);
-        if (__lane_id() == 0 && __ballot(valid) != 0ull)
-            atomicMax(&sims[sim0].xyzmax_bits, (unsigned long long) __double_as_longlong(mx));
-    } else if (valid) {
-        atomicMax(&sims[sim].xyzmax_bits, (unsigned long long) __double_as_longlong(mx));
+    /* wave: one simulation among the lanes that hold one -> combine */
+    const bool has = cur >= 0;
+    const int w = threadIdx.x >> 6;
+    const int sim_l0 = __builtin_amdgcn_readfirstlane(cur);
+    const bool uniform = __ballot(has && cur != sim_l0) == 0ull;
+    if (uniform) {
+        const double m = nbh_wave_max(has ? mx : 0.0);
+        if (__lane_id() == 0) {
+            s_sim[w] = has ? cur : -1;
+            s_mx[w] = m;
+        }
+    } else {
+        if (has) nbh_max_bits(sims, cur, mx);
+        if (__lane_id() == 0) s_sim[w] = -1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int sim = -1;
+        double m = 0.0;
+        for (int k = 0; k < 4; ++k) {
+            if (s_sim[k] < 0) continue;
+            if (s_sim[k] != sim) {
+                if (sim >= 0) nbh_max_bits(sims, sim, m);
+                sim = s_sim[k];
+                m = 0.0;
+            }
+            m = fmax(m, s_mx[k]);
+        }
+        if (sim >= 0) nbh_max_bits(sims, sim, m);
     }
 }
 
@@ -964,7 +1005,8 @@ static inline int nbh_blocks(int n, int per) { return n > 0 ? (n + per - 1) / pe
 void nbh_bbox(nbhip_state* st, int drift)
 {
     const double dt = st->ctx.timestep;
-    hipLaunchKernelGGL(k_drift_bbox, dim3(nbh_blocks(st->n, 256)), dim3(256), 0, st->stream,
+    hipLaunchKernelGGL(k_drift_bbox, dim3(nbh_blocks(st->n, NBH_BBOX_CHUNK)), dim3(256), 0,
+                       st->stream,
                        st->n, drift, dt, 0.5 * dt, st->px, st->py, st->pz, st->vx, st->vy, st->vz,
                        st->ax, st->ay, st->az, st->simid, st->sims);
     hipLaunchKernelGGL(k_rsize, dim3(1), dim3(256), 0, st->stream, st->nsim, st->sims, st->scal);

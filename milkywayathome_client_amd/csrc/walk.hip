@@ -110,25 +110,26 @@ __global__ __launch_bounds__(BS) void k_walk64(const NodeW* __restrict__ node,
     while (q < L.qend) {
         if (COUNT) ++steps;
         const int nd_next = __builtin_amdgcn_readfirstlane(nd.next);
+        const int nd_body = __builtin_amdgcn_readfirstlane(nd.body);
         const bool active = lp == q;
         const double dx = nd.x - p.x;
         const double dy = nd.y - p.y;
         const double dz = nd.z - p.z;
         double drSq = dz * dz + (dy * dy + dx * dx);
-        const bool accept = active && (nd.body >= 0 || drSq >= nd.rcrit2);
+        const bool accept = active && (nd_body >= 0 || drSq >= nd.rcrit2);
         const bool open = active && !accept;
         const bool any_open = __builtin_amdgcn_ballot_w64(open) != 0ull;
-        const int qn = any_open ? q + 1 : nd_next;
+        int qn = any_open ? q + 1 : nd_next;
         if (qn <= q) {
             /* a Next() link that does not advance: corrupt table, stop */
             bad = true;
-            break;
+            qn = L.qend;
         }
         const NodeW nxt = node[min(qn, L.qend - 1)];
         if (accept) {
             /* with eps2 > 0 the self term adds exactly +0 (dx = dy = dz = 0),
              * so lanes need not branch around it */
-            if (eps2_pos || nd.body != self) {
+            if (eps2_pos || nd_body != self) {
                 drSq += eps2;
                 const double drab = sqrt(drSq);
                 const double phii = nd.m / drab;
@@ -136,7 +137,7 @@ __global__ __launch_bounds__(BS) void k_walk64(const NodeW* __restrict__ node,
                 ax += mor3 * dx;
                 ay += mor3 * dy;
                 az += mor3 * dz;
-                if (QUAD && nd.body < 0) {
+                if (QUAD && nd_body < 0) {
                     const double* Q = nquad + 6 * q;
                     const double qxx = Q[0], qxy = Q[1], qxz = Q[2];
                     const double qyy = Q[3], qyz = Q[4], qzz = Q[5];
@@ -154,10 +155,10 @@ __global__ __launch_bounds__(BS) void k_walk64(const NodeW* __restrict__ node,
                     az -= dr5inv * qz;
                 }
                 if (COUNT) {
-                    if (nd.body >= 0) nbb += nd.body != self; else ++nbc;
+                    if (nd_body >= 0) nbb += nd_body != self; else ++nbc;
                 }
             }
-            met |= nd.body == self;
+            met |= nd_body == self;
             lp = nd_next;
         } else if (open) {
             lp = q + 1;
@@ -175,9 +176,23 @@ __global__ __launch_bounds__(BS) void k_walk64(const NodeW* __restrict__ node,
     nbh_walk_finish(o, p.x, p.y, p.z, ax, ay, az, perm[s], oax, oay, oaz, vx, vy, vz);
 }
 
-/* Mixed form: displacements and interactions in FP32, FP32 partial sums
- * over batches of o.batch node steps, batches added into FP64 with Kahan
- * compensation.  Positions, potential and kicks stay FP64. */
+/* Mixed form: displacements and interactions in FP32; FP32 partial sums
+ * per block of 2^batch_shift pre-order node indices (counted from the
+ * body's own simulation root), added into FP64 with Kahan compensation.
+ * Positions, potential and kicks stay FP64.
+ *
+ * The walk is two nested loops: the inner one streams nodes until the wave
+ * reaches the nearest block boundary of any of its lanes (q only grows), the
+ * outer one flushes the lanes whose own block changed.  A lane's partial sum
+ * therefore covers exactly its interactions inside one of its blocks -- a
+ * property of its own interaction list, not of the wave -- and the FP64
+ * state is not touched in the inner loop.
+ *
+ * COH selects the opening rule: false = each lane's own reference decision
+ * (the FP64 walk's interaction lists); true = wave-coherent, the wave opens
+ * a node when any lane's criterion says open and otherwise every lane of the
+ * node's simulation interacts with it (each lane's list then refines its
+ * reference list: every accepted node passes that lane's own test). */
 __device__ __forceinline__ void nbh_kahan_add(double& sum, double& comp, double v)
 {
     const double y = v - comp;
@@ -186,7 +201,7 @@ __device__ __forceinline__ void nbh_kahan_add(double& sum, double& comp, double 
     sum = t;
 }
 
-template <int BS, bool QUAD, bool COUNT>
+template <int BS, bool QUAD, bool COUNT, bool COH>
 __global__ __launch_bounds__(BS) void k_walk32(const NodeF* __restrict__ node,
                                                const NodeW* __restrict__ nodew,
                                                const double4* __restrict__ spos,
@@ -208,7 +223,7 @@ __global__ __launch_bounds__(BS) void k_walk32(const NodeF* __restrict__ node,
     if (__ballot(valid) == 0ull) return;
     const double4 p = valid ? spos[s] : make_double4(0.0, 0.0, 0.0, 0.0);
     const LaneStart L = nbh_lane_start(s, valid, M, simid_s, sims);
-    const int self = L.self, my_end = L.end;
+    const int self = L.self, my_end = L.end, my_root = L.lp;
     int lp = L.lp;
     /* offsets from the simulation's root centre of mass, split hi + lo */
     const int root = sims[L.sim].root;
@@ -221,6 +236,10 @@ __global__ __launch_bounds__(BS) void k_walk32(const NodeF* __restrict__ node,
     const float eps2 = (float) L.eps2;
     const bool eps2_pos = o.all_eps2_pos;
     const int bshift = o.batch_shift;
+    const int bmask = (1 << bshift) - 1;
+    /* every lane of the wave in one simulation: the walk range is one segment */
+    const int root_u = __builtin_amdgcn_readfirstlane(root);
+    const bool one_root = __builtin_amdgcn_ballot_w64(valid && root != root_u) == 0ull;
     float sx = 0.0f, sy = 0.0f, sz = 0.0f;
     double ax = 0.0, ay = 0.0, az = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
     bool met = false, pending = false;
@@ -230,71 +249,89 @@ __global__ __launch_bounds__(BS) void k_walk32(const NodeF* __restrict__ node,
     int q = L.q0;
     NodeF nd = node[max(0, min(q, scal->nnode - 1))];
     while (q < L.qend) {
-        if (COUNT) ++steps;
-        const int nd_next = __builtin_amdgcn_readfirstlane(nd.next);
-        const bool active = lp == q;
-        const float dx = (nd.hx - phx) + (nd.lx - plx);
-        const float dy = (nd.hy - phy) + (nd.ly - ply);
-        const float dz = (nd.hz - phz) + (nd.lz - plz);
-        float drSq = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-        const bool accept = active && drSq >= nd.rcrit2;
-        const bool open = active && !accept;
-        const bool any_open = __builtin_amdgcn_ballot_w64(open) != 0ull;
-        const int qn = any_open ? q + 1 : nd_next;
-        if (qn <= q) {
-            bad = true;
-            break;
+        /* nearest block boundary above q among the lanes' own blocks */
+        int lim;
+        if (one_root) {
+            lim = root_u + ((q - root_u) | bmask) + 1;
+        } else {
+            const int mine = (valid && root >= 0) ? root + ((q - root) | bmask) + 1 : INT_MAX;
+            lim = __builtin_amdgcn_readfirstlane(nbh_wave_min_i(mine));
         }
-        const NodeF nxt = node[min(qn, L.qend - 1)];
-        if (accept) {
-            if (eps2_pos || nd.body != self) {
-                drSq += eps2;
-                const float rinv = __builtin_amdgcn_rsqf(drSq);
-                const float rinv2 = rinv * rinv;
-                const float mor3 = nd.m * rinv * rinv2;
-                if (QUAD && nd.body < 0) {
-                    /* acc += (mor3 + 2.5 dQd / r^7) d - Qd / r^5 */
-                    const float qx = __builtin_fmaf(nd.q[0], dx, __builtin_fmaf(nd.q[1], dy, nd.q[2] * dz));
-                    const float qy = __builtin_fmaf(nd.q[1], dx, __builtin_fmaf(nd.q[3], dy, nd.q[4] * dz));
-                    const float qz = __builtin_fmaf(nd.q[2], dx, __builtin_fmaf(nd.q[4], dy, nd.q[5] * dz));
-                    const float dQd = __builtin_fmaf(qx, dx, __builtin_fmaf(qy, dy, qz * dz));
-                    const float dr5 = rinv2 * rinv2 * rinv;
-                    const float s1 = __builtin_fmaf(dr5 * 2.5f, dQd * rinv2, mor3);
-                    sx = __builtin_fmaf(s1, dx, __builtin_fmaf(-dr5, qx, sx));
-                    sy = __builtin_fmaf(s1, dy, __builtin_fmaf(-dr5, qy, sy));
-                    sz = __builtin_fmaf(s1, dz, __builtin_fmaf(-dr5, qz, sz));
-                } else {
-                    sx = __builtin_fmaf(mor3, dx, sx);
-                    sy = __builtin_fmaf(mor3, dy, sy);
-                    sz = __builtin_fmaf(mor3, dz, sz);
+        lim = min(lim, L.qend);
+        const int qstart = q;
+        do {
+            if (COUNT) ++steps;
+            const int nd_next = __builtin_amdgcn_readfirstlane(nd.next);
+            const int nd_body = __builtin_amdgcn_readfirstlane(nd.body);
+            const float dx = (nd.hx - phx) + (nd.lx - plx);
+            const float dy = (nd.hy - phy) + (nd.ly - ply);
+            const float dz = (nd.hz - phz) + (nd.lz - plz);
+            float drSq = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+            bool accept, any_open;
+            if (COH) {
+                const bool active = one_root ? valid : (q >= my_root && q < my_end);
+                any_open = __builtin_amdgcn_ballot_w64(active && !(drSq >= nd.rcrit2)) != 0ull;
+                accept = active && !any_open;
+            } else {
+                const bool active = lp == q;
+                accept = active && drSq >= nd.rcrit2;
+                any_open = __builtin_amdgcn_ballot_w64(active && !accept) != 0ull;
+                if (active && !accept) {
+                    lp = q + 1;
+                    if (COUNT) ++nop;
                 }
-                if (COUNT) {
-                    if (nd.body >= 0) nbb += nd.body != self; else ++nbc;
-                }
+                if (accept) lp = nd_next;
+                if (o.multi && lp >= my_end) lp = INT_MAX;
             }
-            met |= nd.body == self;
-            pending = true;
-            lp = nd_next;
-        } else if (open) {
-            lp = q + 1;
-            if (COUNT) ++nop;
-        }
-        if (o.multi && lp >= my_end) lp = INT_MAX;
-        /* A lane's FP32 partial sum covers its interactions inside one
-         * block of 2^bshift node indices counted from its simulation's
-         * root.  q increases monotonically, so the grouping depends only on
-         * the lane's own interaction list, not on which lanes share its
-         * wave: results do not change with the batching of simulations or
-         * the workgroup size. */
-        if (((qn - root) >> bshift) != ((q - root) >> bshift) && pending) {
+            int qn = any_open ? q + 1 : nd_next;
+            if (qn <= q) {
+                /* a Next() link that does not advance: corrupt table, stop */
+                bad = true;
+                qn = L.qend;
+            }
+            const NodeF nxt = node[min(qn, L.qend - 1)];
+            if (accept) {
+                met |= nd_body == self;
+                if (eps2_pos || nd_body != self) {
+                    drSq += eps2;
+                    const float rinv = __builtin_amdgcn_rsqf(drSq);
+                    const float rinv2 = rinv * rinv;
+                    const float mor3 = nd.m * rinv * rinv2;
+                    if (QUAD && nd_body < 0) {
+                        /* acc += (mor3 + 2.5 dQd / r^7) d - Qd / r^5 */
+                        const float qx = __builtin_fmaf(nd.q[0], dx, __builtin_fmaf(nd.q[1], dy, nd.q[2] * dz));
+                        const float qy = __builtin_fmaf(nd.q[1], dx, __builtin_fmaf(nd.q[3], dy, nd.q[4] * dz));
+                        const float qz = __builtin_fmaf(nd.q[2], dx, __builtin_fmaf(nd.q[4], dy, nd.q[5] * dz));
+                        const float dQd = __builtin_fmaf(qx, dx, __builtin_fmaf(qy, dy, qz * dz));
+                        const float dr5 = rinv2 * rinv2 * rinv;
+                        const float s1 = __builtin_fmaf(dr5 * 2.5f, dQd * rinv2, mor3);
+                        sx = __builtin_fmaf(s1, dx, __builtin_fmaf(-dr5, qx, sx));
+                        sy = __builtin_fmaf(s1, dy, __builtin_fmaf(-dr5, qy, sy));
+                        sz = __builtin_fmaf(s1, dz, __builtin_fmaf(-dr5, qz, sz));
+                    } else {
+                        sx = __builtin_fmaf(mor3, dx, sx);
+                        sy = __builtin_fmaf(mor3, dy, sy);
+                        sz = __builtin_fmaf(mor3, dz, sz);
+                    }
+                    if (COUNT) {
+                        if (nd_body >= 0) nbb += nd_body != self; else ++nbc;
+                    }
+                }
+                pending = true;
+            }
+            q = qn;
+            nd = nxt;
+        } while (q < lim);
+        /* lanes whose own block changed close their FP32 partial sum */
+        const int rb = one_root ? root_u : root;
+        const bool crossed = ((q - rb) >> bshift) != ((qstart - rb) >> bshift);
+        if (pending && crossed) {
             nbh_kahan_add(ax, cx, (double) sx);
             nbh_kahan_add(ay, cy, (double) sy);
             nbh_kahan_add(az, cz, (double) sz);
             sx = sy = sz = 0.0f;
             pending = false;
         }
-        q = qn;
-        nd = nxt;
     }
     nbh_kahan_add(ax, cx, (double) sx);
     nbh_kahan_add(ay, cy, (double) sy);
@@ -468,19 +505,23 @@ void nbh_walk(nbhip_state* st, int kick, int count, double* oax, double* oay, do
     o.pot = c.pot;
     const int bs = (t.block == 64 || t.block == 128) ? t.block : 256;
     dim3 grid(nbh_blocks(st->n, bs)), block(bs);
-    if (c.precision == NBHIP_PRECISION_MIXED) {
+    if (c.precision != NBHIP_PRECISION_FP64) {
+        const bool coherent = c.precision == NBHIP_PRECISION_MIXED_COHERENT;
         hipLaunchKernelGGL(k_pack32, dim3(nbh_blocks(st->node_cap, 256)), dim3(256), 0, s,
                            st->node, st->nquad, st->nsim_node, st->sims, st->nodef, st->scal);
-#define NBH_W32(B, Q, C)                                                                     \
-    hipLaunchKernelGGL((k_walk32<B, Q, C>), grid, block, 0, s, st->nodef, st->node, st->spos, \
+#define NBH_W32C(B, Q, C, H)                                                                 \
+    hipLaunchKernelGGL((k_walk32<B, Q, C, H>), grid, block, 0, s, st->nodef, st->node, st->spos, \
                        st->perm, st->simid_s, st->sims, oax, oay, oaz, st->vx, st->vy, st->vz, \
                        st->scal, o)
+#define NBH_W32(B, Q, C)                                                                     \
+    if (coherent) NBH_W32C(B, Q, C, true); else NBH_W32C(B, Q, C, false)
 #define NBH_W32B(B)                                                                          \
     if (c.use_quad) { if (count) NBH_W32(B, true, true); else NBH_W32(B, true, false); }     \
     else { if (count) NBH_W32(B, false, true); else NBH_W32(B, false, false); }
         if (bs == 64) { NBH_W32B(64) } else if (bs == 128) { NBH_W32B(128) } else { NBH_W32B(256) }
 #undef NBH_W32B
 #undef NBH_W32
+#undef NBH_W32C
         return;
     }
 #define NBH_W64(B, Q, C)                                                                     \

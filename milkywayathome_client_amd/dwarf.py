@@ -106,6 +106,9 @@ def _eddington_table(target: Component, comps, rgrid):
         arg = E - (tk * sq) ** 2
         F += wk * 0.5 * sq * np.interp(arg, ps, gs)
     F *= 2.0
+    # far out psi stops changing in FP64: keep strictly increasing energies
+    keep = np.concatenate([[True], np.diff(E) > 0.0])
+    E, F = E[keep], F[keep]
     dF = np.gradient(F, E)
     f = np.maximum(dF, 0.0) / (np.sqrt(8.0) * np.pi ** 2)
     return E, f

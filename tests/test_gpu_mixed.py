@@ -80,3 +80,25 @@ def test_mixed_handles_massless_and_tiny():
     a64 = nb.NBodyState(ctx, None, pos, vel, m).accelerations()
     a32 = nb.NBodyState(ctx, None, pos, vel, m, precision="mixed").accelerations()
     np.testing.assert_allclose(a32, a64, rtol=1e-6, atol=1e-9)
+
+
+def test_coherent_mixed_is_at_least_as_accurate(dwarf_case):
+    """Wave-coherent opening refines every body's list: its error against
+    the FP64 direct sum is no larger than the FP64 tree's, it does more
+    interactions (each accepted node passes the body's own test, so there
+    are no fewer), and it does not depend on the workgroup size."""
+    ctx, pot, pos, vel, m = dwarf_case
+    idx = np.arange(0, pos.shape[0], 10, dtype=np.int32)
+    st64 = nb.NBodyState(ctx, None, pos, vel, m)
+    d = st64.direct_accel(idx)
+    e64 = np.median(rel_err(st64.accelerations()[idx], d, 0))
+    coh = nb.NBodyState(ctx, None, pos, vel, m, precision="mixed_coherent")
+    ecoh = np.median(rel_err(coh.accelerations()[idx], d, 0))
+    print("median vs direct: fp64 tree", e64, "coherent mixed", ecoh)
+    assert ecoh <= e64
+    c64, ccoh = st64.walk_counters(), coh.walk_counters()
+    assert ccoh[0] + ccoh[1] >= c64[0] + c64[1]
+    coh.set_tune(64, 1024)
+    a = coh.walk_accel()
+    coh.set_tune(256, 1024)
+    assert np.array_equal(coh.walk_accel(), a)
