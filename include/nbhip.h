@@ -204,6 +204,31 @@ unsigned nbhip_autotune(nbhip_state* st, nbhip_tune* best, nbhip_tune* trials, i
                         int* ntrials);
 unsigned nbhip_set_tune(nbhip_state* st, const nbhip_tune* tune);
 
+/* ---- histogram of the final state ---------------------------------------
+ * HistogramParams (nbody/include/nbody_types.h) fields the lambda-beta
+ * binning uses; angles in degrees, sun_gc_dist in kpc (NBodyCtx.sunGCDist). */
+typedef struct nbhip_histogram_params {
+    double phi, theta, psi;
+    double lambda_start, lambda_end;
+    double beta_start, beta_end;
+    int lambda_bins, beta_bins;
+    double sun_gc_dist;
+} nbhip_histogram_params;
+
+/* Per-body flag (n entries, all simulations, upload order); nonzero = the
+ * body is left out of histograms, as ignoreBody leaves out the dark matter
+ * of a mixed dwarf (nbody/src/nbody_histogram.c:863-865).  NULL clears. */
+unsigned nbhip_set_ignore(nbhip_state* st, const unsigned char* ignore);
+
+/* Raw counts of nbCreateHistogram (nbody/src/nbody_histogram.c:671-940) on
+ * the current positions, computed on the GPU: counts[sim * bins + b] for
+ * b = lambdaIndex * beta_bins + betaIndex, totals[sim] = bodies inside the
+ * histogram (its totalNum); the caller normalises (count / totalNum) as
+ * nbNormaliseHistogram does.  counts holds nsim * lambda_bins * beta_bins
+ * entries, totals nsim (either may be NULL). */
+unsigned nbhip_histogram(nbhip_state* st, const nbhip_histogram_params* hp, unsigned* counts,
+                         unsigned* totals);
+
 /* ---- adjoint leapfrog ------------------------------------------------------
  * No counterpart in the reference.  Gradients of a loss on the final state
  * with respect to the initial positions, velocities, the masses and the
@@ -220,6 +245,11 @@ unsigned nbhip_set_tune(nbhip_state* st, const nbhip_tune* tune);
  * and dL/dtimestep per simulation (nsim); any output may be NULL.  The state
  * is left as the forward run ended. */
 unsigned nbhip_adjoint_forward(nbhip_state* st, unsigned nsteps, unsigned every);
+/* Vector-Jacobian product of one force evaluation (nbhip_gravity) at the
+ * current positions, with the current tree and its opening decisions held
+ * fixed: given dL/dacc (n x 3) returns dL/dpos (n x 3) and dL/dmass (n),
+ * either may be NULL.  The building block of nbhip_adjoint_backward. */
+unsigned nbhip_gravity_vjp(nbhip_state* st, const double* gacc, double* gpos, double* gmass);
 unsigned nbhip_adjoint_backward(nbhip_state* st, const double* gpos, const double* gvel,
                                 double* gpos0, double* gvel0, double* gmass, double* gdt);
 

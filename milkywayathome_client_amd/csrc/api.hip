@@ -96,7 +96,7 @@ static void nbh_free_all(nbhip_state* st)
                      st->simid_s, st->su,  st->dadj,  st->cincl,   st->scan_part, st->blo,
                      st->bhi,  st->bdelta, st->bparent, st->node,  st->nquad,  st->nmid,
                      st->nlevel, st->nparent, st->nsim_node, st->level_list, st->nodef,
-                     st->scal, st->sims };
+                     st->scal, st->sims, st->ignore, st->hist_buf };
     for (void* p : ptrs)
         if (p) (void) hipFree(p);
     if (st->graph_exec) (void) hipGraphExecDestroy(st->graph_exec);
@@ -193,7 +193,7 @@ static unsigned nbh_create(const nbhip_ctx* ctx, int device, int nsim, const int
     A(blo, n); A(bhi, n); A(bdelta, n); A(bparent, 2 * n);
     A(node, nc); A(nquad, 6 * nc); A(nmid, 3 * nc); A(nlevel, nc); A(nparent, nc);
     A(nsim_node, nc); A(level_list, nc);
-    if (ctx->precision != NBHIP_PRECISION_FP64) A(nodef, nc);
+    if (ctx->precision != NBHIP_PRECISION_FP64) A(nodef, nc + 1);   /* + 1: walk prefetch */
     A(scal, 1);
     A(sims, (size_t) nsim);
 #undef A
@@ -824,6 +824,60 @@ static void nbh_adj_rebuild(nbhip_state* st)
     nbh_build(st, 0);
 }
 
+/* adjoint vectors and node records only (no checkpoints): for nbhip_gravity_vjp */
+static unsigned nbh_adj_vectors(nbhip_state* st)
+{
+    if (st->adj && st->adj->g) return NBHIP_SUCCESS;
+    if (!st->adj) st->adj = (NbhAdjoint*) calloc(1, sizeof(NbhAdjoint));
+    NbhAdjoint* A = st->adj;
+    const size_t N = (size_t) st->n, ns = (size_t) st->nsim;
+    hipError_t e = nbh_alloc(&A->g, 10 * N);
+    if (e == hipSuccess) e = nbh_alloc(&A->gdt, ns);
+    if (e == hipSuccess) e = nbh_alloc(&A->gnode, 10 * (size_t) st->node_cap);
+    if (e != hipSuccess) {
+        nbh_set_error("adjoint vectors: %s", hipGetErrorString(e));
+        nbh_adj_free(st);
+        return NBHIP_GPU_ERROR;
+    }
+    return NBHIP_SUCCESS;
+}
+
+extern "C" unsigned nbhip_gravity_vjp(nbhip_state* st, const double* gacc, double* gpos,
+                                      double* gmass)
+{
+    if (!st || !gacc) return NBHIP_USER_ERROR;
+    if (st->ctx.precision != NBHIP_PRECISION_FP64) {
+        nbh_set_error("nbhip_gravity_vjp: needs an FP64 state");
+        return NBHIP_UNSUPPORTED;
+    }
+    if (st->ctx.criterion != NBHIP_EXACT && !st->have_tree) {
+        nbh_set_error("nbhip_gravity_vjp: no tree; call nbhip_gravity first");
+        return NBHIP_USER_ERROR;
+    }
+    NBH_CHECK(hipSetDevice(st->device));
+    unsigned rc = nbh_adj_vectors(st);
+    if (rc) return rc;
+    NbhAdjoint* A = st->adj;
+    const size_t N = (size_t) st->n;
+    std::vector<double> h(10 * N, 0.0);
+    for (size_t i = 0; i < N; ++i)
+        for (int d = 0; d < 3; ++d) h[(6 + d) * N + i] = gacc[3 * i + d];
+    NBH_CHECK(hipMemcpyAsync(A->g, h.data(), 10 * N * sizeof(double), hipMemcpyHostToDevice,
+                             st->stream));
+    nbh_adj_force(st);
+    nbh_adj_body(st, 0, nullptr, nullptr);
+    NBH_CHECK(hipGetLastError());
+    NBH_CHECK(hipMemcpyAsync(h.data(), A->g, 10 * N * sizeof(double), hipMemcpyDeviceToHost,
+                             st->stream));
+    NBH_CHECK(hipStreamSynchronize(st->stream));
+    for (size_t i = 0; i < N; ++i) {
+        for (int d = 0; d < 3; ++d)
+            if (gpos) gpos[3 * i + d] = h[d * N + i];
+        if (gmass) gmass[i] = h[9 * N + i];
+    }
+    return NBHIP_SUCCESS;
+}
+
 extern "C" unsigned nbhip_adjoint_forward(nbhip_state* st, unsigned nsteps, unsigned every)
 {
     if (!st) return NBHIP_USER_ERROR;
@@ -967,4 +1021,58 @@ extern "C" unsigned nbhip_adjoint_backward(nbhip_state* st, const double* gpos, 
     if (gdt)
         for (size_t k = 0; k < ns; ++k) gdt[k] = hdt[k];
     return rc;
+}
+
+/* ---------------------------------------------------------------------- */
+/* Histogram                                                              */
+/* ---------------------------------------------------------------------- */
+extern "C" unsigned nbhip_set_ignore(nbhip_state* st, const unsigned char* ignore)
+{
+    if (!st) return NBHIP_USER_ERROR;
+    NBH_CHECK(hipSetDevice(st->device));
+    if (!ignore) {
+        if (st->ignore) NBH_CHECK(hipFree(st->ignore));
+        st->ignore = nullptr;
+        return NBHIP_SUCCESS;
+    }
+    if (!st->ignore) NBH_CHECK(nbh_alloc(&st->ignore, (size_t) st->n));
+    if (st->n)
+        NBH_CHECK(hipMemcpyAsync(st->ignore, ignore, (size_t) st->n, hipMemcpyHostToDevice,
+                                 st->stream));
+    NBH_CHECK(hipStreamSynchronize(st->stream));
+    return NBHIP_SUCCESS;
+}
+
+extern "C" unsigned nbhip_histogram(nbhip_state* st, const nbhip_histogram_params* hp,
+                                    unsigned* counts, unsigned* totals)
+{
+    if (!st || !hp) return NBHIP_USER_ERROR;
+    if (hp->lambda_bins < 1 || hp->beta_bins < 1 || !(hp->lambda_end > hp->lambda_start) ||
+        !(hp->beta_end > hp->beta_start)) {
+        nbh_set_error("nbhip_histogram: empty bin range");
+        return NBHIP_USER_ERROR;
+    }
+    NBH_CHECK(hipSetDevice(st->device));
+    const size_t nbin = (size_t) hp->lambda_bins * (size_t) hp->beta_bins;
+    const size_t need = (size_t) st->nsim * (nbin + 1);
+    if (need > st->hist_cap) {
+        if (st->hist_buf) NBH_CHECK(hipFree(st->hist_buf));
+        st->hist_buf = nullptr;
+        st->hist_cap = 0;
+        NBH_CHECK(nbh_alloc(&st->hist_buf, need));
+        st->hist_cap = need;
+    }
+    unsigned* d_counts = st->hist_buf;
+    unsigned* d_totals = st->hist_buf + (size_t) st->nsim * nbin;
+    NBH_CHECK(hipMemsetAsync(st->hist_buf, 0, need * sizeof(unsigned), st->stream));
+    nbh_histogram(st, hp, d_counts, d_totals);
+    NBH_CHECK(hipGetLastError());
+    if (counts)
+        NBH_CHECK(hipMemcpyAsync(counts, d_counts, (size_t) st->nsim * nbin * sizeof(unsigned),
+                                 hipMemcpyDeviceToHost, st->stream));
+    if (totals)
+        NBH_CHECK(hipMemcpyAsync(totals, d_totals, (size_t) st->nsim * sizeof(unsigned),
+                                 hipMemcpyDeviceToHost, st->stream));
+    NBH_CHECK(hipStreamSynchronize(st->stream));
+    return NBHIP_SUCCESS;
 }

@@ -147,6 +147,10 @@ struct nbhip_state {
 
     NbhAdjoint* adj;
 
+    unsigned char* ignore;        /* per body, original order: left out of histograms */
+    unsigned* hist_buf;           /* device counts + totals of nbhip_histogram */
+    size_t hist_cap;              /* unsigned entries in hist_buf */
+
     WalkTune tune;
     int use_graph;
     hipGraph_t graph;
@@ -161,6 +165,9 @@ void nbh_build(nbhip_state* st, int graph_safe);
 void nbh_walk(nbhip_state* st, int kick, int count, double* out_ax, double* out_ay,
               double* out_az, int self_only, const WalkTune* tune);
 void nbh_direct(nbhip_state* st, int nsample, const int* d_idx, double* d_acc);
+/* launcher (histogram.hip) */
+void nbh_histogram(nbhip_state* st, const nbhip_histogram_params* hp, unsigned* d_counts,
+                   unsigned* d_totals);
 /* launchers (adjoint.hip) */
 void nbh_adj_kick_end(nbhip_state* st, const double* a_next);
 void nbh_adj_force(nbhip_state* st);

@@ -201,7 +201,7 @@ __device__ __forceinline__ void nbh_kahan_add(double& sum, double& comp, double 
     sum = t;
 }
 
-template <int BS, bool QUAD, bool COUNT, bool COH>
+template <int BS, bool QUAD, bool COUNT, bool COH, bool MULTI>
 __global__ __launch_bounds__(BS) void k_walk32(const NodeF* __restrict__ node,
                                                const NodeW* __restrict__ nodew,
                                                const double4* __restrict__ spos,
@@ -237,18 +237,78 @@ __global__ __launch_bounds__(BS) void k_walk32(const NodeF* __restrict__ node,
     const bool eps2_pos = o.all_eps2_pos;
     const int bshift = o.batch_shift;
     const int bmask = (1 << bshift) - 1;
-    /* every lane of the wave in one simulation: the walk range is one segment */
+    const int qend = L.qend;
+    /* every lane of the wave in one simulation: one root, one walk segment */
     const int root_u = __builtin_amdgcn_readfirstlane(root);
-    const bool one_root = __builtin_amdgcn_ballot_w64(valid && root != root_u) == 0ull;
+    const bool one_root = !MULTI ||
+                          __builtin_amdgcn_ballot_w64(valid && root != root_u) == 0ull;
     float sx = 0.0f, sy = 0.0f, sz = 0.0f;
     double ax = 0.0, ay = 0.0, az = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
     bool met = false, pending = false;
     unsigned nbb = 0, nbc = 0, nop = 0, steps = 0;
-
-    bool bad = false;
     int q = L.q0;
-    NodeF nd = node[max(0, min(q, scal->nnode - 1))];
-    while (q < L.qend) {
+
+    /* One node step: test, vote, successor (fetched now, used next step),
+     * interaction.  The successor lies in [q + 1, qend], so every step
+     * advances and every fetch stays inside the table (it holds one record
+     * past the last node); the summaries already reject broken Next links. */
+    auto step = [&](const NodeF& nd, NodeF& nxt) {
+        if (COUNT) ++steps;
+        const float dx = (nd.hx - phx) + (nd.lx - plx);
+        const float dy = (nd.hy - phy) + (nd.ly - ply);
+        const float dz = (nd.hz - phz) + (nd.lz - plz);
+        const float drSq = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+        bool accept, any_open;
+        if (COH) {
+            const bool active = one_root ? valid : (q >= my_root && q < my_end);
+            any_open = __builtin_amdgcn_ballot_w64(active && !(drSq >= nd.rcrit2)) != 0ull;
+            accept = active && !any_open;
+        } else {
+            const bool active = lp == q;
+            accept = active && drSq >= nd.rcrit2;
+            const bool opn = active && !accept;
+            any_open = __builtin_amdgcn_ballot_w64(opn) != 0ull;
+            if (COUNT) nop += opn;
+            lp = accept ? nd.next : (opn ? q + 1 : lp);
+            if (MULTI && lp >= my_end) lp = INT_MAX;
+        }
+        const int qn = min(max(any_open ? q + 1 : nd.next, q + 1), qend);
+        nxt = node[qn];
+        if (accept) {
+            met |= nd.body == self;
+            pending = true;
+            /* with eps2 = 0 the self term is skipped: d = 0 against 1 gives 0 */
+            const float d2 = (!eps2_pos && nd.body == self) ? 1.0f : drSq + eps2;
+            const float rinv = __builtin_amdgcn_rsqf(d2);
+            const float rinv2 = rinv * rinv;
+            const float mor3 = nd.m * rinv * rinv2;
+            if (QUAD) {
+                /* acc += (mor3 + 2.5 dQd / r^7) d - Qd / r^5; a body's record
+                 * carries Q = 0, which leaves exactly the monopole */
+                const float qx = __builtin_fmaf(nd.q[0], dx, __builtin_fmaf(nd.q[1], dy, nd.q[2] * dz));
+                const float qy = __builtin_fmaf(nd.q[1], dx, __builtin_fmaf(nd.q[3], dy, nd.q[4] * dz));
+                const float qz = __builtin_fmaf(nd.q[2], dx, __builtin_fmaf(nd.q[4], dy, nd.q[5] * dz));
+                const float dQd = __builtin_fmaf(qx, dx, __builtin_fmaf(qy, dy, qz * dz));
+                const float dr5 = rinv2 * rinv2 * rinv;
+                const float s1 = __builtin_fmaf(dr5 * 2.5f, dQd * rinv2, mor3);
+                sx = __builtin_fmaf(s1, dx, __builtin_fmaf(-dr5, qx, sx));
+                sy = __builtin_fmaf(s1, dy, __builtin_fmaf(-dr5, qy, sy));
+                sz = __builtin_fmaf(s1, dz, __builtin_fmaf(-dr5, qz, sz));
+            } else {
+                sx = __builtin_fmaf(mor3, dx, sx);
+                sy = __builtin_fmaf(mor3, dy, sy);
+                sz = __builtin_fmaf(mor3, dz, sz);
+            }
+            if (COUNT) {
+                if (nd.body >= 0) nbb += nd.body != self; else ++nbc;
+            }
+        }
+        q = qn;
+    };
+
+    NodeF nda = node[min(max(q, 0), qend)];
+    NodeF ndb;
+    while (q < qend) {
         /* nearest block boundary above q among the lanes' own blocks */
         int lim;
         if (one_root) {
@@ -257,71 +317,19 @@ __global__ __launch_bounds__(BS) void k_walk32(const NodeF* __restrict__ node,
             const int mine = (valid && root >= 0) ? root + ((q - root) | bmask) + 1 : INT_MAX;
             lim = __builtin_amdgcn_readfirstlane(nbh_wave_min_i(mine));
         }
-        lim = min(lim, L.qend);
+        lim = min(lim, qend);
         const int qstart = q;
-        do {
-            if (COUNT) ++steps;
-            const int nd_next = __builtin_amdgcn_readfirstlane(nd.next);
-            const int nd_body = __builtin_amdgcn_readfirstlane(nd.body);
-            const float dx = (nd.hx - phx) + (nd.lx - plx);
-            const float dy = (nd.hy - phy) + (nd.ly - ply);
-            const float dz = (nd.hz - phz) + (nd.lz - plz);
-            float drSq = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-            bool accept, any_open;
-            if (COH) {
-                const bool active = one_root ? valid : (q >= my_root && q < my_end);
-                any_open = __builtin_amdgcn_ballot_w64(active && !(drSq >= nd.rcrit2)) != 0ull;
-                accept = active && !any_open;
-            } else {
-                const bool active = lp == q;
-                accept = active && drSq >= nd.rcrit2;
-                any_open = __builtin_amdgcn_ballot_w64(active && !accept) != 0ull;
-                if (active && !accept) {
-                    lp = q + 1;
-                    if (COUNT) ++nop;
-                }
-                if (accept) lp = nd_next;
-                if (o.multi && lp >= my_end) lp = INT_MAX;
+        /* two steps per trip with the node records alternating between two
+         * register sets */
+        for (;;) {
+            step(nda, ndb);
+            if (q >= lim) {
+                nda = ndb;
+                break;
             }
-            int qn = any_open ? q + 1 : nd_next;
-            if (qn <= q) {
-                /* a Next() link that does not advance: corrupt table, stop */
-                bad = true;
-                qn = L.qend;
-            }
-            const NodeF nxt = node[min(qn, L.qend - 1)];
-            if (accept) {
-                met |= nd_body == self;
-                if (eps2_pos || nd_body != self) {
-                    drSq += eps2;
-                    const float rinv = __builtin_amdgcn_rsqf(drSq);
-                    const float rinv2 = rinv * rinv;
-                    const float mor3 = nd.m * rinv * rinv2;
-                    if (QUAD && nd_body < 0) {
-                        /* acc += (mor3 + 2.5 dQd / r^7) d - Qd / r^5 */
-                        const float qx = __builtin_fmaf(nd.q[0], dx, __builtin_fmaf(nd.q[1], dy, nd.q[2] * dz));
-                        const float qy = __builtin_fmaf(nd.q[1], dx, __builtin_fmaf(nd.q[3], dy, nd.q[4] * dz));
-                        const float qz = __builtin_fmaf(nd.q[2], dx, __builtin_fmaf(nd.q[4], dy, nd.q[5] * dz));
-                        const float dQd = __builtin_fmaf(qx, dx, __builtin_fmaf(qy, dy, qz * dz));
-                        const float dr5 = rinv2 * rinv2 * rinv;
-                        const float s1 = __builtin_fmaf(dr5 * 2.5f, dQd * rinv2, mor3);
-                        sx = __builtin_fmaf(s1, dx, __builtin_fmaf(-dr5, qx, sx));
-                        sy = __builtin_fmaf(s1, dy, __builtin_fmaf(-dr5, qy, sy));
-                        sz = __builtin_fmaf(s1, dz, __builtin_fmaf(-dr5, qz, sz));
-                    } else {
-                        sx = __builtin_fmaf(mor3, dx, sx);
-                        sy = __builtin_fmaf(mor3, dy, sy);
-                        sz = __builtin_fmaf(mor3, dz, sz);
-                    }
-                    if (COUNT) {
-                        if (nd_body >= 0) nbb += nd_body != self; else ++nbc;
-                    }
-                }
-                pending = true;
-            }
-            q = qn;
-            nd = nxt;
-        } while (q < lim);
+            step(ndb, nda);
+            if (q >= lim) break;
+        }
         /* lanes whose own block changed close their FP32 partial sum */
         const int rb = one_root ? root_u : root;
         const bool crossed = ((q - rb) >> bshift) != ((qstart - rb) >> bshift);
@@ -337,7 +345,6 @@ __global__ __launch_bounds__(BS) void k_walk32(const NodeF* __restrict__ node,
     nbh_kahan_add(ay, cy, (double) sy);
     nbh_kahan_add(az, cz, (double) sz);
 
-    if (bad && __lane_id() == 0) atomicOr(&scal->status, NBHIP_ERROR);
     if (COUNT) nbh_walk_counters(scal, nbb, nbc, nop, steps);
     if (!valid) return;
     if (!met) atomicOr(&sims[L.sim].incest, 1u);
@@ -509,12 +516,14 @@ void nbh_walk(nbhip_state* st, int kick, int count, double* oax, double* oay, do
         const bool coherent = c.precision == NBHIP_PRECISION_MIXED_COHERENT;
         hipLaunchKernelGGL(k_pack32, dim3(nbh_blocks(st->node_cap, 256)), dim3(256), 0, s,
                            st->node, st->nquad, st->nsim_node, st->sims, st->nodef, st->scal);
+#define NBH_W32M(B, Q, C, H, MU)                                                             \
+    hipLaunchKernelGGL((k_walk32<B, Q, C, H, MU>), grid, block, 0, s, st->nodef, st->node,     \
+                       st->spos, st->perm, st->simid_s, st->sims, oax, oay, oaz, st->vx,       \
+                       st->vy, st->vz, st->scal, o)
 #define NBH_W32C(B, Q, C, H)                                                                 \
-    hipLaunchKernelGGL((k_walk32<B, Q, C, H>), grid, block, 0, s, st->nodef, st->node, st->spos, \
-                       st->perm, st->simid_s, st->sims, oax, oay, oaz, st->vx, st->vy, st->vz, \
-                       st->scal, o)
+    do { if (o.multi) NBH_W32M(B, Q, C, H, true); else NBH_W32M(B, Q, C, H, false); } while (0)
 #define NBH_W32(B, Q, C)                                                                     \
-    if (coherent) NBH_W32C(B, Q, C, true); else NBH_W32C(B, Q, C, false)
+    do { if (coherent) NBH_W32C(B, Q, C, true); else NBH_W32C(B, Q, C, false); } while (0)
 #define NBH_W32B(B)                                                                          \
     if (c.use_quad) { if (count) NBH_W32(B, true, true); else NBH_W32(B, true, false); }     \
     else { if (count) NBH_W32(B, false, true); else NBH_W32(B, false, false); }
@@ -522,6 +531,7 @@ void nbh_walk(nbhip_state* st, int kick, int count, double* oax, double* oay, do
 #undef NBH_W32B
 #undef NBH_W32
 #undef NBH_W32C
+#undef NBH_W32M
         return;
     }
 #define NBH_W64(B, Q, C)                                                                     \

@@ -75,6 +75,13 @@ class _TreeInfo(C.Structure):
     ]
 
 
+class _HistParams(C.Structure):
+    _fields_ = [("phi", C.c_double), ("theta", C.c_double), ("psi", C.c_double),
+                ("lambda_start", C.c_double), ("lambda_end", C.c_double),
+                ("beta_start", C.c_double), ("beta_end", C.c_double),
+                ("lambda_bins", C.c_int), ("beta_bins", C.c_int), ("sun_gc_dist", C.c_double)]
+
+
 class _Tune(C.Structure):
     _fields_ = [("block", C.c_int), ("batch", C.c_int), ("ms", C.c_double)]
 
@@ -86,7 +93,8 @@ EXPORTED = [
     "nbhip_phase_times", "nbhip_sync", "nbhip_release", "nbhip_last_error",
     "nbhip_batch_create", "nbhip_batch_upload", "nbhip_batch_download", "nbhip_batch_status",
     "nbhip_batch_tree_info", "nbhip_set_graph", "nbhip_autotune", "nbhip_set_tune",
-    "nbhip_adjoint_forward", "nbhip_adjoint_backward",
+    "nbhip_adjoint_forward", "nbhip_adjoint_backward", "nbhip_gravity_vjp", "nbhip_set_ignore",
+    "nbhip_histogram",
 ]
 
 _LIB: Optional[C.CDLL] = None
@@ -134,7 +142,11 @@ def load_library() -> C.CDLL:
         "nbhip_autotune": (u, [st, C.POINTER(_Tune), C.POINTER(_Tune), C.c_int,
                                C.POINTER(C.c_int)]),
         "nbhip_set_tune": (u, [st, C.POINTER(_Tune)]),
+        "nbhip_set_ignore": (u, [st, C.POINTER(C.c_ubyte)]),
+        "nbhip_histogram": (u, [st, C.POINTER(_HistParams), C.POINTER(C.c_uint),
+                                C.POINTER(C.c_uint)]),
         "nbhip_adjoint_forward": (u, [st, C.c_uint, C.c_uint]),
+        "nbhip_gravity_vjp": (u, [st, _dp, _dp, _dp]),
         "nbhip_adjoint_backward": (u, [st, _dp, _dp, _dp, _dp, _dp, _dp]),
     }
     for name, (res, args) in sig.items():
@@ -423,6 +435,36 @@ class NBodyState:
     def set_tune(self, block: int, batch: int = 1024) -> None:
         t = _Tune(block, batch, 0.0)
         self._check(self.lib.nbhip_set_tune(self._h, C.byref(t)))
+
+    def set_ignore(self, ignore) -> None:
+        """Per-body flags (upload order, all simulations): True = left out
+        of histograms (the dark matter of a mixed dwarf); None clears."""
+        if ignore is None:
+            self._check(self.lib.nbhip_set_ignore(self._h, None))
+            return
+        a = np.ascontiguousarray(np.asarray(ignore).reshape(self.n) != 0, dtype=np.uint8)
+        self._check(self.lib.nbhip_set_ignore(self._h, _ptr(a, C.c_ubyte)))
+
+    def histogram(self, hp):
+        """Raw lambda-beta counts on the GPU for each simulation: returns
+        (counts [nsim, lambdaBins * betaBins], totals [nsim]); hp is a
+        histogram.HistogramParams."""
+        h = _HistParams(hp.phi, hp.theta, hp.psi, hp.lambdaStart, hp.lambdaEnd, hp.betaStart,
+                        hp.betaEnd, hp.lambdaBins, hp.betaBins, hp.sunGCDist)
+        nsim = getattr(self, "nsim", 1)
+        counts = np.zeros((nsim, hp.lambdaBins * hp.betaBins), dtype=np.uint32)
+        totals = np.zeros(nsim, dtype=np.uint32)
+        self._check(self.lib.nbhip_histogram(self._h, C.byref(h), _ptr(counts, C.c_uint),
+                                             _ptr(totals, C.c_uint)))
+        return counts, totals
+
+    def gravity_vjp(self, gacc):
+        """(dL/dpos [n,3], dL/dmass [n]) of one force evaluation at the current
+        positions, tree frozen, from dL/dacc [n,3]."""
+        g = np.ascontiguousarray(gacc, dtype=np.float64).reshape(self.n, 3)
+        gp, gm = np.empty((self.n, 3)), np.empty(self.n)
+        self._check(self.lib.nbhip_gravity_vjp(self._h, _ptr(g), _ptr(gp), _ptr(gm)))
+        return gp, gm
 
     def adjoint_forward(self, nsteps: int, every: int = 50) -> int:
         """Force evaluation + nsteps KDK steps (as run()), checkpointing the

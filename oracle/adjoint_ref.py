@@ -181,3 +181,163 @@ def kdk_adjoint(states, m, eps2, dt, gxN, gvN, pot=None):
     gx = gx + jx + ext_jac(p, x0, ga)
     gm = gm + jm
     return gx, gv, gm, gdt
+
+
+# ---------------------------------------------------------------------------
+# Frozen-tree force and its vector-Jacobian product
+# ---------------------------------------------------------------------------
+# The tree (cells, membership, opening decisions) is the oracle's, exported
+# in pre-order (orc_tree_preorder: kind 0 = cell, 1 = body; skip = Next).
+# frozen_accel recomputes every cell's mass, centre of mass and quadrupole
+# from the body positions (hackCofM / hackQuad order: children in octant
+# order, nbody/src/nbody_tree.c:70-121,308-424) and sums each body's fixed
+# interaction list (nbGravity's terms, nbody/src/nbody_grav.c:53-99), so it
+# is a smooth function of the positions and masses; frozen_vjp is its
+# adjoint, written the way the GPU kernels compute it.
+
+def tree_children(t):
+    kind, skip = t["kind"], t["skip"]
+    children = [[] for _ in range(len(kind))]
+    for c in range(len(kind)):
+        if kind[c] == 0:
+            ch = c + 1
+            while ch < skip[c]:
+                children[c].append(ch)
+                ch = skip[ch]
+    return children
+
+
+def walk_lists(t):
+    """Per massive body: the nodes its reference walk accepts (self left out),
+    replayed on the exported tree with the walk's arithmetic."""
+    kind, ref, skip, pos, rc2 = t["kind"], t["ref"], t["skip"], t["pos"], t["rcrit2"]
+    nn = len(kind)
+    lists = {}
+    for leaf in np.nonzero(kind == 1)[0]:
+        b = int(ref[leaf])
+        x, y, z = pos[leaf]
+        lst, q = [], 0
+        while q < nn:
+            if kind[q] == 1:
+                if ref[q] != b:
+                    lst.append(q)
+                q = int(skip[q])
+            else:
+                dx, dy, dz = pos[q, 0] - x, pos[q, 1] - y, pos[q, 2] - z
+                if dz * dz + (dy * dy + dx * dx) >= rc2[q]:
+                    lst.append(q)
+                    q = int(skip[q])
+                else:
+                    q += 1
+        lists[b] = np.array(lst, dtype=np.int64)
+    return lists
+
+
+def frozen_moments(X, m, t, children):
+    kind, ref = t["kind"], t["ref"]
+    nn = len(kind)
+    cm, M, Q = np.zeros((nn, 3)), np.zeros(nn), np.zeros((nn, 6))
+    for c in range(nn - 1, -1, -1):
+        if kind[c] == 1:
+            cm[c], M[c] = X[ref[c]], m[ref[c]]
+            continue
+        ch = children[c]
+        M[c] = M[ch].sum()
+        cm[c] = (M[ch, None] * cm[ch]).sum(0) / M[c]
+        d = cm[ch] - cm[c]
+        r2 = (d * d).sum(1)
+        mm = M[ch]
+        Q[c] = np.stack([mm * (3 * d[:, 0] * d[:, 0] - r2), mm * 3 * d[:, 0] * d[:, 1],
+                         mm * 3 * d[:, 0] * d[:, 2], mm * (3 * d[:, 1] * d[:, 1] - r2),
+                         mm * 3 * d[:, 1] * d[:, 2], mm * (3 * d[:, 2] * d[:, 2] - r2)], 1).sum(0) \
+            + Q[ch].sum(0)
+    return cm, M, Q
+
+
+def _qmul(Q, v):
+    return np.stack([Q[:, 0] * v[:, 0] + Q[:, 1] * v[:, 1] + Q[:, 2] * v[:, 2],
+                     Q[:, 1] * v[:, 0] + Q[:, 3] * v[:, 1] + Q[:, 4] * v[:, 2],
+                     Q[:, 2] * v[:, 0] + Q[:, 4] * v[:, 1] + Q[:, 5] * v[:, 2]], 1)
+
+
+def frozen_accel(X, m, t, children, lists, eps2, quad=True):
+    cm, M, Q = frozen_moments(X, m, t, children)
+    cellq = (t["kind"] == 0) & quad
+    a = np.zeros((X.shape[0], 3))
+    for b, lst in lists.items():
+        d = cm[lst] - X[b]
+        s2 = (d * d).sum(1) + eps2
+        s = np.sqrt(s2)
+        acc = (M[lst] / s / s2)[:, None] * d
+        qq = np.where(cellq[lst][:, None], Q[lst], 0.0)
+        u = _qmul(qq, d)
+        dqd = (u * d).sum(1)
+        s5 = 1.0 / (s2 * s2 * s)
+        acc += (2.5 * s5 * dqd / s2)[:, None] * d - s5[:, None] * u
+        a[b] = acc.sum(0)
+    return a
+
+
+def frozen_vjp(X, m, t, children, lists, eps2, G, quad=True):
+    """(dL/dX, dL/dm) of frozen_accel given dL/da = G."""
+    kind, ref = t["kind"], t["ref"]
+    nn = len(kind)
+    cm, M, Q = frozen_moments(X, m, t, children)
+    cellq = (kind == 0) & quad
+    gx, gm = np.zeros_like(X), np.zeros(X.shape[0])
+    gcm, gM, gQ = np.zeros((nn, 3)), np.zeros(nn), np.zeros((nn, 6))
+    for b, lst in lists.items():
+        g = G[b]
+        d = cm[lst] - X[b]
+        s2 = (d * d).sum(1) + eps2
+        s3 = 1.0 / (s2 * np.sqrt(s2))
+        s5, s7 = s3 / s2, s3 / s2 / s2
+        s9 = s7 / s2
+        dg = d @ g
+        qq = np.where(cellq[lst][:, None], Q[lst], 0.0)
+        u = _qmul(qq, d)
+        v = _qmul(qq, np.broadcast_to(g, d.shape))
+        dqd = (u * d).sum(1)
+        ug = u @ g
+        cg = M[lst] * s3 + 2.5 * dqd * s7
+        cd = -3.0 * M[lst] * s5 * dg - 17.5 * dqd * s9 * dg + 5.0 * s7 * ug
+        h = cg[:, None] * g + cd[:, None] * d + (5.0 * s7 * dg)[:, None] * u - s5[:, None] * v
+        gx[b] -= h.sum(0)
+        np.add.at(gcm, lst, h)
+        np.add.at(gM, lst, s3 * dg)
+        a2 = 2.5 * s7 * dg
+        cQ = np.stack([a2 * d[:, 0] ** 2 - s5 * d[:, 0] * g[0],
+                       2 * a2 * d[:, 0] * d[:, 1] - s5 * (d[:, 1] * g[0] + d[:, 0] * g[1]),
+                       2 * a2 * d[:, 0] * d[:, 2] - s5 * (d[:, 2] * g[0] + d[:, 0] * g[2]),
+                       a2 * d[:, 1] ** 2 - s5 * d[:, 1] * g[1],
+                       2 * a2 * d[:, 1] * d[:, 2] - s5 * (d[:, 2] * g[1] + d[:, 1] * g[2]),
+                       a2 * d[:, 2] ** 2 - s5 * d[:, 2] * g[2]], 1)
+        np.add.at(gQ, lst, np.where(cellq[lst][:, None], cQ, 0.0))
+    # cells hand their records to their children, roots first
+    for c in range(nn):
+        if kind[c] != 0:
+            continue
+        ch = np.array(children[c])
+        d = cm[ch] - cm[c]
+        w = M[ch] / M[c]
+        gcm[ch] += w[:, None] * gcm[c]
+        gM[ch] += gM[c] + (d @ gcm[c]) / M[c]
+        if quad:
+            G6 = gQ[c]
+            mm = M[ch]
+            gcm[ch, 0] += mm * (4 * d[:, 0] * G6[0] + 3 * d[:, 1] * G6[1] + 3 * d[:, 2] * G6[2]
+                                - 2 * d[:, 0] * G6[3] - 2 * d[:, 0] * G6[5])
+            gcm[ch, 1] += mm * (-2 * d[:, 1] * G6[0] + 3 * d[:, 0] * G6[1] + 4 * d[:, 1] * G6[3]
+                                + 3 * d[:, 2] * G6[4] - 2 * d[:, 1] * G6[5])
+            gcm[ch, 2] += mm * (-2 * d[:, 2] * G6[0] + 3 * d[:, 0] * G6[2] - 2 * d[:, 2] * G6[3]
+                                + 3 * d[:, 1] * G6[4] + 4 * d[:, 2] * G6[5])
+            r2 = (d * d).sum(1)
+            gM[ch] += (G6[0] * (3 * d[:, 0] ** 2 - r2) + G6[1] * 3 * d[:, 0] * d[:, 1]
+                       + G6[2] * 3 * d[:, 0] * d[:, 2] + G6[3] * (3 * d[:, 1] ** 2 - r2)
+                       + G6[4] * 3 * d[:, 1] * d[:, 2] + G6[5] * (3 * d[:, 2] ** 2 - r2))
+            cells = kind[ch] == 0
+            gQ[ch[cells]] += G6
+    leaves = np.nonzero(kind == 1)[0]
+    gx[ref[leaves]] += gcm[leaves]
+    gm[ref[leaves]] += gM[leaves]
+    return gx, gm

@@ -100,3 +100,37 @@ def test_reverse_sweep_equals_finite_differences(use_pot):
         assert abs(gm[i] - want) <= 1e-6 * max(abs(want), 1e-3), ("m", i, gm[i], want)
     want = fd(lambda t: loss(pos, vel, m, t[0]), np.array([dt]), 0, 1e-4 * dt)
     assert abs(gdt - want) <= 1e-6 * abs(want), ("dt", gdt, want)
+
+
+@pytest.mark.parametrize("criterion,quad", [("bh86", True), ("bh86", False), ("sw93", True)])
+def test_frozen_tree_vjp_equals_differences(criterion, quad):
+    """The frozen-tree force reproduces the oracle's tree walk, and its
+    vector-Jacobian product (as the GPU computes it) equals central
+    differences of that force along random directions in positions and
+    masses; with the lists frozen the force is smooth, so they converge."""
+    pos, vel, m = plummer_bodies(300, seed=21)
+    m = m * np.random.default_rng(1).uniform(0.5, 1.5, m.shape)
+    eps2 = 1e-4
+    o = orc.Oracle(orc.make_ctx(eps2=eps2, theta=0.6, criterion=criterion, use_quad=quad),
+                   pos, vel, m)
+    assert o.gravity() == 0
+    t = o.tree()
+    ch = ar.tree_children(t)
+    lists = ar.walk_lists(t)
+    a = ar.frozen_accel(pos, m, t, ch, lists, eps2, quad)
+    np.testing.assert_allclose(a, o.acc, rtol=0, atol=1e-12 * np.abs(o.acc).max())
+    rng = np.random.default_rng(2)
+    G = rng.normal(size=pos.shape)
+    gx, gm = ar.frozen_vjp(pos, m, t, ch, lists, eps2, G, quad)
+    # truncation error falls 100x per 10x step here (smooth frozen map): 1e-7 -> ~1e-9
+    for _ in range(3):
+        D = rng.normal(size=pos.shape)
+        h = 1e-7
+        fd = ((ar.frozen_accel(pos + h * D, m, t, ch, lists, eps2, quad)
+               - ar.frozen_accel(pos - h * D, m, t, ch, lists, eps2, quad)) * G).sum() / (2 * h)
+        assert abs((gx * D).sum() - fd) <= 1e-7 * abs(fd)
+    Dm = rng.normal(size=m.shape) * m
+    h = 1e-7
+    fd = ((ar.frozen_accel(pos, m + h * Dm, t, ch, lists, eps2, quad)
+           - ar.frozen_accel(pos, m - h * Dm, t, ch, lists, eps2, quad)) * G).sum() / (2 * h)
+    assert abs((gm * Dm).sum() - fd) <= 1e-7 * abs(fd)

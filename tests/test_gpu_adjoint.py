@@ -2,11 +2,13 @@
 
 Pinned against: the CPU reverse sweep of oracle/adjoint_ref.py (itself
 checked against finite differences) for the Exact criterion; the Exact
-adjoint again for the tree walk at theta = 0; single-coordinate central
-differences of the GPU forward run for the tree at theta = 0.5 (the tree is
-frozen in the adjoint, so the perturbation must leave every opening
-decision unchanged); individual runs for batched simulations; and the
-forward run itself for the checkpointed forward pass."""
+adjoint again for the tree walk at theta = 0; the numpy frozen-tree
+vector-Jacobian product on the oracle's identical tree for one force
+evaluation at theta = 0.6 (central differences of the real tree run cannot
+serve: moving bodies flips some of the ~10^7 opening tests and adds jumps
+that the frozen-tree derivative by definition leaves out); individual runs
+for batched simulations; and the forward run itself for the checkpointed
+forward pass."""
 import numpy as np
 import pytest
 
@@ -88,70 +90,37 @@ def test_checkpoint_interval_does_not_change_gradients():
             assert np.abs(a - b).max() <= 1e-12 * np.abs(b).max()
 
 
-@pytest.mark.parametrize("quad", [True, False])
-def test_tree_adjoint_matches_single_coordinate_differences(quad):
-    """Frozen-tree gradient vs central differences of the real forward run in
-    single coordinates.  Moving one body (or one mass, one velocity) leaves
-    the opening decisions of the other bodies' walks unchanged, so the
-    forward map is smooth across the difference; a perturbation of all bodies
-    at once flips some of the ~10^7 accept tests and adds jumps.  The steps
-    sit where the differences have converged (measured on this workload:
-    smaller steps are dominated by the forward run's own round-off, larger
-    all-body dt steps by decision flips)."""
-    n, steps, eps2, dt = 4000, 5, 1e-3, 1e-3
-    pos, vel, m = _dwarf(n, 4)
-    pot = nb.milky_way_potential()
-    rng = np.random.default_rng(2)
-    wx, wv = rng.normal(size=(n, 3)), rng.normal(size=(n, 3))
-    c0, v0 = pos.mean(0), vel.mean(0)
-    ctx = nb.NBodyCtx.create(timestep=dt, timeEvolve=dt * steps, eps2=eps2, theta=0.5,
-                             useQuad=quad)
-
-    def run(p, v, mm, ctx_=ctx):
-        st = nb.NBodyState(ctx_, pot, p, v, mm, prestep=False)
-        st.run(steps)
-        x, vv, _ = st.bodies()
-        st.close()
-        return (wx * (x - c0)).sum() + (wv * (vv - v0)).sum() + 0.5 * ((x - c0) ** 2).sum()
-
-    st = nb.NBodyState(ctx, pot, pos, vel, m, prestep=False)
-    st.adjoint_forward(steps, 2)
-    x, v, _ = st.bodies()
-    gx, gv, gm, gdt = st.adjoint_backward(wx + (x - c0), wv)
-
-    def bump(a, idx, h):
-        b = a.copy()
-        b[idx] += h
-        return b
-
-    fails = []
-    for i in (0, 17, 1234, 3999):
-        for d in range(3):
-            h = 1e-5
-            fd = (run(bump(pos, (i, d), h), vel, m) - run(bump(pos, (i, d), -h), vel, m)) / (2 * h)
-            print(f"x[{i},{d}] adjoint {gx[i, d]:.10g} difference {fd:.10g}")
-            if abs(fd - gx[i, d]) > 2e-6 * max(abs(fd), 1.0):
-                fails.append(("x", i, d, gx[i, d], fd))
-        h = 1e-4
-        fd = (run(pos, bump(vel, (i, 1), h), m) - run(pos, bump(vel, (i, 1), -h), m)) / (2 * h)
-        print(f"v[{i},1] adjoint {gv[i, 1]:.10g} difference {fd:.10g}")
-        if abs(fd - gv[i, 1]) > 2e-6 * max(abs(fd), 1e-3):
-            fails.append(("v", i, gv[i, 1], fd))
-        h = 1e-4 * m[i]
-        fd = (run(pos, vel, bump(m, i, h)) - run(pos, vel, bump(m, i, -h))) / (2 * h)
-        print(f"m[{i}] adjoint {gm[i]:.10g} difference {fd:.10g}")
-        if abs(fd - gm[i]) > 2e-6 * max(abs(fd), 1e-3):
-            fails.append(("m", i, gm[i], fd))
-    h = 1e-6
-    def ctx_dt(t):
-        return nb.NBodyCtx.create(timestep=t, timeEvolve=t * steps, eps2=eps2, theta=0.5,
-                                  useQuad=quad)
-    fd = (run(pos, vel, m, ctx_dt(dt * (1 + h))) - run(pos, vel, m, ctx_dt(dt * (1 - h))))
-    fd /= 2 * h * dt
-    print(f"dt adjoint {gdt[0]:.10g} difference {fd:.10g}")
-    if abs(fd - gdt[0]) > 1e-6 * abs(fd):
-        fails.append(("dt", gdt[0], fd))
-    assert not fails, fails
+@pytest.mark.parametrize("criterion,quad,use_pot", [("bh86", True, True), ("bh86", False, False),
+                                                    ("sw93", True, True), ("treecode", True, False)])
+def test_gravity_vjp_matches_frozen_tree_reference(criterion, quad, use_pot):
+    """One force adjoint on the GPU (walk replay, node records summed across
+    the wave, top-down hand-off, leaf scatter, potential Jacobian) against the
+    numpy frozen-tree VJP on the oracle's identical tree and interaction
+    lists (itself checked against central differences of the frozen-list
+    force in tests/test_adjoint_ref.py)."""
+    n, eps2 = 2000, 1e-3
+    pos, vel, m = _dwarf(n, 5) if use_pot else plummer_bodies(n, seed=5)
+    m = m * np.random.default_rng(7).uniform(0.5, 1.5, n)
+    pot = nb.milky_way_potential() if use_pot else None
+    ctx = nb.NBodyCtx.create(timestep=1e-3, timeEvolve=1.0, eps2=eps2, theta=0.6,
+                             criterion=criterion, useQuad=quad)
+    st = nb.NBodyState(ctx, pot, pos, vel, m)
+    G = np.random.default_rng(8).normal(size=(n, 3))
+    gx, gm = st.gravity_vjp(G)
+    opot = orc.milky_way() if use_pot else None
+    o = orc.Oracle(orc.make_ctx(eps2=eps2, theta=0.6, criterion=criterion, use_quad=quad,
+                                potential=opot), pos, vel, m)
+    assert o.gravity() == 0
+    t = o.tree()
+    ch = ar.tree_children(t)
+    lists = ar.walk_lists(t)
+    rx, rm = ar.frozen_vjp(pos, m, t, ch, lists, eps2, G, quad)
+    if use_pot:
+        rx = rx + ar.ext_jac(ar.pot_params(opot), pos, G)
+    ex = np.abs(gx - rx).max() / np.abs(rx).max()
+    em = np.abs(gm - rm).max() / np.abs(rm).max()
+    print(criterion, quad, use_pot, "max rel: pos", ex, "mass", em)
+    assert ex <= 1e-11 and em <= 1e-11
 
 
 def test_tree_adjoint_at_theta_zero_equals_exact_adjoint():

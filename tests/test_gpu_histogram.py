@@ -1,0 +1,67 @@
+"""GPU lambda-beta histogram (nbhip_histogram) against the host restatement
+of nbCreateHistogram's binning (milkywayathome_client_amd/histogram.py,
+itself pinned by tests/test_histogram.py): identical counts and totals,
+for one state and per simulation of a batch, with ignored bodies left out.
+Bodies within 1e-9 degrees of a bin edge could fall either side (device and
+host atan2/asin may differ in the last bit); the workloads below have none."""
+import numpy as np
+import pytest
+
+from helpers import plummer_bodies
+from milkywayathome_client_amd import histogram as H
+from milkywayathome_client_amd import nbody as nb
+
+pytestmark = pytest.mark.gpu
+
+
+def _stream(n, seed, centre):
+    rng = np.random.default_rng(seed)
+    # bodies spread along a ring through the frame so many bins are filled
+    t = rng.uniform(0, 2 * np.pi, n)
+    pos = np.stack([30 * np.cos(t), 30 * np.sin(t), rng.normal(0, 3, n)], 1) + centre
+    vel = rng.normal(0, 10, (n, 3))
+    return pos, vel, np.full(n, 1e-3)
+
+
+def _edges_clear(pos, hp):
+    lam, beta = H.lambda_beta(pos, hp)
+    w = (hp.lambdaEnd - hp.lambdaStart) / hp.lambdaBins
+    fl = (lam - hp.lambdaStart) / w
+    return np.all(np.abs(fl - np.round(fl)) > 1e-9)
+
+
+@pytest.mark.parametrize("beta_bins", [1, 3])
+def test_histogram_matches_host(beta_bins):
+    hp = H.HistogramParams(betaBins=beta_bins)
+    pos, vel, m = _stream(20000, 1, (0.0, 0.0, 0.0))
+    ignore = np.zeros(len(m), dtype=bool)
+    ignore[::3] = True
+    assert _edges_clear(pos, hp)
+    ctx = nb.NBodyCtx.create(timestep=1e-3, timeEvolve=1.0, eps2=1e-3, theta=0.5)
+    st = nb.NBodyState(ctx, None, pos, vel, m, prestep=False)
+    st.set_ignore(ignore)
+    counts, totals = st.histogram(hp)
+    ref, tot = H.create_histogram(pos, hp, ignore=ignore)
+    assert totals[0] == tot
+    np.testing.assert_array_equal(counts[0], np.round(ref * tot).astype(np.int64))
+    st.set_ignore(None)
+    counts2, totals2 = st.histogram(hp)
+    ref2, tot2 = H.create_histogram(pos, hp)
+    assert totals2[0] == tot2 and totals2[0] > totals[0]
+
+
+def test_batch_histograms_per_simulation():
+    hp = H.HistogramParams()
+    members = [_stream(k, 10 + i, (5.0 * i, 0.0, 0.0)) for i, k in enumerate([900, 7000, 1, 4000])]
+    ctx = nb.NBodyCtx.create(timestep=1e-3, timeEvolve=1.0, eps2=1e-3, theta=0.5)
+    bt = nb.NBodyBatch(ctx, None, members, prestep=False)
+    ignore = np.concatenate([np.arange(len(mm[2])) % 2 == 1 for mm in members])
+    bt.set_ignore(ignore)
+    counts, totals = bt.histogram(hp)
+    off = 0
+    for k, (pos, _, m) in enumerate(members):
+        ig = ignore[off:off + len(m)]
+        ref, tot = H.create_histogram(pos, hp, ignore=ig)
+        assert totals[k] == tot, k
+        np.testing.assert_array_equal(counts[k], np.round(ref * tot).astype(np.int64))
+        off += len(m)
