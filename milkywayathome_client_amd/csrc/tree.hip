@@ -832,6 +832,7 @@ __global__ __launch_bounds__(256) void k_emit_leaves(const int* __restrict__ blo
                                                      int* __restrict__ nparent,
                                                      int* __restrict__ nsim,
                                                      SimTab* __restrict__ sims,
+                                                     NodeF* __restrict__ nodef,
                                                      DevScalars* __restrict__ scal)
 {
     const int s = blockIdx.x * 256 + threadIdx.x;
@@ -866,6 +867,18 @@ __global__ __launch_bounds__(256) void k_emit_leaves(const int* __restrict__ blo
     nlevel[pre] = -1;
     nparent[pre] = parent;
     nsim[pre] = sim;
+    if (nodef) {
+        NodeF f;
+        nbh_split(p.x, f.hx, f.lx);
+        nbh_split(p.y, f.hy, f.ly);
+        nbh_split(p.z, f.hz, f.lz);
+        f.m = (float) p.w;
+        f.rcrit2 = -1.0f;
+        f.next = pre + 1;
+        f.body = s;
+        for (int k = 0; k < 6; ++k) f.q[k] = 0.0f;
+        nodef[pre] = f;
+    }
 }
 
 /* Cells grouped by level: exclusive scan of the per-level counts, then
@@ -881,25 +894,29 @@ __global__ void k_level_scan(DevScalars* scal)
     scal->level_off[64] = acc;
 }
 
+/* Grid-stride over the nodes actually built (the grid is sized for a
+ * typical tree, not the table's capacity). */
 __global__ __launch_bounds__(256) void k_level_scatter(const int* __restrict__ nlevel,
                                                        int* __restrict__ level_list,
                                                        DevScalars* __restrict__ scal)
 {
     __shared__ int lh[64];
     __shared__ int base[64];
-    if (threadIdx.x < 64) lh[threadIdx.x] = 0;
-    __syncthreads();
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    const bool ok = !(scal->status & (NBHIP_CELL_OVERFLOW_ERROR | NBHIP_ERROR)) &&
-                    i < scal->nnode;
-    const int l = ok ? nlevel[i] : -1;
-    const bool cell = l >= 0 && l <= 63;
-    const int rank = cell ? atomicAdd(&lh[l], 1) : 0;
-    __syncthreads();
-    if (threadIdx.x < 64 && lh[threadIdx.x])
-        base[threadIdx.x] = atomicAdd(&scal->level_fill[threadIdx.x], lh[threadIdx.x]);
-    __syncthreads();
-    if (cell) level_list[scal->level_off[l] + base[l] + rank] = i;
+    const int nn = (scal->status & (NBHIP_CELL_OVERFLOW_ERROR | NBHIP_ERROR)) ? 0 : scal->nnode;
+    for (int b0 = blockIdx.x * 256; b0 < nn; b0 += gridDim.x * 256) {
+        if (threadIdx.x < 64) lh[threadIdx.x] = 0;
+        __syncthreads();
+        const int i = b0 + (int) threadIdx.x;
+        const int l = i < nn ? nlevel[i] : -1;
+        const bool cell = l >= 0 && l <= 63;
+        const int rank = cell ? atomicAdd(&lh[l], 1) : 0;
+        __syncthreads();
+        if (threadIdx.x < 64 && lh[threadIdx.x])
+            base[threadIdx.x] = atomicAdd(&scal->level_fill[threadIdx.x], lh[threadIdx.x]);
+        __syncthreads();
+        if (cell) level_list[scal->level_off[l] + base[l] + rank] = i;
+        __syncthreads();
+    }
 }
 
 /* ---------------------------------------------------------------------- */
@@ -916,6 +933,7 @@ __global__ __launch_bounds__(256) void k_summarize_level(const int* __restrict__
                                                          const double* __restrict__ nmid,
                                                          const int* __restrict__ nsim,
                                                          SimTab* __restrict__ sims,
+                                                         NodeF* __restrict__ nodef,
                                                          nbhip_ctx ctx, DevScalars* scal)
 {
     if (scal->status & (NBHIP_CELL_OVERFLOW_ERROR | NBHIP_ERROR)) return;
@@ -981,9 +999,16 @@ __global__ __launch_bounds__(256) void k_summarize_level(const int* __restrict__
             }
             double* pq = &nquad[6 * p];
             pq[0] = q0; pq[1] = q1; pq[2] = q2; pq[3] = q3; pq[4] = q4; pq[5] = q5;
+            if (nodef) {
+                NodeF& f = nodef[p];
+                f.q[0] = (float) q0; f.q[1] = (float) q1; f.q[2] = (float) q2;
+                f.q[3] = (float) q3; f.q[4] = (float) q4; f.q[5] = (float) q5;
+            }
         } else {
             double* pq = &nquad[6 * p];
             pq[0] = pq[1] = pq[2] = pq[3] = pq[4] = pq[5] = 0.0;
+            if (nodef)
+                for (int k = 0; k < 6; ++k) nodef[p].q[k] = 0.0f;
         }
         NodeW w;
         w.x = cx;
@@ -994,6 +1019,17 @@ __global__ __launch_bounds__(256) void k_summarize_level(const int* __restrict__
         w.next = end;
         w.body = -1;
         node[p] = w;
+        if (nodef) {
+            /* the mixed walk's record (absolute position split hi + lo) */
+            NodeF& f = nodef[p];
+            nbh_split(cx, f.hx, f.lx);
+            nbh_split(cy, f.hy, f.ly);
+            nbh_split(cz, f.hz, f.lz);
+            f.m = (float) mass;
+            f.rcrit2 = (float) rc2;
+            f.next = end;
+            f.body = -1;
+        }
     }
 }
 
@@ -1067,9 +1103,10 @@ void nbh_build(nbhip_state* st, int graph_safe)
                        st->nmid, st->nlevel, st->nparent, st->nsim_node, st->sims, st->scal);
     hipLaunchKernelGGL(k_emit_leaves, dim3(g256), dim3(256), 0, s, st->blo, st->bdelta,
                        st->bparent, st->dadj, st->cincl, st->spos, st->simid_s, st->node,
-                       st->nmid, st->nlevel, st->nparent, st->nsim_node, st->sims, st->scal);
+                       st->nmid, st->nlevel, st->nparent, st->nsim_node, st->sims, st->nodef,
+                       st->scal);
     hipLaunchKernelGGL(k_level_scan, dim3(1), dim3(1), 0, s, st->scal);
-    hipLaunchKernelGGL(k_level_scatter, dim3(nbh_blocks(st->node_cap, 256)), dim3(256), 0, s,
+    hipLaunchKernelGGL(k_level_scatter, dim3(nbh_blocks(2 * n + 1024, 256)), dim3(256), 0, s,
                        st->nlevel, st->level_list, st->scal);
     nbh_mark(st, 3);
     int counts[64];
@@ -1093,7 +1130,8 @@ void nbh_build(nbhip_state* st, int graph_safe)
         if (have && counts[l] == 0) continue;
         const int g = have ? nbh_blocks(counts[l], 256) : gmax;
         hipLaunchKernelGGL(k_summarize_level, dim3(g), dim3(256), 0, s, st->level_list, l,
-                           st->node, st->nquad, st->nmid, st->nsim_node, st->sims, st->ctx, st->scal);
+                           st->node, st->nquad, st->nmid, st->nsim_node, st->sims, st->nodef,
+                           st->ctx, st->scal);
     }
     nbh_mark(st, 4);
 }

@@ -201,7 +201,7 @@ __device__ __forceinline__ void nbh_kahan_add(double& sum, double& comp, double 
     sum = t;
 }
 
-template <int BS, bool QUAD, bool COUNT, bool COH, bool MULTI>
+template <int BS, bool QUAD, bool COUNT, bool COH, bool MULTI, bool EPS0>
 __global__ __launch_bounds__(BS) void k_walk32(const NodeF* __restrict__ node,
                                                const NodeW* __restrict__ nodew,
                                                const double4* __restrict__ spos,
@@ -225,16 +225,13 @@ __global__ __launch_bounds__(BS) void k_walk32(const NodeF* __restrict__ node,
     const LaneStart L = nbh_lane_start(s, valid, M, simid_s, sims);
     const int self = L.self, my_end = L.end, my_root = L.lp;
     int lp = L.lp;
-    /* offsets from the simulation's root centre of mass, split hi + lo */
+    /* the body's coordinates split hi + lo, as the node records hold them */
     const int root = sims[L.sim].root;
-    const NodeW r0 = nodew[root >= 0 ? root : 0];
-    const double ox = p.x - r0.x, oy = p.y - r0.y, oz = p.z - r0.z;
-    const float phx = (float) ox, phy = (float) oy, phz = (float) oz;
-    const float plx = (float) (ox - (double) phx);
-    const float ply = (float) (oy - (double) phy);
-    const float plz = (float) (oz - (double) phz);
+    float phx, phy, phz, plx, ply, plz;
+    nbh_split(p.x, phx, plx);
+    nbh_split(p.y, phy, ply);
+    nbh_split(p.z, phz, plz);
     const float eps2 = (float) L.eps2;
-    const bool eps2_pos = o.all_eps2_pos;
     const int bshift = o.batch_shift;
     const int bmask = (1 << bshift) - 1;
     const int qend = L.qend;
@@ -278,7 +275,7 @@ __global__ __launch_bounds__(BS) void k_walk32(const NodeF* __restrict__ node,
             met |= nd.body == self;
             pending = true;
             /* with eps2 = 0 the self term is skipped: d = 0 against 1 gives 0 */
-            const float d2 = (!eps2_pos && nd.body == self) ? 1.0f : drSq + eps2;
+            const float d2 = (EPS0 && nd.body == self) ? 1.0f : drSq + eps2;
             const float rinv = __builtin_amdgcn_rsqf(d2);
             const float rinv2 = rinv * rinv;
             const float mor3 = nd.m * rinv * rinv2;
@@ -349,40 +346,6 @@ __global__ __launch_bounds__(BS) void k_walk32(const NodeF* __restrict__ node,
     if (!valid) return;
     if (!met) atomicOr(&sims[L.sim].incest, 1u);
     nbh_walk_finish(o, p.x, p.y, p.z, ax, ay, az, perm[s], oax, oay, oaz, vx, vy, vz);
-}
-
-/* Mixed-precision walk records from the FP64 tree: offsets from the
- * simulation's root centre of mass, split hi + lo. */
-__global__ __launch_bounds__(256) void k_pack32(const NodeW* __restrict__ node,
-                                                const double* __restrict__ nquad,
-                                                const int* __restrict__ nsim,
-                                                const SimTab* __restrict__ sims,
-                                                NodeF* __restrict__ nodef,
-                                                const DevScalars* __restrict__ scal)
-{
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (scal->status & (NBHIP_CELL_OVERFLOW_ERROR | NBHIP_ERROR)) return;
-    if (i >= scal->nnode) return;
-    const NodeW r0 = node[sims[nsim[i]].root];
-    const NodeW w = node[i];
-    const double dx = w.x - r0.x, dy = w.y - r0.y, dz = w.z - r0.z;
-    NodeF f;
-    f.hx = (float) dx;
-    f.hy = (float) dy;
-    f.hz = (float) dz;
-    f.lx = (float) (dx - (double) f.hx);
-    f.ly = (float) (dy - (double) f.hy);
-    f.lz = (float) (dz - (double) f.hz);
-    f.m = (float) w.m;
-    f.rcrit2 = w.body >= 0 ? -1.0f : (float) w.rcrit2;
-    f.next = w.next;
-    f.body = w.body;
-    if (w.body < 0) {
-        for (int k = 0; k < 6; ++k) f.q[k] = (float) nquad[6 * i + k];
-    } else {
-        for (int k = 0; k < 6; ++k) f.q[k] = 0.0f;
-    }
-    nodef[i] = f;
 }
 
 /* nbMapForceBody_Exact (nbody/src/nbody_grav.c:229-293): O(N^2) in body
@@ -514,12 +477,13 @@ void nbh_walk(nbhip_state* st, int kick, int count, double* oax, double* oay, do
     dim3 grid(nbh_blocks(st->n, bs)), block(bs);
     if (c.precision != NBHIP_PRECISION_FP64) {
         const bool coherent = c.precision == NBHIP_PRECISION_MIXED_COHERENT;
-        hipLaunchKernelGGL(k_pack32, dim3(nbh_blocks(st->node_cap, 256)), dim3(256), 0, s,
-                           st->node, st->nquad, st->nsim_node, st->sims, st->nodef, st->scal);
-#define NBH_W32M(B, Q, C, H, MU)                                                             \
-    hipLaunchKernelGGL((k_walk32<B, Q, C, H, MU>), grid, block, 0, s, st->nodef, st->node,     \
+#define NBH_W32E(B, Q, C, H, MU, E0)                                                         \
+    hipLaunchKernelGGL((k_walk32<B, Q, C, H, MU, E0>), grid, block, 0, s, st->nodef, st->node, \
                        st->spos, st->perm, st->simid_s, st->sims, oax, oay, oaz, st->vx,       \
                        st->vy, st->vz, st->scal, o)
+#define NBH_W32M(B, Q, C, H, MU)                                                             \
+    do { if (o.all_eps2_pos) NBH_W32E(B, Q, C, H, MU, false);                                 \
+         else NBH_W32E(B, Q, C, H, MU, true); } while (0)
 #define NBH_W32C(B, Q, C, H)                                                                 \
     do { if (o.multi) NBH_W32M(B, Q, C, H, true); else NBH_W32M(B, Q, C, H, false); } while (0)
 #define NBH_W32(B, Q, C)                                                                     \
@@ -532,6 +496,7 @@ void nbh_walk(nbhip_state* st, int kick, int count, double* oax, double* oay, do
 #undef NBH_W32
 #undef NBH_W32C
 #undef NBH_W32M
+#undef NBH_W32E
         return;
     }
 #define NBH_W64(B, Q, C)                                                                     \
