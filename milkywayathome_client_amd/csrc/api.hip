@@ -96,7 +96,7 @@ static void nbh_free_all(nbhip_state* st)
                      st->simid_s, st->su,  st->dadj,  st->cincl,   st->scan_part, st->blo,
                      st->bhi,  st->bdelta, st->bparent, st->node,  st->nquad,  st->nmid,
                      st->nlevel, st->nparent, st->nsim_node, st->level_list, st->nodef,
-                     st->scal, st->sims, st->ignore, st->hist_buf };
+                     st->scal, st->sims, st->ignore, st->hist_buf, st->pos4 };
     for (void* p : ptrs)
         if (p) (void) hipFree(p);
     if (st->graph_exec) (void) hipGraphExecDestroy(st->graph_exec);
@@ -188,7 +188,7 @@ static unsigned nbh_create(const nbhip_ctx* ctx, int device, int nsim, const int
     A(key, n); A(key_alt, n); A(perm, n); A(perm_alt, n);
     A(hist, 256 * (size_t) st->sort_blocks);
     A(dtot, 256);
-    A(spos, n); A(simid_s, n); A(su, 3 * n); A(dadj, n); A(cincl, n);
+    A(pos4, n); A(spos, n); A(simid_s, n); A(su, 3 * n); A(dadj, n); A(cincl, n);
     A(scan_part, 4096);
     A(blo, n); A(bhi, n); A(bdelta, n); A(bparent, 2 * n);
     A(node, nc); A(nquad, 6 * nc); A(nmid, 3 * nc); A(nlevel, nc); A(nparent, nc);

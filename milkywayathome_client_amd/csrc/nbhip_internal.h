@@ -82,7 +82,7 @@ struct WalkTune {
     int batch;                     /* mixed walk: node block per FP32 partial sum (power of 2) */
 };
 
-#define NBH_SORT_TILE 4096   /* keys per radix-sort block: 256 threads x 16 */
+#define NBH_SORT_TILE 2048   /* keys per radix-sort block: 256 threads x 8 */
 #define NBH_SCAN_TILE 4096   /* items per scan block: 1024 threads x 4 */
 
 /* Adjoint bookkeeping (FP64 states only): checkpoints of the forward run
@@ -121,9 +121,10 @@ struct nbhip_state {
     unsigned *hist;
     unsigned *dtot;               /* per-digit totals of one radix pass */
 
+    double4 *pos4;                /* x, y, z, m per body, original order (build input) */
     double4 *spos;
     int *simid_s;                 /* simulation of each sorted body */
-    unsigned long long *su;       /* 3 * n: ux | uy | uz */
+    unsigned long long *su;       /* 3 per sorted body: ux, uy, uz */
     int *dadj;
     unsigned *cincl;              /* inclusive scan of cells-per-first-body */
     unsigned *scan_part;

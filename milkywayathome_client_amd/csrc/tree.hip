@@ -51,6 +51,8 @@ __global__ __launch_bounds__(256) void k_drift_bbox(int n, int drift, double dt,
                                                     const double* __restrict__ ax,
                                                     const double* __restrict__ ay,
                                                     const double* __restrict__ az,
+                                                    const double* __restrict__ m,
+                                                    double4* __restrict__ pos4,
                                                     const int* __restrict__ simid,
                                                     SimTab* __restrict__ sims)
 {
@@ -72,6 +74,7 @@ __global__ __launch_bounds__(256) void k_drift_bbox(int n, int drift, double dt,
             vx[i] = ux; vy[i] = uy; vz[i] = uz;
             px[i] = x; py[i] = y; pz[i] = z;
         }
+        pos4[i] = make_double4(x, y, z, m[i]);     /* gathered by the tree build */
         const int sim = simid[i];
         if (sim != cur) {
             if (cur >= 0) nbh_max_bits(sims, cur, mx);
@@ -383,10 +386,9 @@ __global__ __launch_bounds__(256) void k_tiefix(const unsigned long long* __rest
 /* sorted gather, adjacent prefixes, cells-per-first-body                  */
 /* ---------------------------------------------------------------------- */
 __global__ __launch_bounds__(256) void k_gather(int n, const unsigned* __restrict__ perm,
-                                                const double* __restrict__ px,
-                                                const double* __restrict__ py,
-                                                const double* __restrict__ pz,
-                                                const double* __restrict__ m,
+                                                const double4* __restrict__ pos4,
+                                                const unsigned long long* __restrict__ key,
+                                                int sim_shift,
                                                 const int* __restrict__ simid,
                                                 const SimTab* __restrict__ sims,
                                                 double4* __restrict__ spos,
@@ -397,15 +399,19 @@ __global__ __launch_bounds__(256) void k_gather(int n, const unsigned* __restric
     const int s = blockIdx.x * 256 + threadIdx.x;
     if (s >= n) return;
     const unsigned j = perm[s];
-    const double x = px[j], y = py[j], z = pz[j];
-    const int sim = simid[j];
-    spos[s] = make_double4(x, y, z, m[j]);
+    /* one 32-byte record per body; the simulation of a massive body is the
+     * top of its sorted key (massless bodies carry an all-ones key) */
+    const double4 p = pos4[j];
+    const double x = p.x, y = p.y, z = p.z;
+    const bool massive = s < scal->nmassive;
+    const int sim = (sim_shift >= 64) ? 0 : (massive ? (int) (key[s] >> sim_shift) : simid[j]);
+    spos[s] = p;
     simid_s[s] = sim;
-    if (s < scal->nmassive) {
+    if (massive) {
         const double ks = sims[sim].key_scale;
-        su[s] = nbh_fixed(x, ks);
-        su[n + s] = nbh_fixed(y, ks);
-        su[2 * n + s] = nbh_fixed(z, ks);
+        su[3 * (size_t) s + 0] = nbh_fixed(x, ks);
+        su[3 * (size_t) s + 1] = nbh_fixed(y, ks);
+        su[3 * (size_t) s + 2] = nbh_fixed(z, ks);
     }
 }
 
@@ -419,7 +425,9 @@ __device__ __forceinline__ int nbh_kdelta(const unsigned long long* __restrict__
     if (j < 0 || j >= M) return -1;
     const int si = simid_s[i], sj = simid_s[j];
     if (si != sj) return __clz((unsigned) (si ^ sj)) - (32 - sb);
-    int d = nbh_prefix_bits(su[i], su[n + i], su[2 * n + i], su[j], su[n + j], su[2 * n + j]);
+    const unsigned long long* a = su + 3 * (size_t) i;
+    const unsigned long long* b = su + 3 * (size_t) j;
+    int d = nbh_prefix_bits(a[0], a[1], a[2], b[0], b[1], b[2]);
     if (d == 189) d = 189 + __clz((unsigned) (i ^ j));
     return sb + d;
 }
@@ -769,7 +777,8 @@ __device__ __forceinline__ void nbh_emit_cells_body(
     const int next = hi + 1 + (int) cincl[hi];
     const int sim = simid_s[lo];
     const double R = sims[sim].rsize;
-    const unsigned long long ux = su[lo], uy = su[n + lo], uz = su[2 * n + lo];
+    const unsigned long long ux = su[3 * (size_t) lo], uy = su[3 * (size_t) lo + 1],
+                             uz = su[3 * (size_t) lo + 2];
     const int nn = scal->nnode;
     int top_parent = -1;
     if (Lp + 1 > 0) {
@@ -1044,7 +1053,7 @@ void nbh_bbox(nbhip_state* st, int drift)
     hipLaunchKernelGGL(k_drift_bbox, dim3(nbh_blocks(st->n, NBH_BBOX_CHUNK)), dim3(256), 0,
                        st->stream,
                        st->n, drift, dt, 0.5 * dt, st->px, st->py, st->pz, st->vx, st->vy, st->vz,
-                       st->ax, st->ay, st->az, st->simid, st->sims);
+                       st->ax, st->ay, st->az, st->m, st->pos4, st->simid, st->sims);
     hipLaunchKernelGGL(k_rsize, dim3(1), dim3(256), 0, st->stream, st->nsim, st->sims, st->scal);
 }
 
@@ -1082,8 +1091,9 @@ void nbh_build(nbhip_state* st, int graph_safe)
                        st->pz, st->simid, st->sims, st->top_levels, st->scal);
     nbh_mark(st, 2);
 
-    hipLaunchKernelGGL(k_gather, dim3(g256), dim3(256), 0, s, n, st->perm, st->px, st->py, st->pz,
-                       st->m, st->simid, st->sims, st->spos, st->simid_s, st->su, st->scal);
+    hipLaunchKernelGGL(k_gather, dim3(g256), dim3(256), 0, s, n, st->perm, st->pos4, st->key,
+                       st->sim_shift, st->simid, st->sims, st->spos, st->simid_s, st->su,
+                       st->scal);
     hipLaunchKernelGGL(k_delta_count, dim3(g256), dim3(256), 0, s, n, st->su, st->simid_s,
                        st->dadj, st->cincl, st->sims, st->scal);
     const int nscan = nbh_blocks(n, NBH_SCAN_TILE);
