@@ -28,24 +28,16 @@ struct __attribute__((aligned(16))) NodeW {
 };
 
 /* Walk record of the mixed-precision walk (64 B, one scalar load).
- * Position = (hx + lx, hy + ly, hz + lz): the FP64 coordinate split into two
- * floats (~48 significant bits), so the float difference of a record and a
- * body, formed as (hx - bhx) + (lx - blx), keeps ~1e-7 relative accuracy
- * however far both are from the origin.  rcrit2 is -1 for bodies (always
- * accepted); q[6] is the quadrupole in FP32 (zero for bodies). */
+ * The position stays FP64: the walk forms each displacement as one FP64
+ * subtraction rounded once to FP32, exact to FP32 however far the record and
+ * the body are from the origin.  rcrit2 is -1 for bodies (always accepted);
+ * q[6] is the quadrupole in FP32 (zero for bodies). */
 struct __attribute__((aligned(16))) NodeF {
-    float hx, hy, hz, m;
-    float lx, ly, lz, rcrit2;
+    double x, y, z;
+    float m, rcrit2;
     int next, body;
     float q[6];
 };
-
-/* Split of one FP64 coordinate into hi + lo floats. */
-__host__ __device__ inline void nbh_split(double v, float& hi, float& lo)
-{
-    hi = (float) v;
-    lo = (float) (v - (double) hi);
-}
 
 /* Per-simulation device record (one per dwarf of a batch; one for a single
  * simulation). */

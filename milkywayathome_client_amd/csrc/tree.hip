@@ -878,9 +878,9 @@ __global__ __launch_bounds__(256) void k_emit_leaves(const int* __restrict__ blo
     nsim[pre] = sim;
     if (nodef) {
         NodeF f;
-        nbh_split(p.x, f.hx, f.lx);
-        nbh_split(p.y, f.hy, f.ly);
-        nbh_split(p.z, f.hz, f.lz);
+        f.x = p.x;
+        f.y = p.y;
+        f.z = p.z;
         f.m = (float) p.w;
         f.rcrit2 = -1.0f;
         f.next = pre + 1;
@@ -1029,11 +1029,11 @@ __global__ __launch_bounds__(256) void k_summarize_level(const int* __restrict__
         w.body = -1;
         node[p] = w;
         if (nodef) {
-            /* the mixed walk's record (absolute position split hi + lo) */
+            /* the mixed walk's record: FP64 position, the rest in FP32 */
             NodeF& f = nodef[p];
-            nbh_split(cx, f.hx, f.lx);
-            nbh_split(cy, f.hy, f.ly);
-            nbh_split(cz, f.hz, f.lz);
+            f.x = cx;
+            f.y = cy;
+            f.z = cz;
             f.m = (float) mass;
             f.rcrit2 = (float) rc2;
             f.next = end;

@@ -225,12 +225,7 @@ __global__ __launch_bounds__(BS) void k_walk32(const NodeF* __restrict__ node,
     const LaneStart L = nbh_lane_start(s, valid, M, simid_s, sims);
     const int self = L.self, my_end = L.end, my_root = L.lp;
     int lp = L.lp;
-    /* the body's coordinates split hi + lo, as the node records hold them */
     const int root = sims[L.sim].root;
-    float phx, phy, phz, plx, ply, plz;
-    nbh_split(p.x, phx, plx);
-    nbh_split(p.y, phy, ply);
-    nbh_split(p.z, phz, plz);
     const float eps2 = (float) L.eps2;
     const int bshift = o.batch_shift;
     const int bmask = (1 << bshift) - 1;
@@ -241,7 +236,7 @@ __global__ __launch_bounds__(BS) void k_walk32(const NodeF* __restrict__ node,
                           __builtin_amdgcn_ballot_w64(valid && root != root_u) == 0ull;
     float sx = 0.0f, sy = 0.0f, sz = 0.0f;
     double ax = 0.0, ay = 0.0, az = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
-    bool met = false, pending = false;
+    unsigned metv = ~0u;   /* 0 once the body met its own record */
     unsigned nbb = 0, nbc = 0, nop = 0, steps = 0;
     int q = L.q0;
 
@@ -251,9 +246,9 @@ __global__ __launch_bounds__(BS) void k_walk32(const NodeF* __restrict__ node,
      * past the last node); the summaries already reject broken Next links. */
     auto step = [&](const NodeF& nd, NodeF& nxt) {
         if (COUNT) ++steps;
-        const float dx = (nd.hx - phx) + (nd.lx - plx);
-        const float dy = (nd.hy - phy) + (nd.ly - ply);
-        const float dz = (nd.hz - phz) + (nd.lz - plz);
+        const float dx = (float) (nd.x - p.x);
+        const float dy = (float) (nd.y - p.y);
+        const float dz = (float) (nd.z - p.z);
         const float drSq = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
         bool accept, any_open;
         if (COH) {
@@ -272,8 +267,7 @@ __global__ __launch_bounds__(BS) void k_walk32(const NodeF* __restrict__ node,
         const int qn = min(max(any_open ? q + 1 : nd.next, q + 1), qend);
         nxt = node[qn];
         if (accept) {
-            met |= nd.body == self;
-            pending = true;
+            metv = min(metv, (unsigned) (nd.body ^ self));
             /* with eps2 = 0 the self term is skipped: d = 0 against 1 gives 0 */
             const float d2 = (EPS0 && nd.body == self) ? 1.0f : drSq + eps2;
             const float rinv = __builtin_amdgcn_rsqf(d2);
@@ -327,15 +321,16 @@ __global__ __launch_bounds__(BS) void k_walk32(const NodeF* __restrict__ node,
             step(ndb, nda);
             if (q >= lim) break;
         }
-        /* lanes whose own block changed close their FP32 partial sum */
+        /* lanes whose own block changed close their FP32 partial sum (a sum
+         * that is exactly zero leaves the FP64 state alone: the rule depends
+         * on the lane's own interactions only, and it needs no per-step flag) */
         const int rb = one_root ? root_u : root;
         const bool crossed = ((q - rb) >> bshift) != ((qstart - rb) >> bshift);
-        if (pending && crossed) {
+        if (crossed && (sx != 0.0f || sy != 0.0f || sz != 0.0f)) {
             nbh_kahan_add(ax, cx, (double) sx);
             nbh_kahan_add(ay, cy, (double) sy);
             nbh_kahan_add(az, cz, (double) sz);
             sx = sy = sz = 0.0f;
-            pending = false;
         }
     }
     nbh_kahan_add(ax, cx, (double) sx);
@@ -344,7 +339,7 @@ __global__ __launch_bounds__(BS) void k_walk32(const NodeF* __restrict__ node,
 
     if (COUNT) nbh_walk_counters(scal, nbb, nbc, nop, steps);
     if (!valid) return;
-    if (!met) atomicOr(&sims[L.sim].incest, 1u);
+    if (metv != 0u) atomicOr(&sims[L.sim].incest, 1u);
     nbh_walk_finish(o, p.x, p.y, p.z, ax, ay, az, perm[s], oax, oay, oaz, vx, vy, vz);
 }
 
