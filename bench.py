@@ -307,6 +307,74 @@ def run_adjoint(args):
     print(json.dumps(line), flush=True)
 
 
+def run_precision(args):
+    """Precision workload: the N = 10^6 two-component dwarf advanced 1000
+    steps with each form (FP64, mixed, coherent mixed).  At steps 0, 100
+    and 1000, 4096 sampled bodies are checked against the FP64 direct sum
+    (median and 99th percentile); the mixed forms' step-0 accelerations are
+    compared body by body with the FP64 tree's; the final lambda histograms
+    of the baryons (dark matter ignored) are compared with the FP64 run's by
+    EMD."""
+    from milkywayathome_client_amd import histogram as H, nbody as nb
+    log(f"[bench] precision workload: building N={args.n}")
+    pot, pos, vel, m, eps2, dt = make_workload(args.n, args.seed)
+    n = args.n
+    nbar = int(round(n * 0.5))
+    ignore = np.arange(n) >= nbar
+    hp = H.HistogramParams()
+    checkpoints = [0, 100, 1000]
+    out, hists, acc0 = {}, {}, {}
+    for prec in ("fp64", "mixed", "mixed_coherent"):
+        ctx = nb.NBodyCtx.create(timestep=dt, timeEvolve=dt * checkpoints[-1], eps2=eps2,
+                                 theta=args.theta, criterion="BH86", useQuad=True,
+                                 treeRSize=4.0)
+        st = nb.NBodyState(ctx, pot, pos, vel, m, precision=prec)
+        tune = None
+        if prec != "fp64" and not args.no_autotune:
+            best, _ = st.autotune()
+            tune = {"block": best[0], "batch": best[1]}
+            st.gravity()        # step-0 forces with the tuned settings
+        st.set_ignore(ignore)
+        acc0[prec] = st.bodies()[2]
+        errs, done, t_run = {}, 0, 0.0
+        for c in checkpoints:
+            if c > done:
+                st.sync()
+                t0 = time.perf_counter()
+                st.steps(c - done)
+                st.sync()
+                t_run += time.perf_counter() - t0
+                done = c
+            errs[f"step_{c}"] = accel_errors(st, pot, n)
+            log(f"[bench] {prec} step {c}: {errs[f'step_{c}']}")
+        counts, totals = st.histogram(hp)
+        hists[prec] = counts[0].astype(np.float64) / max(int(totals[0]), 1)
+        out[prec] = {"errors_vs_fp64_direct": errs, "walk_tune": tune,
+                     "ms_per_step": 1e3 * t_run / checkpoints[-1],
+                     "status": st.last_status}
+        st.close()
+    for prec in ("mixed", "mixed_coherent"):
+        a, b = acc0[prec], acc0["fp64"]
+        rel = np.linalg.norm(a - b, axis=1) / np.linalg.norm(b, axis=1)
+        out[prec]["step0_vs_fp64_tree"] = {"median": float(np.median(rel)),
+                                           "p99": float(np.percentile(rel, 99)),
+                                           "max": float(rel.max())}
+        out[prec]["final_histogram_emd_vs_fp64"] = H.emd(hists[prec], hists["fp64"], hp)
+    line = {
+        "metric": "per-body acceleration error vs FP64 direct sum at steps 0/100/1000, "
+                  "final-histogram EMD vs the FP64 run",
+        "n_gpus": 1,
+        "data": "synthetic (two-component dwarf ICs sampled in-process)",
+        "config": {"workload": f"two-component dwarf N={n} on its orbit in the MW potential, "
+                               f"BH86 theta={args.theta} + quadrupoles, dt={dt}, "
+                               f"{checkpoints[-1]} steps",
+                   "n": n, "seed": args.seed, "eps2": eps2, "nsample": 4096,
+                   "histogram": "50 lambda bins in [-150, 150], 1 beta bin, baryons only"},
+        "results": out,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def load_traffic():
     path = os.path.join(ROOT, "profiles", "walk_traffic.json")
     if not os.path.exists(path):
@@ -336,7 +404,8 @@ def main():
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run a plain device-to-device copy of BYTES (profiler "
                          "calibration of FETCH_SIZE / WRITE_SIZE) and exit")
-    ap.add_argument("--workload", choices=["single", "batch", "adjoint"], default="single",
+    ap.add_argument("--workload", choices=["single", "batch", "adjoint", "precision"],
+                    default="single",
                     help="single: the N=1M dwarf (the headline metric); batch: many "
                          "independent dwarfs per launch; adjoint: the gradient workload")
     ap.add_argument("--adj-n", type=int, default=20000)
@@ -352,7 +421,8 @@ def main():
         if args.gpus != 1:
             print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
             sys.exit(2)
-        (run_batch if args.workload == "batch" else run_adjoint)(args)
+        {"batch": run_batch, "adjoint": run_adjoint,
+         "precision": run_precision}[args.workload](args)
         return
     if args.calibrate_copy:
         import torch
