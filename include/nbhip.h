@@ -250,6 +250,18 @@ unsigned nbhip_adjoint_forward(nbhip_state* st, unsigned nsteps, unsigned every)
  * fixed: given dL/dacc (n x 3) returns dL/dpos (n x 3) and dL/dmass (n),
  * either may be NULL.  The building block of nbhip_adjoint_backward. */
 unsigned nbhip_gravity_vjp(nbhip_state* st, const double* gacc, double* gpos, double* gmass);
+/* The reverse orbit of the dwarf's centre as a differentiable function of
+ * the reversal time: nbReverseOrbit's KDK scheme
+ * (nbody/src/nbody_orbit_integrator.c:44-86) with a fixed step count, step
+ * t_rev / nsteps, in the external potential `pot` (the same expressions as
+ * the force path's).  Writes the reversed end point (out_pos, out_vel, each
+ * 3 doubles) and, by forward-mode differentiation through every step, its
+ * derivative with respect to t_rev (dpos, dvel; either may be NULL).  Host
+ * code (one body), no device needed.  No counterpart in the reference, whose
+ * orbit is not differentiated. */
+unsigned nbhip_orbit_tangent(const nbhip_potential* pot, const double* pos, const double* vel,
+                             double t_rev, int nsteps, double* out_pos, double* out_vel,
+                             double* dpos, double* dvel);
 unsigned nbhip_adjoint_backward(nbhip_state* st, const double* gpos, const double* gvel,
                                 double* gpos0, double* gvel0, double* gmass, double* gdt);
 

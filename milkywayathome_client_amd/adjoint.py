@@ -158,6 +158,24 @@ def reverse_orbit_t(pot: nb.Potential, pos, vel, t_rev: torch.Tensor, nsteps: in
     return x, -v
 
 
+class ReverseOrbit(torch.autograd.Function):
+    """reverse_orbit_t through the library: the end point and its derivative
+    with respect to t_rev come from one forward-mode pass in native code
+    (nbhip_orbit_tangent) instead of a torch graph of every step."""
+
+    @staticmethod
+    def forward(ctx, t_rev, pos, vel, pot: nb.Potential, nsteps: int):
+        x, v, dx, dv = nb.orbit_tangent(pot, pos.detach().numpy(), vel.detach().numpy(),
+                                        float(t_rev), nsteps)
+        ctx.save_for_backward(torch.from_numpy(dx), torch.from_numpy(dv))
+        return torch.from_numpy(x), torch.from_numpy(v)
+
+    @staticmethod
+    def backward(ctx, gx, gv):
+        dx, dv = ctx.saved_tensors
+        return (gx * dx).sum() + (gv * dv).sum(), None, None, None, None
+
+
 # ---------------------------------------------------------------------------
 # Differentiable two-Plummer dwarf
 # ---------------------------------------------------------------------------
@@ -273,7 +291,7 @@ class StreamModel:
         T, t_rev = dwarf_parameters(p)[:2]
         x0 = torch.as_tensor(models.lbr_to_cartesian(list(self.orbit_pos)), dtype=F64)
         v0 = torch.tensor(self.orbit_vel, dtype=F64)
-        cx, cv = reverse_orbit_t(self.potential, x0, v0, t_rev, self.orbit_steps)
+        cx, cv = ReverseOrbit.apply(t_rev, x0, v0, self.potential, self.orbit_steps)
         x, v, m = dwarf_ic(p, self.draws, cx, cv)
         return x, v, m, T / self.cfg.nsteps
 

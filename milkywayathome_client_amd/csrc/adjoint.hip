@@ -26,7 +26,7 @@
  *   3. k_adj_body adds each leaf's record to its body and, inside a step,
  *      applies the adjoint of the drift and the opening half-kick.
  * The external potential adds J_ext g per body (nbh_ext_jac).  The Exact
- * criterion uses the O(N^2) form in k_adj_exact.
+ * criterion uses the O(N^2) form (exact.hip).
  */
 #include "nbhip_device.h"
 
@@ -319,55 +319,6 @@ __global__ __launch_bounds__(256) void k_adj_body(int n, int tree, const unsigne
     if (drift_kick) nbh_sim_add(gdt, sim, valid, contrib);
 }
 
-/* Exact criterion: (J^T g)_i = sum_k T(d_ik) (m_i g_k - m_k g_i),
- * dL/dm_i = -sum_k g_k . d_ik / s^3, with d_ik = x_k - x_i and
- * T(d) = I / s^3 - 3 d d^T / s^5 (the self term vanishes identically). */
-__global__ __launch_bounds__(256) void k_adj_exact(int n, const double* __restrict__ px,
-                                                   const double* __restrict__ py,
-                                                   const double* __restrict__ pz,
-                                                   const double* __restrict__ m,
-                                                   const int* __restrict__ simid,
-                                                   const SimTab* __restrict__ sims,
-                                                   double* __restrict__ g, int use_pot,
-                                                   nbhip_potential pot)
-{
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    const size_t N = (size_t) n;
-    const double* ga = g + 6 * N;
-    const double x0 = px[i], y0 = py[i], z0 = pz[i], mi = m[i];
-    const double gi0 = ga[i], gi1 = ga[N + i], gi2 = ga[2 * N + i];
-    const SimTab& t = sims[simid[i]];
-    const double eps2 = t.eps2;
-    double ox = 0.0, oy = 0.0, oz = 0.0, om = 0.0;
-    for (int k = t.body_off; k < t.body_off + t.body_n; ++k) {
-        const double dx = px[k] - x0, dy = py[k] - y0, dz = pz[k] - z0;
-        const double s2 = dx * dx + dy * dy + dz * dz + eps2;
-        const double s3 = 1.0 / (s2 * sqrt(s2));
-        const double s5 = s3 / s2;
-        const double gk0 = ga[k], gk1 = ga[N + k], gk2 = ga[2 * N + k];
-        const double w0 = mi * gk0 - m[k] * gi0;
-        const double w1 = mi * gk1 - m[k] * gi1;
-        const double w2 = mi * gk2 - m[k] * gi2;
-        const double dw = dx * w0 + dy * w1 + dz * w2;
-        ox += s3 * w0 - 3.0 * s5 * dw * dx;
-        oy += s3 * w1 - 3.0 * s5 * dw * dy;
-        oz += s3 * w2 - 3.0 * s5 * dw * dz;
-        om -= (gk0 * dx + gk1 * dy + gk2 * dz) * s3;
-    }
-    if (use_pot) {
-        double ex, ey, ez;
-        nbh_ext_jac(pot, x0, y0, z0, gi0, gi1, gi2, ex, ey, ez);
-        ox += ex;
-        oy += ey;
-        oz += ez;
-    }
-    g[i] += ox;
-    g[N + i] += oy;
-    g[2 * N + i] += oz;
-    g[9 * N + i] += om;
-}
-
 static inline int nbh_blocks(int n, int per) { return n > 0 ? (n + per - 1) / per : 1; }
 
 void nbh_adj_kick_end(nbhip_state* st, const double* a_next)
@@ -389,9 +340,7 @@ void nbh_adj_force(nbhip_state* st)
     hipStream_t s = st->stream;
     const size_t N = (size_t) st->n;
     if (c.criterion == NBHIP_EXACT) {
-        hipLaunchKernelGGL(k_adj_exact, dim3(nbh_blocks(st->n, 256)), dim3(256), 0, s, st->n,
-                           st->px, st->py, st->pz, st->m, st->simid, st->sims, A->g,
-                           c.use_potential, c.pot);
+        nbh_exact_adjoint(st, A->g);
         return;
     }
     const int cap = st->node_cap;

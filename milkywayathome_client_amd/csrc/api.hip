@@ -8,7 +8,9 @@
  * potential + closing half-kick.
  */
 #include "nbhip_internal.h"
+#include "nbhip_device.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -96,7 +98,8 @@ static void nbh_free_all(nbhip_state* st)
                      st->simid_s, st->su,  st->dadj,  st->cincl,   st->scan_part, st->blo,
                      st->bhi,  st->bdelta, st->bparent, st->node,  st->nquad,  st->nmid,
                      st->nlevel, st->nparent, st->nsim_node, st->level_list, st->nodef,
-                     st->scal, st->sims, st->ignore, st->hist_buf, st->pos4 };
+                     st->scal, st->sims, st->ignore, st->hist_buf, st->pos4,
+                     st->exact_part };
     for (void* p : ptrs)
         if (p) (void) hipFree(p);
     if (st->graph_exec) (void) hipGraphExecDestroy(st->graph_exec);
@@ -176,6 +179,7 @@ static unsigned nbh_create(const nbhip_ctx* ctx, int device, int nsim, const int
         if (!(t.eps2 > 0.0)) st->all_eps2_pos = 0;
         st->h_body_off[k] = off;
         st->h_body_n[k] = nbody[k];
+        st->exact_seg = std::max(st->exact_seg, nbh_exact_segments(nbody[k]));
         for (int i = 0; i < nbody[k]; ++i) simid[off + i] = k;
         off += nbody[k];
     }
@@ -194,6 +198,7 @@ static unsigned nbh_create(const nbhip_ctx* ctx, int device, int nsim, const int
     A(node, nc); A(nquad, 6 * nc); A(nmid, 3 * nc); A(nlevel, nc); A(nparent, nc);
     A(nsim_node, nc); A(level_list, nc);
     if (ctx->precision != NBHIP_PRECISION_FP64) A(nodef, nc + 1);   /* + 1: walk prefetch */
+    if (ctx->criterion == NBHIP_EXACT) A(exact_part, 4 * (size_t) st->exact_seg * n);
     A(scal, 1);
     A(sims, (size_t) nsim);
 #undef A
@@ -1074,5 +1079,51 @@ extern "C" unsigned nbhip_histogram(nbhip_state* st, const nbhip_histogram_param
         NBH_CHECK(hipMemcpyAsync(totals, d_totals, (size_t) st->nsim * sizeof(unsigned),
                                  hipMemcpyDeviceToHost, st->stream));
     NBH_CHECK(hipStreamSynchronize(st->stream));
+    return NBHIP_SUCCESS;
+}
+
+/* Reverse orbit with its t_rev tangent (forward mode): each KDK step is
+ * differentiated alongside, d(dt)/d(t_rev) = 1 / nsteps, and the change of
+ * the acceleration is the potential's Jacobian times the position tangent
+ * (nbh_ext_jac, symmetric). */
+extern "C" unsigned nbhip_orbit_tangent(const nbhip_potential* pot, const double* pos,
+                                        const double* vel, double t_rev, int nsteps,
+                                        double* out_pos, double* out_vel, double* dpos,
+                                        double* dvel)
+{
+    if (!pot || !pos || !vel || !out_pos || !out_vel || nsteps < 1 || !std::isfinite(t_rev)) {
+        nbh_set_error("nbhip_orbit_tangent: bad arguments");
+        return NBHIP_USER_ERROR;
+    }
+    const double dt = t_rev / nsteps;
+    const double half = 0.5 * dt;
+    const double ddt = 1.0 / nsteps, dhalf = 0.5 * ddt;
+    double x[3] = { pos[0], pos[1], pos[2] };
+    double v[3] = { -vel[0], -vel[1], -vel[2] };
+    double X[3] = { 0.0, 0.0, 0.0 }, V[3] = { 0.0, 0.0, 0.0 }, A[3] = { 0.0, 0.0, 0.0 };
+    double a[3];
+    nbh_ext_accel(*pot, x[0], x[1], x[2], a[0], a[1], a[2]);
+    for (int k = 0; k < nsteps; ++k) {
+        for (int d = 0; d < 3; ++d) {
+            V[d] = V[d] + (dhalf * a[d] + half * A[d]);
+            v[d] = v[d] + half * a[d];
+        }
+        for (int d = 0; d < 3; ++d) {
+            X[d] = X[d] + (ddt * v[d] + dt * V[d]);
+            x[d] = x[d] + dt * v[d];
+        }
+        nbh_ext_accel(*pot, x[0], x[1], x[2], a[0], a[1], a[2]);
+        nbh_ext_jac(*pot, x[0], x[1], x[2], X[0], X[1], X[2], A[0], A[1], A[2]);
+        for (int d = 0; d < 3; ++d) {
+            V[d] = V[d] + (dhalf * a[d] + half * A[d]);
+            v[d] = v[d] + half * a[d];
+        }
+    }
+    for (int d = 0; d < 3; ++d) {
+        out_pos[d] = x[d];
+        out_vel[d] = -v[d];
+        if (dpos) dpos[d] = X[d];
+        if (dvel) dvel[d] = -V[d];
+    }
     return NBHIP_SUCCESS;
 }

@@ -77,7 +77,7 @@ __device__ __forceinline__ int nbh_levels(int bits, int sb)
 /* pow_rn, milkyway/include/milkyway_math_double.h:59): x^1.5 and t^3 are  */
 /* formed in double-double and rounded once.                               */
 /* ---------------------------------------------------------------------- */
-__device__ __forceinline__ double nbh_pow1p5(double x)
+__host__ __device__ __forceinline__ double nbh_pow1p5(double x)
 {
     double s = sqrt(x);
     double e = fma(-s, s, x);
@@ -87,7 +87,7 @@ __device__ __forceinline__ double nbh_pow1p5(double x)
     return p + (pe + x * ds);
 }
 
-__device__ __forceinline__ double nbh_cube(double t)
+__host__ __device__ __forceinline__ double nbh_cube(double t)
 {
     double h = t * t;
     double l = fma(t, t, -h);
@@ -100,7 +100,7 @@ __device__ __forceinline__ double nbh_cube(double t)
  * supported components: Miyamoto-Nagai disk (:173-194), logarithmic halo
  * (:488-504), NFW halo (:506-515), Hernquist (:157-162) and Plummer
  * (:164-169) spheres. */
-__device__ __forceinline__ void nbh_ext_accel(const nbhip_potential& pot, double x, double y,
+__host__ __device__ __forceinline__ void nbh_ext_accel(const nbhip_potential& pot, double x, double y,
                                               double z, double& oax, double& oay, double& oaz)
 {
     const double limit = 0.00390625;
@@ -181,7 +181,7 @@ __device__ __forceinline__ void nbh_ext_accel(const nbhip_potential& pot, double
  *   logarithmic:     a = k w, w = (x, y, z / q^2), k = -2 v0^2 / D,
  *                    D = d^2 + x^2 + y^2 + (z/q)^2, J = k diag(1, 1, 1/q^2) - 2 (k/D) w w^T
  *   radial f(r) x:   J = f I + f'(r) x x^T / r  (NFW, Hernquist, Plummer) */
-__device__ __forceinline__ void nbh_mn_jac(double M, double da, double db, double x, double y,
+__host__ __device__ __forceinline__ void nbh_mn_jac(double M, double da, double db, double x, double y,
                                            double z, double gx, double gy, double gz,
                                            double& ox, double& oy, double& oz)
 {
@@ -198,7 +198,7 @@ __device__ __forceinline__ void nbh_mn_jac(double M, double da, double db, doubl
     oz += -M * (czz * gz / D15 - 3.0 * uz * ug / D25);
 }
 
-__device__ __forceinline__ void nbh_ext_jac(const nbhip_potential& pot, double x, double y,
+__host__ __device__ __forceinline__ void nbh_ext_jac(const nbhip_potential& pot, double x, double y,
                                             double z, double gx, double gy, double gz,
                                             double& ox, double& oy, double& oz)
 {

@@ -39,6 +39,14 @@ struct __attribute__((aligned(16))) NodeF {
     float q[6];
 };
 
+/* Exact criterion: the sources of a simulation of n bodies are summed in
+ * this many consecutive segments (one per grid row), 1 below 4096 bodies. */
+__host__ __device__ inline int nbh_exact_segments(int n)
+{
+    const int s = n / 2048;
+    return s < 1 ? 1 : (s > 16 ? 16 : s);
+}
+
 /* Per-simulation device record (one per dwarf of a batch; one for a single
  * simulation). */
 struct SimTab {
@@ -129,6 +137,8 @@ struct nbhip_state {
     int *nlevel, *nparent, *nsim_node;
     int *level_list;              /* cell pre-order indices grouped by level */
     NodeF *nodef;                 /* mixed-precision walk records (precision MIXED) */
+    double *exact_part;           /* Exact criterion: per-segment partial sums */
+    int exact_seg;                /* Exact criterion: most segments of any simulation */
 
     DevScalars *scal;
     SimTab *sims;
@@ -172,6 +182,9 @@ void nbh_histogram(nbhip_state* st, const nbhip_histogram_params* hp, unsigned* 
 /* launchers (adjoint.hip) */
 void nbh_adj_kick_end(nbhip_state* st, const double* a_next);
 void nbh_adj_force(nbhip_state* st);
+void nbh_exact_force(nbhip_state* st, int kick, double* oax, double* oay, double* oaz,
+                     int use_pot);
+void nbh_exact_adjoint(nbhip_state* st, double* g);
 void nbh_adj_body(nbhip_state* st, int drift_kick, const double* v_prev, const double* a_prev);
 
 #define NBH_CHECK(x)                                                         \

@@ -343,58 +343,6 @@ __global__ __launch_bounds__(BS) void k_walk32(const NodeF* __restrict__ node,
     nbh_walk_finish(o, p.x, p.y, p.z, ax, ay, az, perm[s], oax, oay, oaz, vx, vy, vz);
 }
 
-/* nbMapForceBody_Exact (nbody/src/nbody_grav.c:229-293): O(N^2) in body
- * order, one body per lane, sources broadcast across the wave. */
-__global__ __launch_bounds__(256) void k_exact_all(int n, const double* __restrict__ px,
-                                                   const double* __restrict__ py,
-                                                   const double* __restrict__ pz,
-                                                   const double* __restrict__ m,
-                                                   const int* __restrict__ simid,
-                                                   const SimTab* __restrict__ sims,
-                                                   int use_pot, nbhip_potential pot, int kick,
-                                                   double dtHalf, double* __restrict__ oax,
-                                                   double* __restrict__ oay,
-                                                   double* __restrict__ oaz,
-                                                   double* __restrict__ vx,
-                                                   double* __restrict__ vy,
-                                                   double* __restrict__ vz)
-{
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    const bool valid = i < n;
-    const double x0 = valid ? px[i] : 0.0, y0 = valid ? py[i] : 0.0, z0 = valid ? pz[i] : 0.0;
-    const SimTab& t = sims[valid ? simid[i] : 0];
-    const double eps2 = t.eps2;
-    double ax = 0.0, ay = 0.0, az = 0.0;
-    for (int k = t.body_off; k < t.body_off + t.body_n; ++k) {
-        const double dx = px[k] - x0;
-        const double dy = py[k] - y0;
-        const double dz = pz[k] - z0;
-        const double drSq = dz * dz + (dy * dy + dx * dx) + eps2;
-        const double drab = sqrt(drSq);
-        const double phii = m[k] / drab;
-        const double mor3 = phii / drSq;
-        ax += mor3 * dx;
-        ay += mor3 * dy;
-        az += mor3 * dz;
-    }
-    if (!valid) return;
-    if (use_pot) {
-        double ex, ey, ez;
-        nbh_ext_accel(pot, x0, y0, z0, ex, ey, ez);
-        ax = ax + ex;
-        ay = ay + ey;
-        az = az + ez;
-    }
-    oax[i] = ax;
-    oay[i] = ay;
-    oaz[i] = az;
-    if (kick) {
-        vx[i] = vx[i] + dtHalf * (ax + 0.0);
-        vy[i] = vy[i] + dtHalf * (ay + 0.0);
-        vz[i] = vz[i] + dtHalf * (az + 0.0);
-    }
-}
-
 /* FP64 direct-sum checker on sampled bodies: one block per sample, the
  * sum over sources split across 256 lanes and reduced in a fixed order. */
 __global__ __launch_bounds__(256) void k_direct_sample(const double* __restrict__ px,
@@ -450,10 +398,7 @@ void nbh_walk(nbhip_state* st, int kick, int count, double* oax, double* oay, do
     const nbhip_ctx& c = st->ctx;
     hipStream_t s = st->stream;
     if (c.criterion == NBHIP_EXACT) {
-        hipLaunchKernelGGL(k_exact_all, dim3(nbh_blocks(st->n, 256)), dim3(256), 0, s, st->n,
-                           st->px, st->py, st->pz, st->m, st->simid, st->sims,
-                           self_only ? 0 : c.use_potential, c.pot, kick, 0.5 * c.timestep, oax,
-                           oay, oaz, st->vx, st->vy, st->vz);
+        nbh_exact_force(st, kick, oax, oay, oaz, self_only ? 0 : c.use_potential);
         return;
     }
     const WalkTune t = tune ? *tune : st->tune;

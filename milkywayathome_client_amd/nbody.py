@@ -94,7 +94,7 @@ EXPORTED = [
     "nbhip_batch_create", "nbhip_batch_upload", "nbhip_batch_download", "nbhip_batch_status",
     "nbhip_batch_tree_info", "nbhip_set_graph", "nbhip_autotune", "nbhip_set_tune",
     "nbhip_adjoint_forward", "nbhip_adjoint_backward", "nbhip_gravity_vjp", "nbhip_set_ignore",
-    "nbhip_histogram",
+    "nbhip_histogram", "nbhip_orbit_tangent",
 ]
 
 _LIB: Optional[C.CDLL] = None
@@ -148,6 +148,8 @@ def load_library() -> C.CDLL:
         "nbhip_adjoint_forward": (u, [st, C.c_uint, C.c_uint]),
         "nbhip_gravity_vjp": (u, [st, _dp, _dp, _dp]),
         "nbhip_adjoint_backward": (u, [st, _dp, _dp, _dp, _dp, _dp, _dp]),
+        "nbhip_orbit_tangent": (u, [C.POINTER(_Potential), _dp, _dp, C.c_double, C.c_int, _dp,
+                                    _dp, _dp, _dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -243,6 +245,22 @@ class Potential:
         p.halo_type, p.halo_vhalo = self.halo.type, self.halo.vhalo
         p.halo_scale_length, p.halo_flatten_z = self.halo.scaleLength, self.halo.flattenZ
         return p
+
+
+def orbit_tangent(potential: Potential, pos, vel, t_rev: float, nsteps: int):
+    """Reverse orbit with a fixed step count (step t_rev / nsteps) and its
+    derivative with respect to t_rev: returns (pos, vel, dpos, dvel), each
+    of 3 (nbhip_orbit_tangent; host code, no device needed)."""
+    lib = load_library()
+    p = potential.to_c()
+    x = np.ascontiguousarray(pos, dtype=np.float64).reshape(3)
+    v = np.ascontiguousarray(vel, dtype=np.float64).reshape(3)
+    out = [np.empty(3) for _ in range(4)]
+    rc = lib.nbhip_orbit_tangent(C.byref(p), _ptr(x), _ptr(v), float(t_rev), int(nsteps),
+                                 *[_ptr(o) for o in out])
+    if rc:
+        raise NBodyError(rc, lib.nbhip_last_error().decode())
+    return tuple(out)
 
 
 def milky_way_potential() -> Potential:

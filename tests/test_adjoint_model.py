@@ -27,6 +27,54 @@ def test_torch_orbit_equals_c_reverse_orbit():
     np.testing.assert_allclose(tv.numpy(), fv, rtol=1e-12)
 
 
+def test_native_orbit_equals_c_reverse_orbit():
+    """nbhip_orbit_tangent's end point (host code in the library, no device)
+    against the C restatement of nbReverseOrbit over the same steps."""
+    pot = nb.milky_way_potential()
+    x0 = models.lbr_to_cartesian([218.0, 53.5, 28.6])
+    v0 = np.array([-156.0, 79.0, 107.0])
+    fx, fv = models.reverse_orbit(pot, x0, v0, 0.2, 1e-4)     # loop runs t = 0 .. -0.2
+    nx, nv, _, _ = nb.orbit_tangent(pot, x0, v0, 2001 * 1e-4, 2001)
+    np.testing.assert_allclose(nx, fx, rtol=1e-12)
+    np.testing.assert_allclose(nv, fv, rtol=1e-12)
+
+
+@pytest.mark.parametrize("halo", ["log", "nfw"])
+def test_orbit_tangent_matches_autograd_and_differences(halo):
+    """d(end point)/d(t_rev) from the forward-mode pass against torch autograd
+    through every step of reverse_orbit_t, and against central differences of
+    the end point itself."""
+    pot = nb.milky_way_potential()
+    if halo == "nfw":
+        pot = nb.Potential.create(nb.Spherical.plummer(3e4, 0.9),
+                                  nb.Disk.miyamotoNagai(2e5, 4.0, 0.5),
+                                  nb.Disk.miyamotoNagai(5e4, 2.0, 0.1), nb.Halo.nfw(120.0, 15.0))
+    x0 = models.lbr_to_cartesian([218.0, 53.5, 28.6])
+    v0 = np.array([-156.0, 79.0, 107.0])
+    t_rev, nsteps = 0.7, 700
+    nx, nv, dx, dv = nb.orbit_tangent(pot, x0, v0, t_rev, nsteps)
+    t = torch.tensor(t_rev, dtype=F64, requires_grad=True)
+    tx, tv = adj.reverse_orbit_t(pot, torch.tensor(x0, dtype=F64), torch.tensor(v0, dtype=F64),
+                                 t, nsteps)
+    np.testing.assert_allclose(tx.detach().numpy(), nx, rtol=1e-12)
+    np.testing.assert_allclose(tv.detach().numpy(), nv, rtol=1e-12)
+    jac = np.array([torch.autograd.grad(c, t, retain_graph=True)[0].item()
+                    for c in list(tx) + list(tv)])
+    np.testing.assert_allclose(np.concatenate([dx, dv]), jac, rtol=1e-9,
+                               atol=1e-9 * np.abs(jac).max())
+    h = 1e-6 * t_rev
+    a = nb.orbit_tangent(pot, x0, v0, t_rev + h, nsteps)
+    b = nb.orbit_tangent(pot, x0, v0, t_rev - h, nsteps)
+    fd = np.concatenate([(a[0] - b[0]) / (2 * h), (a[1] - b[1]) / (2 * h)])
+    np.testing.assert_allclose(np.concatenate([dx, dv]), fd, rtol=1e-6,
+                               atol=1e-6 * np.abs(fd).max())
+
+
+def test_orbit_tangent_rejects_bad_arguments():
+    with pytest.raises(nb.NBodyError):
+        nb.orbit_tangent(nb.milky_way_potential(), np.zeros(3), np.zeros(3), 1.0, 0)
+
+
 def test_torch_ext_accel_equals_c():
     rng = np.random.default_rng(1)
     x = rng.normal(size=(300, 3)) * 25.0
