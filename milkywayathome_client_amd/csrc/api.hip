@@ -204,6 +204,9 @@ static unsigned nbh_create(const nbhip_ctx* ctx, int device, int nsim, const int
 #undef A
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&st->stream, hipStreamNonBlocking);
     for (int k = 0; k < 6 && e == hipSuccess; ++k) e = hipEventCreate(&st->ev[k]);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&st->ev_counts, hipEventDisableTiming);
+    if (e == hipSuccess)
+        e = hipHostMalloc((void**) &st->h_counts_pinned, 64 * sizeof(int), hipHostMallocDefault);
     if (e != hipSuccess) {
         nbh_set_error("allocation for %d bodies failed: %s", st->n, hipGetErrorString(e));
         nbh_free_all(st);
@@ -243,6 +246,8 @@ extern "C" void nbhip_release(nbhip_state* st)
     if (st->stream) (void) hipStreamSynchronize(st->stream);
     for (int k = 0; k < 6; ++k)
         if (st->ev[k]) (void) hipEventDestroy(st->ev[k]);
+    if (st->ev_counts) (void) hipEventDestroy(st->ev_counts);
+    if (st->h_counts_pinned) (void) hipHostFree(st->h_counts_pinned);
     if (st->stream) (void) hipStreamDestroy(st->stream);
     nbh_free_all(st);
     free(st);

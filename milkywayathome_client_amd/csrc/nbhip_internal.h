@@ -155,6 +155,8 @@ struct nbhip_state {
 
     int h_level_count[64];        /* cells per level of the last eager build */
     int h_levels_valid;
+    int *h_counts_pinned;         /* pinned landing buffer of the level-count readback */
+    hipEvent_t ev_counts;         /* recorded after that readback */
 
     NbhAdjoint* adj;
 

@@ -903,27 +903,41 @@ __global__ void k_level_scan(DevScalars* scal)
     scal->level_off[64] = acc;
 }
 
-/* Grid-stride over the nodes actually built (the grid is sized for a
- * typical tree, not the table's capacity). */
+/* Each workgroup takes 4096 consecutive nodes at a time (grid-stride over
+ * the nodes actually built): it counts their cells per level in LDS, claims
+ * one range per non-empty level with a single global atomic, then places
+ * them.  Claims per level stay few (a few hundred workgroups), so the 64
+ * level counters see little contention. */
+#define NBH_LS_CHUNK 4096
 __global__ __launch_bounds__(256) void k_level_scatter(const int* __restrict__ nlevel,
                                                        int* __restrict__ level_list,
                                                        DevScalars* __restrict__ scal)
 {
     __shared__ int lh[64];
     __shared__ int base[64];
+    const int tid = threadIdx.x;
     const int nn = (scal->status & (NBHIP_CELL_OVERFLOW_ERROR | NBHIP_ERROR)) ? 0 : scal->nnode;
-    for (int b0 = blockIdx.x * 256; b0 < nn; b0 += gridDim.x * 256) {
-        if (threadIdx.x < 64) lh[threadIdx.x] = 0;
+    for (int c0 = blockIdx.x * NBH_LS_CHUNK; c0 < nn; c0 += gridDim.x * NBH_LS_CHUNK) {
+        const int cend = min(c0 + NBH_LS_CHUNK, nn);
+        if (tid < 64) lh[tid] = 0;
         __syncthreads();
-        const int i = b0 + (int) threadIdx.x;
-        const int l = i < nn ? nlevel[i] : -1;
-        const bool cell = l >= 0 && l <= 63;
-        const int rank = cell ? atomicAdd(&lh[l], 1) : 0;
+        for (int i = c0 + tid; i < cend; i += 256) {
+            const int l = nlevel[i];
+            if (l >= 0 && l <= 63) atomicAdd(&lh[l], 1);
+        }
         __syncthreads();
-        if (threadIdx.x < 64 && lh[threadIdx.x])
-            base[threadIdx.x] = atomicAdd(&scal->level_fill[threadIdx.x], lh[threadIdx.x]);
+        if (tid < 64) {
+            base[tid] = lh[tid] ? atomicAdd(&scal->level_fill[tid], lh[tid]) : 0;
+            lh[tid] = 0;
+        }
         __syncthreads();
-        if (cell) level_list[scal->level_off[l] + base[l] + rank] = i;
+        for (int i = c0 + tid; i < cend; i += 256) {
+            const int l = nlevel[i];
+            if (l >= 0 && l <= 63) {
+                const int r = atomicAdd(&lh[l], 1);
+                level_list[scal->level_off[l] + base[l] + r] = i;
+            }
+        }
         __syncthreads();
     }
 }
@@ -948,7 +962,7 @@ __global__ __launch_bounds__(256) void k_summarize_level(const int* __restrict__
     if (scal->status & (NBHIP_CELL_OVERFLOW_ERROR | NBHIP_ERROR)) return;
     const int cnt = scal->level_count[level];
     const int nn = scal->nnode;
-    for (int i = blockIdx.x * 256 + threadIdx.x; i < cnt; i += gridDim.x * 256) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
         const int p = level_list[scal->level_off[level] + i];
         const int sim = nsim[p];
         const double rsize = sims[sim].rsize;
@@ -1115,31 +1129,33 @@ void nbh_build(nbhip_state* st, int graph_safe)
                        st->bparent, st->dadj, st->cincl, st->spos, st->simid_s, st->node,
                        st->nmid, st->nlevel, st->nparent, st->nsim_node, st->sims, st->nodef,
                        st->scal);
+    /* the per-level cell counts are final once the cells are emitted: read
+     * them back (eager builds) while the level lists are being built */
+    const bool readback = !graph_safe &&
+        hipMemcpyAsync(st->h_counts_pinned, st->scal->level_count, 64 * sizeof(int),
+                       hipMemcpyDeviceToHost, s) == hipSuccess &&
+        hipEventRecord(st->ev_counts, s) == hipSuccess;
     hipLaunchKernelGGL(k_level_scan, dim3(1), dim3(1), 0, s, st->scal);
-    hipLaunchKernelGGL(k_level_scatter, dim3(nbh_blocks(2 * n + 1024, 256)), dim3(256), 0, s,
-                       st->nlevel, st->level_list, st->scal);
+    hipLaunchKernelGGL(k_level_scatter, dim3(nbh_blocks(2 * n + 1024, NBH_LS_CHUNK)), dim3(256),
+                       0, s, st->nlevel, st->level_list, st->scal);
     nbh_mark(st, 3);
     int counts[64];
     int top = 63;
-    bool have = false;
-    if (!graph_safe) {
-        /* per-level launches sized from the device counts (one small readback) */
-        have = hipMemcpyAsync(counts, st->scal->level_count, sizeof(counts),
-                              hipMemcpyDeviceToHost, s) == hipSuccess &&
-               hipStreamSynchronize(s) == hipSuccess;
-        if (have) {
-            top = 0;
-            for (int l = 0; l < 64; ++l)
-                if (counts[l] > 0) top = l;
-        }
+    const bool have = readback && hipEventSynchronize(st->ev_counts) == hipSuccess;
+    if (have) {
+        memcpy(counts, st->h_counts_pinned, sizeof(counts));
+        top = 0;
+        for (int l = 0; l < 64; ++l)
+            if (counts[l] > 0) top = l;
     }
     st->h_levels_valid = have ? 1 : 0;
     if (have) memcpy(st->h_level_count, counts, sizeof(counts));
-    const int gmax = nbh_blocks(n, 256 * 8) + 1;
+    /* 64-lane workgroups: a level's cells spread over more CUs */
+    const int gmax = nbh_blocks(n, 64 * 8) + 1;
     for (int l = top; l >= 0; --l) {
         if (have && counts[l] == 0) continue;
-        const int g = have ? nbh_blocks(counts[l], 256) : gmax;
-        hipLaunchKernelGGL(k_summarize_level, dim3(g), dim3(256), 0, s, st->level_list, l,
+        const int g = have ? nbh_blocks(counts[l], 64) : gmax;
+        hipLaunchKernelGGL(k_summarize_level, dim3(g), dim3(64), 0, s, st->level_list, l,
                            st->node, st->nquad, st->nmid, st->nsim_node, st->sims, st->nodef,
                            st->ctx, st->scal);
     }
