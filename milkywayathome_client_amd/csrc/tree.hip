@@ -209,11 +209,16 @@ __global__ __launch_bounds__(256) void k_radix_hist(int n, int shift,
     cnt[threadIdx.x] = 0;
     __syncthreads();
     const int base = blockIdx.x * NBH_SORT_TILE;
-    #pragma unroll 4
+    /* all of the thread's keys in flight at once, then the counts */
+    unsigned dg[NBH_SORT_TILE / 256];
+    #pragma unroll
     for (int r = 0; r < NBH_SORT_TILE / 256; ++r) {
-        int idx = base + r * 256 + threadIdx.x;
-        if (idx < n) atomicAdd(&cnt[(unsigned) (key[idx] >> shift) & 255u], 1u);
+        const int idx = base + r * 256 + threadIdx.x;
+        dg[r] = idx < n ? (unsigned) (key[idx] >> shift) & 255u : 256u;
     }
+    #pragma unroll
+    for (int r = 0; r < NBH_SORT_TILE / 256; ++r)
+        if (dg[r] < 256u) atomicAdd(&cnt[dg[r]], 1u);
     __syncthreads();
     hist[threadIdx.x * nb + blockIdx.x] = cnt[threadIdx.x];
 }
@@ -284,11 +289,22 @@ __global__ __launch_bounds__(256) void k_radix_scatter(int n, int shift,
     __syncthreads();
     const unsigned long long lt = nbh_lanemask_lt();
     const int base = blockIdx.x * NBH_SORT_TILE;
+    /* all of the thread's keys and values in flight at once (one memory
+     * latency per tile instead of one per row), then the rows in order */
+    unsigned long long kr[NBH_SORT_TILE / 256];
+    unsigned vr[NBH_SORT_TILE / 256];
+    #pragma unroll
+    for (int r = 0; r < NBH_SORT_TILE / 256; ++r) {
+        const int idx = base + r * 256 + t;
+        kr[r] = idx < n ? kin[idx] : 0ull;
+        vr[r] = idx < n ? vin[idx] : 0u;
+    }
+    #pragma unroll
     for (int r = 0; r < NBH_SORT_TILE / 256; ++r) {
         const int idx = base + r * 256 + t;
         const bool valid = idx < n;
-        unsigned long long k = valid ? kin[idx] : 0ull;
-        unsigned v = valid ? vin[idx] : 0u;
+        const unsigned long long k = kr[r];
+        const unsigned v = vr[r];
         unsigned d = (unsigned) (k >> shift) & 255u;
         unsigned long long peers = __ballot(valid);
         #pragma unroll
