@@ -315,7 +315,7 @@ def run_precision(args):
     compared body by body with the FP64 tree's; the final lambda histograms
     of the baryons (dark matter ignored) are compared with the FP64 run's by
     EMD."""
-    from milkywayathome_client_amd import histogram as H, nbody as nb
+    from milkywayathome_client_amd import histogram as H, likelihood as Lk, nbody as nb
     log(f"[bench] precision workload: building N={args.n}")
     pot, pos, vel, m, eps2, dt = make_workload(args.n, args.seed)
     n = args.n
@@ -323,7 +323,7 @@ def run_precision(args):
     ignore = np.arange(n) >= nbar
     hp = H.HistogramParams()
     checkpoints = [0, 100, 1000]
-    out, hists, acc0 = {}, {}, {}
+    out, hists, acc0, raw = {}, {}, {}, {}
     for prec in ("fp64", "mixed", "mixed_coherent"):
         ctx = nb.NBodyCtx.create(timestep=dt, timeEvolve=dt * checkpoints[-1], eps2=eps2,
                                  theta=args.theta, criterion="BH86", useQuad=True,
@@ -349,6 +349,8 @@ def run_precision(args):
             log(f"[bench] {prec} step {c}: {errs[f'step_{c}']}")
         counts, totals = st.histogram(hp)
         hists[prec] = counts[0].astype(np.float64) / max(int(totals[0]), 1)
+        raw[prec] = Lk.from_counts(counts[0], hp, total_simulated=nbar,
+                                   mass_per_particle=float(m[0]))
         out[prec] = {"errors_vs_fp64_direct": errs, "walk_tune": tune,
                      "ms_per_step": 1e3 * t_run / checkpoints[-1],
                      "status": st.last_status}
@@ -360,6 +362,11 @@ def run_precision(args):
                                            "p99": float(np.percentile(rel, 99)),
                                            "max": float(rel.max())}
         out[prec]["final_histogram_emd_vs_fp64"] = H.emd(hists[prec], hists["fp64"], hp)
+        # nbSystemLikelihood of this run's histogram against the FP64 run's
+        lk = Lk.system_likelihood(raw["fp64"], raw[prec], n)
+        out[prec]["final_likelihood_vs_fp64"] = {"likelihood": float(lk[0]),
+                                                 "emd_term": float(lk[1]),
+                                                 "cost_term": float(lk[2])}
     line = {
         "metric": "per-body acceleration error vs FP64 direct sum at steps 0/100/1000, "
                   "final-histogram EMD vs the FP64 run",

@@ -65,3 +65,30 @@ def test_batch_histograms_per_simulation():
         assert totals[k] == tot, k
         np.testing.assert_array_equal(counts[k], np.round(ref * tot).astype(np.int64))
         off += len(m)
+
+
+def test_likelihood_from_gpu_counts():
+    """nbSystemLikelihood on the GPU histogram (likelihood.from_counts) equals
+    the one on the host binning of the same bodies; a run against its own
+    histogram scores 0, and against another run a positive geometry term."""
+    from milkywayathome_client_amd import likelihood as L
+    hp = H.HistogramParams()
+    pos, vel, m = _stream(20000, 3, (0.0, 0.0, 0.0))
+    pos2, _, _ = _stream(20000, 4, (2.0, 0.0, 0.0))
+    assert _edges_clear(pos, hp) and _edges_clear(pos2, hp)
+    ctx = nb.NBodyCtx.create(timestep=1e-3, timeEvolve=1.0, eps2=1e-3, theta=0.5)
+    st = nb.NBodyState(ctx, None, pos, vel, m, prestep=False)
+    counts, totals = st.histogram(hp)
+    sim = L.from_counts(counts[0], hp, total_simulated=len(m), mass_per_particle=1e-3)
+    ref, tot = H.create_histogram(pos, hp)
+    host = L.from_counts(np.round(ref * tot).astype(np.int64), hp, total_simulated=len(m),
+                         mass_per_particle=1e-3)
+    np.testing.assert_array_equal(sim.variable, host.variable)
+    assert np.all(L.system_likelihood(host, sim, len(m))[:3] == 0.0)
+    st2 = nb.NBodyState(ctx, None, pos2, vel, m, prestep=False)
+    other = L.from_counts(st2.histogram(hp)[0][0], hp, total_simulated=len(m),
+                          mass_per_particle=1e-3)
+    out = L.system_likelihood(other, sim, len(m))
+    assert out[1] > 0.0 and np.isfinite(out[0])
+    st.close()
+    st2.close()
