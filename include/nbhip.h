@@ -135,6 +135,17 @@ unsigned nbhip_steps(nbhip_state* st, unsigned nsteps);
 
 unsigned nbhip_tree_info_get(nbhip_state* st, nbhip_tree_info* info);
 
+/* Replaces nbReadCheckpointHeader's state restore (nbody/src/nbody_checkpoint.c:
+ * 124-130: st->step, st->tree.rsize, st->treeIncest) for a GPU state resumed
+ * from a checkpoint: after nbhip_upload (or nbhip_batch_upload) of the
+ * checkpointed bodies, sets the step counter, the persistent root size of
+ * simulation `sim` (0 for a single state; a power of two, as every size the
+ * doubling rule produces from a power-of-two treeRSize) and ORs `status`
+ * (sticky NBODY_* warning bits such as the incest flag) into its status.  The
+ * next nbhip_run evaluates the force at the restored positions with the
+ * restored root, so the resumed run continues bit for bit. */
+unsigned nbhip_restore(nbhip_state* st, int sim, unsigned step, double rsize, unsigned status);
+
 /* Pre-order node table of the last tree (cells and massive bodies in
  * More/Next threading order, nbody/src/nbody_tree.c:127-150).  Arrays hold
  * info.nnode entries; kind 0 cell / 1 body; ref = body index (original order)

@@ -552,6 +552,29 @@ extern "C" unsigned nbhip_tree_info_get(nbhip_state* st, nbhip_tree_info* info)
     return nbh_info(st, 0, info);
 }
 
+extern "C" unsigned nbhip_restore(nbhip_state* st, int sim, unsigned step, double rsize,
+                                  unsigned status)
+{
+    if (!st || sim < 0 || sim >= st->nsim) return NBHIP_USER_ERROR;
+    int ex = 0;
+    if (!(rsize > 0.0) || !std::isfinite(rsize) || std::frexp(rsize, &ex) != 0.5) {
+        nbh_set_error("nbhip_restore: root size %.17g is not a positive power of two", rsize);
+        return NBHIP_USER_ERROR;
+    }
+    NBH_CHECK(hipSetDevice(st->device));
+    SimTab t;
+    NBH_CHECK(hipMemcpyAsync(&t, st->sims + sim, sizeof(t), hipMemcpyDeviceToHost, st->stream));
+    NBH_CHECK(hipStreamSynchronize(st->stream));
+    t.rsize = rsize;
+    t.key_scale = 9223372036854775808.0 / rsize;
+    NBH_CHECK(hipMemcpyAsync(st->sims + sim, &t, sizeof(t), hipMemcpyHostToDevice, st->stream));
+    NBH_CHECK(hipStreamSynchronize(st->stream));
+    st->step = step;
+    st->h_sim_status[sim] |= status;
+    st->have_tree = 0;        /* a captured graph step reads the root from the table */
+    return NBHIP_SUCCESS;
+}
+
 extern "C" unsigned nbhip_batch_tree_info(nbhip_state* st, int sim, nbhip_tree_info* info)
 {
     if (!st || !info || sim < 0 || sim >= st->nsim) return NBHIP_USER_ERROR;

@@ -94,7 +94,7 @@ EXPORTED = [
     "nbhip_batch_create", "nbhip_batch_upload", "nbhip_batch_download", "nbhip_batch_status",
     "nbhip_batch_tree_info", "nbhip_set_graph", "nbhip_autotune", "nbhip_set_tune",
     "nbhip_adjoint_forward", "nbhip_adjoint_backward", "nbhip_gravity_vjp", "nbhip_set_ignore",
-    "nbhip_histogram", "nbhip_orbit_tangent",
+    "nbhip_histogram", "nbhip_orbit_tangent", "nbhip_restore",
 ]
 
 _LIB: Optional[C.CDLL] = None
@@ -150,6 +150,7 @@ def load_library() -> C.CDLL:
         "nbhip_adjoint_backward": (u, [st, _dp, _dp, _dp, _dp, _dp, _dp]),
         "nbhip_orbit_tangent": (u, [C.POINTER(_Potential), _dp, _dp, C.c_double, C.c_int, _dp,
                                     _dp, _dp, _dp]),
+        "nbhip_restore": (u, [st, C.c_int, C.c_uint, C.c_double, C.c_uint]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -398,6 +399,12 @@ class NBodyState:
     def accelerations(self) -> np.ndarray:
         return self.bodies()[2]
 
+    def restore(self, step: int, rsize: float, status: int = 0) -> None:
+        """Resume from a checkpoint (nbReadCheckpointHeader): after uploading
+        the checkpointed bodies, set the step counter, the persistent root
+        size and sticky status bits; the next run() continues bit for bit."""
+        self._check(self.lib.nbhip_restore(self._h, 0, int(step), float(rsize), int(status)))
+
     def tree_info(self) -> TreeInfo:
         ti = _TreeInfo()
         self._check(self.lib.nbhip_tree_info_get(self._h, C.byref(ti)))
@@ -568,6 +575,10 @@ class NBodyBatch(NBodyState):
         pos, vel, acc = np.empty((n, 3)), np.empty((n, 3)), np.empty((n, 3))
         self._check(self.lib.nbhip_batch_download(self._h, k, _ptr(pos), _ptr(vel), _ptr(acc)))
         return pos, vel, acc
+
+    def restore_sim(self, k: int, step: int, rsize: float, status: int = 0) -> None:
+        """restore() for simulation k (the step counter is shared)."""
+        self._check(self.lib.nbhip_restore(self._h, int(k), int(step), float(rsize), int(status)))
 
     def statuses(self):
         out = (C.c_uint * self.nsim)()

@@ -1,0 +1,95 @@
+"""Checkpoint / resume of a GPU state (nbhip_restore), as the reference's
+checkpoint does it (nbody/src/nbody_checkpoint.c:108-130 keeps the bodies,
+st->step and st->tree.rsize; nbRunSystemPlain, nbody/src/nbody_plain.c:
+366-391, recomputes the force on resume): a run interrupted after k steps,
+its bodies and header marshalled out and restored into a fresh state, must
+end bit for bit where the uninterrupted run ends.  The root size of the run
+has grown past treeRSize by then, so a fresh state without the restore would
+build different trees; a restored 4x root is shown to be the one in use
+(two more levels of chain cells above the same aligned cells)."""
+import numpy as np
+import pytest
+
+from helpers import plummer_bodies
+from milkywayathome_client_amd import nbody as nb
+
+pytestmark = pytest.mark.gpu
+
+
+def _ctx(dt=1e-3):
+    # a small initial root: the bounding-box rule doubles it during the run
+    return nb.NBodyCtx.create(timestep=dt, timeEvolve=dt * 100, eps2=1e-4, theta=0.5,
+                              criterion="BH86", useQuad=True, treeRSize=0.125)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_resume_continues_bit_for_bit(precision):
+    pos, vel, m = plummer_bodies(4000, seed=9)
+    k = 20
+    a = nb.NBodyState(_ctx(), None, pos, vel, m, precision=precision, prestep=False)
+    a.run(2 * k)
+    pa, va, aa = a.bodies()
+    b = nb.NBodyState(_ctx(), None, pos, vel, m, precision=precision, prestep=False)
+    b.run(k)
+    pb, vb, _ = b.bodies()
+    info = b.tree_info()
+    assert info.step == k and info.rsize > 0.125
+    c = nb.NBodyState(_ctx(), None, pb, vb, m, precision=precision, prestep=False)
+    c.restore(info.step, info.rsize, info.status)
+    c.run(k)
+    pc, vc, ac = c.bodies()
+    assert c.tree_info().step == 2 * k
+    assert c.tree_info().rsize == a.tree_info().rsize
+    np.testing.assert_array_equal(pc, pa)
+    np.testing.assert_array_equal(vc, va)
+    np.testing.assert_array_equal(ac, aa)
+    # the restored root is the one in use: a 4x root puts two more levels of
+    # chain cells above the same (aligned, power-of-two) cells
+    d = nb.NBodyState(_ctx(), None, pb, vb, m, precision=precision, prestep=False)
+    d.restore(info.step, 4.0 * info.rsize, info.status)
+    d.run(k)
+    ti_a, ti_d = a.tree_info(), d.tree_info()
+    assert ti_d.rsize == 4.0 * ti_a.rsize
+    assert ti_d.nnode > ti_a.nnode and ti_d.max_depth == ti_a.max_depth + 2
+    for s in (a, b, c, d):
+        s.close()
+
+
+def test_restore_rejects_bad_root_and_simulation():
+    pos, vel, m = plummer_bodies(100, seed=1)
+    st = nb.NBodyState(_ctx(), None, pos, vel, m, prestep=False)
+    with pytest.raises(nb.NBodyError):
+        st.restore(5, 3.0)                 # not a power of two
+    with pytest.raises(nb.NBodyError):
+        st.restore(5, -1.0)
+    with pytest.raises(nb.NBodyError):
+        st._check(st.lib.nbhip_restore(st._h, 1, 5, 1.0, 0))   # no simulation 1
+    st.restore(5, 2.0)
+    assert st.tree_info().step == 5 and st.tree_info().rsize == 2.0
+    st.close()
+
+
+def test_batch_resume_per_simulation():
+    members = [plummer_bodies(n, seed=20 + i, radius=0.1 * (i + 1)) for i, n in
+               enumerate([1500, 700, 2600])]
+    k = 10
+    ctx = _ctx()
+    a = nb.NBodyBatch(ctx, None, members, prestep=False)
+    a.run(2 * k)
+    b = nb.NBodyBatch(ctx, None, members, prestep=False)
+    b.run(k)
+    state = [b.sim_bodies(i) for i in range(3)]
+    infos = [b.sim_tree_info(i) for i in range(3)]
+    assert len({t.rsize for t in infos}) > 1           # roots differ per simulation
+    c = nb.NBodyBatch(ctx, None, [(p, v, mm[2]) for (p, v, _), mm in zip(state, members)],
+                      prestep=False)
+    for i, t in enumerate(infos):
+        c.restore_sim(i, t.step, t.rsize, t.status)
+    c.run(k)
+    for i in range(3):
+        pa, va, _ = a.sim_bodies(i)
+        pc, vc, _ = c.sim_bodies(i)
+        np.testing.assert_array_equal(pc, pa)
+        np.testing.assert_array_equal(vc, va)
+    for s in (a, b, c):
+        s.close()
