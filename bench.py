@@ -464,9 +464,12 @@ def main():
             log(f"[bench] {prec} failed: {e}")
             continue
         errs = accel_errors(st, pot, args.n)
+        ti = st.tree_info()
         results[prec] = {"wall": wall, "phases_ms": phases, "counts": counts, "errors": errs,
                          "body_steps_per_s": args.n * args.steps / wall, "tune": tune,
-                         "graph_ms": graph_ms}
+                         "graph_ms": graph_ms,
+                         "tree": {"ncell": ti.ncell, "nnode": ti.nnode,
+                                  "max_depth": ti.max_depth, "rsize": ti.rsize}}
         log(f"[bench] {prec}: {results[prec]['body_steps_per_s']:.4g} body-steps/s, "
             f"ms/step {1e3 * wall / args.steps:.3f}, phases {np.round(phases, 3)}, errs {errs}")
         st.close()
@@ -511,6 +514,7 @@ def main():
             ["drift_bbox", "keys_sort", "build", "summarize", "walk"], r["phases_ms"])},
         "accel_error_vs_fp64_direct": r["errors"],
     }
+    line["tree"] = r["tree"]
     line["walk_tune"] = r["tune"]
     line["graph_ms_per_step"] = r["graph_ms"]
     if "fp64" in results:
