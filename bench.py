@@ -350,7 +350,8 @@ def run_precision(args):
         counts, totals = st.histogram(hp)
         hists[prec] = counts[0].astype(np.float64) / max(int(totals[0]), 1)
         raw[prec] = Lk.from_counts(counts[0], hp, total_simulated=nbar,
-                                   mass_per_particle=float(m[0]))
+                                   mass_per_particle=float(m[0]),
+                                   dispersions=st.histogram_dispersion(hp))
         out[prec] = {"errors_vs_fp64_direct": errs, "walk_tune": tune,
                      "ms_per_step": 1e3 * t_run / checkpoints[-1],
                      "status": st.last_status}
@@ -362,11 +363,13 @@ def run_precision(args):
                                            "p99": float(np.percentile(rel, 99)),
                                            "max": float(rel.max())}
         out[prec]["final_histogram_emd_vs_fp64"] = H.emd(hists[prec], hists["fp64"], hp)
-        # nbSystemLikelihood of this run's histogram against the FP64 run's
-        lk = Lk.system_likelihood(raw["fp64"], raw[prec], n)
+        # nbSystemLikelihood of this run's histogram against the FP64 run's,
+        # with the beta-dispersion term as the sample workunit enables it
+        lk = Lk.system_likelihood(raw["fp64"], raw[prec], n, use_beta_disp=True)
         out[prec]["final_likelihood_vs_fp64"] = {"likelihood": float(lk[0]),
                                                  "emd_term": float(lk[1]),
-                                                 "cost_term": float(lk[2])}
+                                                 "cost_term": float(lk[2]),
+                                                 "beta_dispersion_term": float(lk[3])}
     line = {
         "metric": "per-body acceleration error vs FP64 direct sum at steps 0/100/1000, "
                   "final-histogram EMD vs the FP64 run",

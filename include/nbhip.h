@@ -240,6 +240,26 @@ unsigned nbhip_set_ignore(nbhip_state* st, const unsigned char* ignore);
 unsigned nbhip_histogram(nbhip_state* st, const nbhip_histogram_params* hp, unsigned* counts,
                          unsigned* totals);
 
+/* The outlier-rejection settings of the dispersion histograms (NBodyCtx's
+ * IterMax, BetaSigma, VelSigma, BetaCorrect, VelCorrect;
+ * nbody/src/nbody_lua_types/nbody_lua_nbodyctx.c:127-133). */
+typedef struct nbhip_disp_params {
+    int iter_max;
+    double beta_sigma, vel_sigma;
+    double beta_correct, vel_correct;
+} nbhip_disp_params;
+
+/* Histograms 1 and 2 of nbCreateHistogram (nbody/src/nbody_histogram.c:
+ * 858-1000) for simulation `sim` on the current state: per bin the beta
+ * dispersion and the line-of-sight velocity dispersion (calc_vLOS,
+ * nbody/src/nbody_mass.c:171-179) with their errors, after nbCalcDisp and
+ * iter_max rounds of nbRemoveOutliers (nbody/src/nbody_mass.c:195-316).
+ * Ignored bodies are left out as in nbhip_histogram.  Each output holds
+ * lambda_bins * beta_bins entries (lambda-major); any may be NULL. */
+unsigned nbhip_histogram_dispersion(nbhip_state* st, int sim, const nbhip_histogram_params* hp,
+                                    const nbhip_disp_params* dp, double* beta_disp,
+                                    double* beta_err, double* vlos_disp, double* vlos_err);
+
 /* ---- adjoint leapfrog ------------------------------------------------------
  * No counterpart in the reference.  Gradients of a loss on the final state
  * with respect to the initial positions, velocities, the masses and the

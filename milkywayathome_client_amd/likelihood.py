@@ -12,14 +12,17 @@ the output layer that consumes the GPU histogram (nbhip_histogram).
   nbody/src/nbody_emd.c:1201-1268): exact 1-D transport when every point
   shares one beta, a linear program otherwise.
 * `match_emd` (nbMatchEMD, nbody/src/nbody_emd.c:1277-1428), `cost_component`
-  (nbCostComponent, nbody/src/nbody_mass.c:318-392) and `system_likelihood`
-  (nbSystemLikelihood, nbody/src/nbody_likelihood.c:155-323): the geometry
-  and mass terms, with the reference's NaN / infinity conventions.
+  (nbCostComponent, nbody/src/nbody_mass.c:318-392), `nb_likelihood`
+  (nbLikelihood, :395-470, the comparison of a per-bin quantity such as a
+  dispersion, averaged over betaDispBins bins) and `system_likelihood`
+  (nbSystemLikelihood, nbody/src/nbody_likelihood.c:155-323): the geometry,
+  mass and dispersion terms, with the reference's NaN / infinity conventions.
 
-The dispersion and average components (beta dispersion, velocity dispersion,
-line-of-sight velocity, beta, distance, proper motions) need per-bin body
-statistics with iterative outlier rejection that the GPU histogram does not
-compute; asking for them raises NotImplementedError.
+The beta and line-of-sight velocity dispersions of a simulation come from
+the GPU (nbhip_histogram_dispersion, with the reference's outlier
+rejection).  The average line-of-sight velocity, beta, distance and proper
+motion components are not computed; asking for them raises
+NotImplementedError.
 """
 from __future__ import annotations
 
@@ -59,6 +62,10 @@ class Histogram:
     emd_range: List[float] = field(default_factory=list)
     beta_disp_bins: int = 0
     extra: Optional[np.ndarray] = None  # [nbin, k] remaining columns, as read
+    # histograms 1.. of the MainStruct: k -> (variable, err) per bin
+    # (1 beta dispersion, 2 line-of-sight velocity dispersion, 3 LOS velocity,
+    # 4 beta average, 5 distance, 6 / 7 proper motions)
+    comps: dict = field(default_factory=dict)
 
     @property
     def nbin(self) -> int:
@@ -108,6 +115,8 @@ def read_histogram(path: str) -> Histogram:
                   use_bin=a[:, 0].astype(np.int64), lam=a[:, 1].copy(), beta=a[:, 2].copy(),
                   variable=a[:, 3].copy(), err=a[:, 4].copy(),
                   beta_disp_bins=int(hdr.get("betaDispBins", "0")), extra=a[:, 5:].copy())
+    for k in range(1, (ncol - 5) // 2 + 1):
+        h.comps[k] = (a[:, 3 + 2 * k].copy(), a[:, 4 + 2 * k].copy())
     if "EMDRange" in hdr:
         h.emd_range = _emd_range_tokens(hdr["EMDRange"])
     else:
@@ -120,7 +129,7 @@ def read_histogram(path: str) -> Histogram:
 
 def write_histogram(path: str, h: Histogram, comment: str = "") -> None:
     """nbPrintHistogram's 9-column layout: the counts and their errors, then
-    the beta and velocity dispersion columns (-1: not computed)."""
+    the beta and velocity dispersion columns (-1 where not computed)."""
     with open(path, "w") as f:
         for c in comment.splitlines():
             f.write(f"# {c}\n")
@@ -130,9 +139,17 @@ def write_histogram(path: str, h: Histogram, comment: str = "") -> None:
         f.write(f"totalSimulated = {h.total_simulated}\n")
         f.write(f"lambdaBins = {h.lambda_bins}\n")
         f.write(f"betaBins = {h.beta_bins}\n")
+        if h.beta_disp_bins:
+            f.write(f"betaDispBins = {h.beta_disp_bins}\n")
         for i in range(h.nbin):
+            cols = []
+            for k in (1, 2):
+                if k in h.comps:
+                    cols += [f"{h.comps[k][0][i]:12.15f}", f"{h.comps[k][1][i]:12.15f}"]
+                else:
+                    cols += ["-1", "-1"]
             f.write(f"{int(h.use_bin[i])} {h.lam[i]:12.15f} {h.beta[i]:12.15f} "
-                    f"{h.variable[i]:12.15f} {h.err[i]:12.15f} " + " ".join(["-1"] * 4) + "\n")
+                    f"{h.variable[i]:12.15f} {h.err[i]:12.15f} " + " ".join(cols) + "\n")
         f.write("</histogram>\n")
 
 
@@ -145,12 +162,14 @@ def normalized_error(n: int, total_histogram: float, total_simulated: float) -> 
 
 
 def from_counts(counts, hp: HistogramParams, total_simulated: int,
-                mass_per_particle: float) -> Histogram:
+                mass_per_particle: float, dispersions=None) -> Histogram:
     """A simulated histogram from raw counts [lambdaBins * betaBins]
     (lambda-major, as nbhip_histogram returns them): bin centres, normalised
     counts and errors as nbNormalizeHistogram computes them.  Like the
     reference, the error of bin (i, j) is computed from the raw count of bin
-    i (identical for one beta bin)."""
+    i (identical for one beta bin).  `dispersions` = (beta_disp, beta_err,
+    vlos_disp, vlos_err) per bin, as NBodyState.histogram_dispersion returns
+    them, fills histograms 1 and 2."""
     c = np.asarray(counts, dtype=np.int64).reshape(-1)
     lb, bb = hp.lambdaBins, hp.betaBins
     if c.size != lb * bb:
@@ -169,11 +188,16 @@ def from_counts(counts, hp: HistogramParams, total_simulated: int,
             beta[k] = (j + 0.5) * bs + hp.betaStart
             var[k] = c[k] / total if total else math.nan
             err[k] = normalized_error(int(c[i]), total, total_simulated) if total else math.nan
-    return Histogram(lambda_bins=lb, beta_bins=bb, total_num=total,
-                     total_simulated=int(total_simulated),
-                     mass_per_particle=float(mass_per_particle),
-                     use_bin=np.ones(c.size, dtype=np.int64), lam=lam, beta=beta, variable=var,
-                     err=err, beta_disp_bins=1)
+    h = Histogram(lambda_bins=lb, beta_bins=bb, total_num=total,
+                  total_simulated=int(total_simulated),
+                  mass_per_particle=float(mass_per_particle),
+                  use_bin=np.ones(c.size, dtype=np.int64), lam=lam, beta=beta, variable=var,
+                  err=err, beta_disp_bins=1)
+    if dispersions is not None:
+        bd, be, vd, ve = (np.asarray(x, dtype=np.float64).reshape(-1) for x in dispersions)
+        h.comps[1] = (bd, be)
+        h.comps[2] = (vd, ve)
+    return h
 
 
 def emd_calc(w1, lam1, beta1, w2, lam2, beta2) -> float:
@@ -287,6 +311,55 @@ def cost_component(data: Histogram, hist: Histogram) -> float:
     return -(num / denom)
 
 
+def nb_likelihood(data_var, data_err, use_bin, hist_var, hist_err, avg_bins: int) -> float:
+    """nbLikelihood: half the sum over used data bins of
+    (data - sim)^2 / (err_data^2 + err_sim^2); 25 (five sigma) where the
+    simulation has no error; bins whose data error is not positive are
+    skipped.  With avg_bins > 1 the simulated value is the average over the
+    avg_bins bins centred on the data bin (an even count is reduced by one),
+    leaving out bins outside the histogram or without an error."""
+    data_var = np.asarray(data_var, dtype=np.float64)
+    data_err = np.asarray(data_err, dtype=np.float64)
+    hist_var = np.asarray(hist_var, dtype=np.float64)
+    hist_err = np.asarray(hist_err, dtype=np.float64)
+    nbins = data_var.size
+    nsig = 0.0
+    for i in range(nbins):
+        if not use_bin[i]:
+            continue
+        if avg_bins > 1:
+            if avg_bins % 2 == 0:
+                avg_bins -= 1
+            n = avg_bins
+            var_sum = 0.0
+            val_sum = 0.0
+            for k in range(avg_bins):
+                idx = i - (avg_bins - 1) // 2 + k
+                if idx < 0 or idx >= nbins:
+                    n -= 1
+                elif hist_err[idx] <= 0:
+                    n -= 1
+                else:
+                    var_sum += hist_err[idx] ** 2
+                    val_sum += hist_var[idx]
+            with np.errstate(divide="ignore", invalid="ignore"):
+                err_hist = float(np.sqrt(var_sum) / n) if n else math.nan
+                hist_v = val_sum / n if n else math.nan
+            err_data = data_err[i]
+            data_v = data_var[i]
+        else:
+            err_data = data_err[i]
+            err_hist = hist_err[i]
+            data_v = data_var[i]
+            hist_v = hist_var[i]
+        if err_data > 0:
+            if err_hist > 0:
+                nsig += (data_v - hist_v) ** 2 / (err_data ** 2 + err_hist ** 2)
+            else:
+                nsig += 25.0
+    return nsig / 2.0
+
+
 def worst_case_emd() -> float:
     return DEFAULT_WORST_CASE
 
@@ -302,9 +375,13 @@ def system_likelihood(data: Histogram, hist: Histogram, nbody: int, method: str 
     out = np.full(10, math.nan)
     if data.lambda_bins != hist.lambda_bins:
         return out
-    if any((use_beta_disp, use_vel_disp, use_vlos, use_beta_comp, use_dist, use_prop_mot)):
-        raise NotImplementedError("dispersion and average components need per-bin body "
-                                  "statistics the GPU histogram does not compute")
+    if any((use_vlos, use_beta_comp, use_dist, use_prop_mot)):
+        raise NotImplementedError("the average line-of-sight velocity, beta, distance and "
+                                  "proper motion components are not computed")
+    for flag, k in ((use_beta_disp, 1), (use_vel_disp, 2)):
+        if flag and (k not in data.comps or k not in hist.comps):
+            raise ValueError(f"histogram {k} (dispersion) missing from the data or the "
+                             "simulation (from_counts(..., dispersions=...))")
     if method != "emd":
         raise NotImplementedError("only the EMD geometry term is implemented")
     if hist.total_num < 0.0001 * float(nbody):
@@ -313,7 +390,15 @@ def system_likelihood(data: Histogram, hist: Histogram, nbody: int, method: str 
         return out
     geometry = match_emd(data, hist)
     cost = cost_component(data, hist)
-    out[0] = geometry + cost
+    like = geometry + cost
+    if use_beta_disp:
+        out[3] = nb_likelihood(*data.comps[1], data.use_bin, *hist.comps[1],
+                               data.beta_disp_bins)
+        like += out[3]
+    if use_vel_disp:
+        out[4] = nb_likelihood(*data.comps[2], data.use_bin, *hist.comps[2], 1)
+        like += out[4]
+    out[0] = like
     out[1] = geometry
     out[2] = cost
     return out

@@ -75,6 +75,11 @@ class _TreeInfo(C.Structure):
     ]
 
 
+class _DispParams(C.Structure):
+    _fields_ = [("iter_max", C.c_int), ("beta_sigma", C.c_double), ("vel_sigma", C.c_double),
+                ("beta_correct", C.c_double), ("vel_correct", C.c_double)]
+
+
 class _HistParams(C.Structure):
     _fields_ = [("phi", C.c_double), ("theta", C.c_double), ("psi", C.c_double),
                 ("lambda_start", C.c_double), ("lambda_end", C.c_double),
@@ -94,7 +99,7 @@ EXPORTED = [
     "nbhip_batch_create", "nbhip_batch_upload", "nbhip_batch_download", "nbhip_batch_status",
     "nbhip_batch_tree_info", "nbhip_set_graph", "nbhip_autotune", "nbhip_set_tune",
     "nbhip_adjoint_forward", "nbhip_adjoint_backward", "nbhip_gravity_vjp", "nbhip_set_ignore",
-    "nbhip_histogram", "nbhip_orbit_tangent", "nbhip_restore",
+    "nbhip_histogram", "nbhip_orbit_tangent", "nbhip_restore", "nbhip_histogram_dispersion",
 ]
 
 _LIB: Optional[C.CDLL] = None
@@ -151,6 +156,8 @@ def load_library() -> C.CDLL:
         "nbhip_orbit_tangent": (u, [C.POINTER(_Potential), _dp, _dp, C.c_double, C.c_int, _dp,
                                     _dp, _dp, _dp]),
         "nbhip_restore": (u, [st, C.c_int, C.c_uint, C.c_double, C.c_uint]),
+        "nbhip_histogram_dispersion": (u, [st, C.c_int, C.POINTER(_HistParams),
+                                           C.POINTER(_DispParams), _dp, _dp, _dp, _dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -482,6 +489,25 @@ class NBodyState:
         self._check(self.lib.nbhip_histogram(self._h, C.byref(h), _ptr(counts, C.c_uint),
                                              _ptr(totals, C.c_uint)))
         return counts, totals
+
+    def histogram_dispersion(self, hp, sim: int = 0, iter_max: int = 6, beta_sigma: float = 2.5,
+                             vel_sigma: float = 2.5, beta_correct: float = 1.111,
+                             vel_correct: float = 1.111):
+        """Beta and line-of-sight velocity dispersions per bin with their
+        errors (nbCreateHistogram's histograms 1 and 2 after outlier
+        rejection), for simulation `sim`.  The defaults are the sample
+        workunit's (SigmaCutoff 2.5, SigmaIter 6, Correction 1.111,
+        nbody/sample_workunits/for_developers.lua:93-95).  Returns
+        (beta_disp, beta_err, vlos_disp, vlos_err), each [bins]."""
+        h = _HistParams(hp.phi, hp.theta, hp.psi, hp.lambdaStart, hp.lambdaEnd, hp.betaStart,
+                        hp.betaEnd, hp.lambdaBins, hp.betaBins, hp.sunGCDist)
+        d = _DispParams(int(iter_max), float(beta_sigma), float(vel_sigma), float(beta_correct),
+                        float(vel_correct))
+        nbin = hp.lambdaBins * hp.betaBins
+        out = [np.empty(nbin) for _ in range(4)]
+        self._check(self.lib.nbhip_histogram_dispersion(self._h, int(sim), C.byref(h),
+                                                        C.byref(d), *[_ptr(o) for o in out]))
+        return tuple(out)
 
     def gravity_vjp(self, gacc):
         """(dL/dpos [n,3], dL/dmass [n]) of one force evaluation at the current

@@ -1,0 +1,94 @@
+"""Test infrastructure only (the checker; the product never imports this):
+a numpy restatement of nbCreateHistogram's dispersion histograms, beta
+(histogram 1) and line-of-sight velocity (histogram 2), with the reference's
+outlier rejection, summed in body order as the reference does.
+
+  * binning and (lambda, beta): nbody/src/nbody_histogram.c:858-903, through
+    the host lambda_beta (milkywayathome_client_amd/histogram.py, pinned by
+    tests/test_histogram.py);
+  * calc_vLOS: nbody/src/nbody_mass.c:171-179;
+  * per-bin sums: nbody/src/nbody_histogram.c:905-917 (np.add.at adds in
+    index order, i.e. body order);
+  * nbCalcDisp: nbody/src/nbody_mass.c:195-245 (only bins with more than 10
+    bodies change; the correction factor on every call after the first);
+  * nbRemoveOutliers: nbody/src/nbody_mass.c:247-316 (mean and dispersion of
+    the previous round, keep |mean - value| < sigma * dispersion);
+  * the rounds: nbody/src/nbody_histogram.c:964-999.
+
+Pinned by closed forms in tests/test_dispersion_ref.py (symmetric bins,
+a planted outlier, bins of 10 or fewer bodies, the line-of-sight velocity of
+a radial orbit).
+"""
+import numpy as np
+
+from milkywayathome_client_amd import histogram as H
+
+
+def body_values(pos, vel, hp: H.HistogramParams, ignore=None):
+    """Bin index (-1: ignored or outside), beta (degrees), v_los per body."""
+    pos = np.asarray(pos, dtype=np.float64)
+    vel = np.asarray(vel, dtype=np.float64)
+    lam, beta = H.lambda_beta(pos, hp)
+    ls = (hp.lambdaEnd - hp.lambdaStart) / hp.lambdaBins
+    bs = (hp.betaEnd - hp.betaStart) / hp.betaBins
+    li = np.floor((lam - hp.lambdaStart) / ls)
+    bi = np.floor((beta - hp.betaStart) / bs)
+    ok = (li >= 0) & (li < hp.lambdaBins) & (bi >= 0) & (bi < hp.betaBins)
+    if ignore is not None:
+        ok &= ~np.asarray(ignore, dtype=bool)
+    b = np.where(ok, li * hp.betaBins + bi, -1).astype(np.int64)
+    xs = pos[:, 0] + hp.sunGCDist
+    mag = np.sqrt(xs * xs + pos[:, 1] ** 2 + pos[:, 2] ** 2)
+    vlos = (xs * vel[:, 0] + pos[:, 1] * vel[:, 1] + pos[:, 2] * vel[:, 2]) / mag
+    return b, beta, vlos
+
+
+def _calc_disp(sum_, sq, removed, raw, initial, correction, var, err):
+    for i in range(raw.size):
+        count = raw[i] - removed[i]
+        if count > 10.0:
+            n_new = count - 1.0
+            n_ratio = count / n_new
+            dispsq = (sq[i] / n_new) - n_ratio * (sum_[i] / count) ** 2
+            if not initial:
+                dispsq *= correction
+            var[i] = np.sqrt(dispsq)
+            err[i] = np.sqrt((count + 1) / (count * n_new)) * var[i]
+
+
+def _sums(b, v, nbin, keep):
+    s = np.zeros(nbin)
+    q = np.zeros(nbin)
+    r = np.zeros(nbin)
+    sel = b >= 0
+    np.add.at(s, b[sel & keep], v[sel & keep])
+    np.add.at(q, b[sel & keep], v[sel & keep] ** 2)
+    np.add.at(r, b[sel & ~keep], 1.0)
+    return s, q, r
+
+
+def dispersion(b, v, nbin, iter_max, sigma, correction):
+    """(dispersion, error) per bin for the values v of binned bodies b."""
+    raw = np.bincount(b[b >= 0], minlength=nbin).astype(np.float64)
+    var = np.zeros(nbin)
+    err = np.zeros(nbin)
+    s, q, r = _sums(b, v, nbin, np.ones(b.size, dtype=bool))
+    _calc_disp(s, q, r, raw, True, correction, var, err)
+    for _ in range(iter_max):
+        with np.errstate(divide="ignore", invalid="ignore"):
+            ave = s / (raw - r)
+        bb = np.where(b >= 0, b, 0)
+        keep = np.abs(ave[bb] - v) < sigma * var[bb]
+        s, q, r = _sums(b, v, nbin, keep)
+        _calc_disp(s, q, r, raw, False, correction, var, err)
+    return var, err
+
+
+def dispersions(pos, vel, hp, ignore=None, iter_max=6, beta_sigma=2.5, vel_sigma=2.5,
+                beta_correct=1.111, vel_correct=1.111):
+    """(beta_disp, beta_err, vlos_disp, vlos_err) as nbhip_histogram_dispersion."""
+    b, beta, vlos = body_values(pos, vel, hp, ignore)
+    nbin = hp.lambdaBins * hp.betaBins
+    bd, be = dispersion(b, beta, nbin, iter_max, beta_sigma, beta_correct)
+    vd, ve = dispersion(b, vlos, nbin, iter_max, vel_sigma, vel_correct)
+    return bd, be, vd, ve

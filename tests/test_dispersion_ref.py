@@ -1,0 +1,133 @@
+"""The dispersion terms of the likelihood, on the CPU:
+
+* nbLikelihood (likelihood.nb_likelihood) against the known answers of the
+  reference's nbody/tests/average_bins_test.c: its histogram file
+  (tests/golden/average_bins_test.hist, copied as a fixture) compared with
+  its four hand-built simulated histograms gives 0.5 for every averaging
+  width it tries;
+* the numpy restatement of the dispersion histograms (oracle/histogram_ref.py,
+  the checker of nbhip_histogram_dispersion) against closed forms: a bin of
+  bodies at beta = +-a, a planted outlier that the rejection removes, bins of
+  ten or fewer bodies, and the line-of-sight velocity of a radial orbit."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from milkywayathome_client_amd import histogram as H
+from milkywayathome_client_amd import likelihood as L
+from oracle import histogram_ref as R
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def test_average_bins_known_answers():
+    data = L.read_histogram(os.path.join(GOLDEN, "average_bins_test.hist"))
+    assert data.beta_disp_bins == 3
+    dv, de = data.comps[1]
+    use = data.use_bin
+    h2 = ([10.0, 22.0, 10.0], [3 * math.sqrt(5.0), 3 * math.sqrt(15.0), 3 * math.sqrt(5.0)])
+    h3 = ([0.0, 12.0, 16.0], [0.0, 6.0, 8.0])
+    h4 = ([6.0, 14.0, 4.0], [2.0, 5.0, 12.0])
+    h5_data = (np.array([0.0, 0.0, 1.0]), np.array([-1.0, -1.0, 12.0]))
+    cases = [
+        (dv, de, h2, data.beta_disp_bins),      # avgScore
+        (dv, de, h3, data.beta_disp_bins),      # avgScore2: one bin without data left out
+        (dv, de, h2, 4),                        # avgScore3: even width -> 3
+        (dv, de, h4, 0),                        # score: centre bin only
+        (dv, de, h4, 1),                        # score1
+        (dv, de, h4, 2),                        # score2
+        (h5_data[0], h5_data[1], h3, 3),        # boundScore: last bin at the edge
+    ]
+    for var, err, (hv, he), avg in cases:
+        got = L.nb_likelihood(var, err, use, np.array(hv), np.array(he), avg)
+        assert got == pytest.approx(0.5, abs=1e-12), (hv, avg)
+
+
+def _bodies_at(hp, lam_deg, beta_deg, dist=20.0):
+    """Positions whose (lambda, beta) in the histogram frame are given."""
+    R_ = hp.rotation()
+    lam = np.radians(np.asarray(lam_deg, dtype=float))
+    bet = np.radians(np.asarray(beta_deg, dtype=float))
+    # the frame's third axis is minus the beta direction (beta = asin(-z'/r))
+    u = np.stack([np.cos(bet) * np.cos(lam), np.cos(bet) * np.sin(lam), -np.sin(bet)], 1)
+    helio = dist * u @ R_
+    return helio - np.array([hp.sunGCDist, 0.0, 0.0])
+
+
+def test_frame_helper():
+    hp = H.HistogramParams()
+    pos = _bodies_at(hp, [-31.0, 12.5], [2.0, -7.0])
+    lam, beta = H.lambda_beta(pos, hp)
+    np.testing.assert_allclose(lam, [-31.0, 12.5], atol=1e-10)
+    np.testing.assert_allclose(beta, [2.0, -7.0], atol=1e-10)
+
+
+def test_symmetric_bin_closed_form():
+    hp = H.HistogramParams(betaStart=-15.0, betaEnd=15.0, betaBins=1)
+    n, a = 40, 1.5
+    beta = np.where(np.arange(n) % 2 == 0, a, -a)
+    pos = _bodies_at(hp, np.full(n, 3.1), beta)
+    vel = np.zeros_like(pos)
+    bd, be, vd, ve = R.dispersions(pos, vel, hp)
+    k = int(np.floor((3.1 - hp.lambdaStart) / 6.0))
+    # no body is rejected: the variance of +-a with n - 1 weighting, times the
+    # correction of the later rounds
+    disp = a * math.sqrt(n / (n - 1.0) * 1.111)
+    assert bd[k] == pytest.approx(disp, rel=1e-12)
+    assert be[k] == pytest.approx(math.sqrt((n + 1.0) / (n * (n - 1.0))) * disp, rel=1e-12)
+    others = np.arange(bd.size) != k
+    assert np.all(bd[others] == 0.0) and np.all(be[others] == 0.0)
+    assert vd[k] == 0.0                      # every v_los is 0
+
+
+def test_planted_outlier_is_rejected():
+    hp = H.HistogramParams()
+    beta = np.concatenate([np.where(np.arange(40) % 2 == 0, 1.0, -1.0), [12.0]])
+    pos = _bodies_at(hp, np.full(41, -60.5), beta)
+    vel = np.zeros_like(pos)
+    bd, be, _, _ = R.dispersions(pos, vel, hp)
+    k = int(np.floor((-60.5 - hp.lambdaStart) / 6.0))
+    # round 0 sees the outlier: mean 12/41, dispersion sqrt(184/40 - 41/40
+    # (12/41)^2) = 2.12; 12 is 11.7 > 2.5 * 2.12 from the mean, so every later
+    # round leaves it out and keeps the 40 bodies at +-1
+    assert bd[k] == pytest.approx(math.sqrt(40.0 / 39.0 * 1.111), rel=1e-12)
+    b, vals, _ = R.body_values(pos, vel, hp)
+    v0, _ = R.dispersion(b, vals, 50, 0, 2.5, 1.111)       # no rejection round
+    assert v0[k] > 2.0
+
+
+def test_small_bins_keep_zero():
+    hp = H.HistogramParams()
+    pos = _bodies_at(hp, np.full(10, 40.0), np.linspace(-3, 3, 10))
+    bd, be, _, _ = R.dispersions(pos, np.zeros_like(pos), hp)
+    assert np.all(bd == 0.0) and np.all(be == 0.0)
+
+
+def test_line_of_sight_velocity_of_radial_motion():
+    hp = H.HistogramParams()
+    pos = _bodies_at(hp, [10.0, -40.0], [1.0, -2.0], dist=25.0)
+    helio = pos + np.array([hp.sunGCDist, 0.0, 0.0])
+    u = helio / np.linalg.norm(helio, axis=1, keepdims=True)
+    vel = np.array([[150.0], [-80.0]]) * u
+    _, _, vlos = R.body_values(pos, vel, hp)
+    np.testing.assert_allclose(vlos, [150.0, -80.0], rtol=1e-13)
+
+
+def test_likelihood_with_dispersions_against_itself_is_zero():
+    hp = H.HistogramParams()
+    rng = np.random.default_rng(4)
+    lam = rng.uniform(-100, 100, 30000)
+    beta = rng.normal(0.0, 2.0, 30000)
+    pos = _bodies_at(hp, lam, beta)
+    vel = rng.normal(0.0, 30.0, pos.shape)
+    counts = np.bincount(R.body_values(pos, vel, hp)[0].clip(0), minlength=50)
+    disp = R.dispersions(pos, vel, hp)
+    sim = L.from_counts(counts, hp, 30000, 1e-3, dispersions=disp)
+    out = L.system_likelihood(sim, sim, 30000, use_beta_disp=True, use_vel_disp=True)
+    assert out[0] == 0.0 and out[3] == 0.0 and out[4] == 0.0
+    # a dwarf with a wider beta spread scores a positive beta-dispersion term
+    wide = L.from_counts(counts, hp, 30000, 1e-3,
+                         dispersions=R.dispersions(_bodies_at(hp, lam, beta * 1.5), vel, hp))
+    assert L.system_likelihood(sim, wide, 30000, use_beta_disp=True)[3] > 1.0

@@ -92,3 +92,36 @@ def test_likelihood_from_gpu_counts():
     assert out[1] > 0.0 and np.isfinite(out[0])
     st.close()
     st2.close()
+
+
+def test_dispersions_match_host_restatement():
+    """nbhip_histogram_dispersion (beta and line-of-sight velocity dispersions
+    with outlier rejection) against the numpy restatement of the reference's
+    loop (oracle/histogram_ref.py): same bins, same rejections, the sums
+    differing only in association (1e-12); for one state with ignored bodies
+    and for a member of a batch."""
+    from oracle import histogram_ref as R
+    hp = H.HistogramParams()
+    pos, vel, m = _stream(40000, 7, (0.0, 0.0, 0.0))
+    rng = np.random.default_rng(8)
+    vel[rng.choice(len(m), 50, replace=False)] *= 20.0        # velocity outliers
+    ignore = np.zeros(len(m), dtype=bool)
+    ignore[::5] = True
+    assert _edges_clear(pos, hp)
+    ctx = nb.NBodyCtx.create(timestep=1e-3, timeEvolve=1.0, eps2=1e-3, theta=0.5)
+    st = nb.NBodyState(ctx, None, pos, vel, m, prestep=False)
+    st.set_ignore(ignore)
+    got = st.histogram_dispersion(hp)
+    ref = R.dispersions(pos, vel, hp, ignore=ignore)
+    for g, r in zip(got, ref):
+        np.testing.assert_allclose(g, r, rtol=1e-12, atol=0.0)
+    assert np.count_nonzero(got[0]) >= 15 and np.count_nonzero(got[2]) >= 15
+    st.close()
+    members = [_stream(k, 20 + i, (4.0 * i, 0.0, 0.0)) for i, k in enumerate([3000, 25000, 8000])]
+    bt = nb.NBodyBatch(ctx, None, members, prestep=False)
+    for k, (p, v, _) in enumerate(members):
+        got = bt.histogram_dispersion(hp, sim=k)
+        ref = R.dispersions(p, v, hp)
+        for g, r in zip(got, ref):
+            np.testing.assert_allclose(g, r, rtol=1e-12, atol=0.0)
+    bt.close()

@@ -125,6 +125,8 @@ def test_worst_case_and_mismatch():
     assert out[0] == out[1] == L.DEFAULT_WORST_CASE and np.all(out[2:] == 0.0)
     assert np.all(np.isnan(L.system_likelihood(data, _hist([1.0], [0.0], 50, 100, 1.0), 100)))
     with pytest.raises(NotImplementedError):
+        L.system_likelihood(data, data, 100, use_vlos=True)
+    with pytest.raises(ValueError):                      # no dispersion columns
         L.system_likelihood(data, data, 100, use_vel_disp=True)
     # an EMD above 50 (the maximum distance) is NaN
     far = _hist([1.0, 0.0], [0.0, 0.0], 100, 100, 1.0)
