@@ -240,25 +240,28 @@ unsigned nbhip_set_ignore(nbhip_state* st, const unsigned char* ignore);
 unsigned nbhip_histogram(nbhip_state* st, const nbhip_histogram_params* hp, unsigned* counts,
                          unsigned* totals);
 
-/* The outlier-rejection settings of the dispersion histograms (NBodyCtx's
- * IterMax, BetaSigma, VelSigma, BetaCorrect, VelCorrect;
- * nbody/src/nbody_lua_types/nbody_lua_nbodyctx.c:127-133). */
+/* The outlier-rejection settings of the per-bin histograms (NBodyCtx's
+ * IterMax, BetaSigma, VelSigma, DistSigma, BetaCorrect, VelCorrect,
+ * DistCorrect; nbody/src/nbody_lua_types/nbody_lua_nbodyctx.c:127-135). */
 typedef struct nbhip_disp_params {
     int iter_max;
-    double beta_sigma, vel_sigma;
-    double beta_correct, vel_correct;
+    double beta_sigma, vel_sigma, dist_sigma;
+    double beta_correct, vel_correct, dist_correct;
 } nbhip_disp_params;
 
-/* Histograms 1 and 2 of nbCreateHistogram (nbody/src/nbody_histogram.c:
- * 858-1000) for simulation `sim` on the current state: per bin the beta
- * dispersion and the line-of-sight velocity dispersion (calc_vLOS,
- * nbody/src/nbody_mass.c:171-179) with their errors, after nbCalcDisp and
- * iter_max rounds of nbRemoveOutliers (nbody/src/nbody_mass.c:195-316).
- * Ignored bodies are left out as in nbhip_histogram.  Each output holds
- * lambda_bins * beta_bins entries (lambda-major); any may be NULL. */
+/* Histograms 1-5 of nbCreateHistogram (nbody/src/nbody_histogram.c:858-1070)
+ * for simulation `sim` on the current state, after nbCalcDisp and iter_max
+ * rounds of nbRemoveOutliers (nbody/src/nbody_mass.c:195-316): 1 the beta
+ * dispersion, 2 the line-of-sight velocity dispersion (calc_vLOS,
+ * nbody/src/nbody_mass.c:171-179), 3 the average line-of-sight velocity,
+ * 4 the average beta, 5 the average heliocentric distance (calc_distance,
+ * :181-187).  out holds 10 * lambda_bins * beta_bins doubles:
+ * out[(2 (k - 1) + 0) * bins + b] the value of histogram k in bin b
+ * (lambda-major), out[(2 (k - 1) + 1) * bins + b] its error (-1 for an
+ * average over 10 or fewer bodies).  Ignored bodies are left out as in
+ * nbhip_histogram. */
 unsigned nbhip_histogram_dispersion(nbhip_state* st, int sim, const nbhip_histogram_params* hp,
-                                    const nbhip_disp_params* dp, double* beta_disp,
-                                    double* beta_err, double* vlos_disp, double* vlos_err);
+                                    const nbhip_disp_params* dp, double* out);
 
 /* ---- adjoint leapfrog ------------------------------------------------------
  * No counterpart in the reference.  Gradients of a loss on the final state

@@ -1112,11 +1112,9 @@ extern "C" unsigned nbhip_histogram(nbhip_state* st, const nbhip_histogram_param
 
 extern "C" unsigned nbhip_histogram_dispersion(nbhip_state* st, int sim,
                                                const nbhip_histogram_params* hp,
-                                               const nbhip_disp_params* dp, double* beta_disp,
-                                               double* beta_err, double* vlos_disp,
-                                               double* vlos_err)
+                                               const nbhip_disp_params* dp, double* out)
 {
-    if (!st || !hp || !dp || sim < 0 || sim >= st->nsim) return NBHIP_USER_ERROR;
+    if (!st || !hp || !dp || !out || sim < 0 || sim >= st->nsim) return NBHIP_USER_ERROR;
     if (hp->lambda_bins < 1 || hp->beta_bins < 1 || !(hp->lambda_end > hp->lambda_start) ||
         !(hp->beta_end > hp->beta_start) || dp->iter_max < 0 || dp->iter_max > 1000) {
         nbh_set_error("nbhip_histogram_dispersion: bad histogram or rejection settings");
@@ -1125,37 +1123,33 @@ extern "C" unsigned nbhip_histogram_dispersion(nbhip_state* st, int sim,
     NBH_CHECK(hipSetDevice(st->device));
     const int off = st->h_body_off[sim], cnt = st->h_body_n[sim];
     const size_t nbin = (size_t) hp->lambda_bins * (size_t) hp->beta_bins;
-    const size_t nblk = cnt > 0 ? ((size_t) cnt + 1023) / 1024 : 1;
-    std::vector<double> h_state(12 * nbin, 0.0);
-    if (cnt > 0) {
-        int* dbin = nullptr;
-        double *dbeta = nullptr, *dvlos = nullptr, *part = nullptr, *state = nullptr;
-        hipError_t e = nbh_alloc(&dbin, (size_t) cnt);
-        if (e == hipSuccess) e = nbh_alloc(&dbeta, (size_t) cnt);
-        if (e == hipSuccess) e = nbh_alloc(&dvlos, (size_t) cnt);
-        if (e == hipSuccess) e = nbh_alloc(&part, 7 * nblk * nbin);
-        if (e == hipSuccess) e = nbh_alloc(&state, 12 * nbin);
-        if (e == hipSuccess) {
-            nbh_dispersion(st, off, cnt, hp, dp, dbin, dbeta, dvlos, part, state);
-            e = hipGetLastError();
-        }
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(h_state.data(), state, 12 * nbin * sizeof(double),
-                               hipMemcpyDeviceToHost, st->stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(st->stream);
-        for (void* p : { (void*) dbin, (void*) dbeta, (void*) dvlos, (void*) part, (void*) state })
-            if (p) (void) hipFree(p);
-        if (e != hipSuccess) {
-            nbh_set_error("nbhip_histogram_dispersion: %s", hipGetErrorString(e));
-            return NBHIP_GPU_ERROR;
-        }
+    if (cnt <= 0) {
+        /* no bodies: every dispersion 0, every average 0 with error -1 */
+        for (size_t k = 0; k < 10; ++k)
+            for (size_t b = 0; b < nbin; ++b) out[k * nbin + b] = (k >= 4 && k % 2) ? -1.0 : 0.0;
+        return NBHIP_SUCCESS;
     }
-    for (size_t b = 0; b < nbin; ++b) {
-        const double* t = &h_state[12 * b];
-        if (beta_disp) beta_disp[b] = t[3];
-        if (beta_err) beta_err[b] = t[4];
-        if (vlos_disp) vlos_disp[b] = t[8];
-        if (vlos_err) vlos_err[b] = t[9];
+    const size_t nblk = ((size_t) cnt + 1023) / 1024;
+    int* dbin = nullptr;
+    double *dval = nullptr, *part = nullptr, *state = nullptr, *dout = nullptr;
+    hipError_t e = nbh_alloc(&dbin, (size_t) cnt);
+    if (e == hipSuccess) e = nbh_alloc(&dval, 3 * (size_t) cnt);
+    if (e == hipSuccess) e = nbh_alloc(&part, 10 * nblk * nbin);
+    if (e == hipSuccess) e = nbh_alloc(&state, 20 * nbin);
+    if (e == hipSuccess) e = nbh_alloc(&dout, 10 * nbin);
+    if (e == hipSuccess) {
+        nbh_dispersion(st, off, cnt, hp, dp, dbin, dval, part, state, dout);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(out, dout, 10 * nbin * sizeof(double), hipMemcpyDeviceToHost,
+                           st->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(st->stream);
+    for (void* p : { (void*) dbin, (void*) dval, (void*) part, (void*) state, (void*) dout })
+        if (p) (void) hipFree(p);
+    if (e != hipSuccess) {
+        nbh_set_error("nbhip_histogram_dispersion: %s", hipGetErrorString(e));
+        return NBHIP_GPU_ERROR;
     }
     return NBHIP_SUCCESS;
 }

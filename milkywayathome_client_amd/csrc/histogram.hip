@@ -92,26 +92,35 @@ __global__ __launch_bounds__(256) void k_histogram(int n, const double* __restri
 }
 
 /* ---------------------------------------------------------------------- */
-/* beta and line-of-sight velocity dispersions per bin                     */
+/* per-bin dispersions and averages of beta, v_los and distance            */
 /* ---------------------------------------------------------------------- */
-/* nbCreateHistogram's dispersion histograms 1 (beta) and 2 (line-of-sight
- * velocity) with the outlier rejection that follows
- * (nbody/src/nbody_histogram.c:858-1000; nbCalcDisp and nbRemoveOutliers,
- * nbody/src/nbody_mass.c:195-316; calc_vLOS :171-179), for the bodies of
- * one simulation.
+/* nbCreateHistogram's histograms 1-5 (nbody/src/nbody_histogram.c:858-1070):
+ * the beta and line-of-sight velocity dispersions, the average line-of-sight
+ * velocity, beta and distance, each after nbCalcDisp and the rounds of
+ * nbRemoveOutliers (nbody/src/nbody_mass.c:195-316; calc_vLOS and
+ * calc_distance :171-187), for the bodies of one simulation.  The averages go
+ * through the same rounds as the dispersions of their quantity (histograms 3
+ * and 2 see the same values, sigma and correction; so do 4 and 1), so three
+ * quantities suffice: q = 0 beta, 1 v_los, 2 distance.
  *
- * k_disp_values records each body's bin (-1: ignored or outside), beta and
- * line-of-sight velocity once.  Each round of sums is k_disp_part, one
- * workgroup per 1024 consecutive bodies with a thread per bin that adds the
- * workgroup's bodies in body order, then k_disp_finish, a thread per bin
- * that adds the workgroups' partial sums in workgroup order and applies
- * nbCalcDisp.  The sums are therefore deterministic (independent of timing
- * and batching) and differ from the reference's single running sum only in
- * association.  Round 0 takes every binned body; each later round keeps a
- * body when |mean - value| < sigma * dispersion of the previous round. */
+ * k_disp_values records each body's bin (-1: ignored or outside) and the
+ * three values once.  Each round of sums is k_disp_part, one workgroup per
+ * 1024 consecutive bodies with a thread per bin that adds the workgroup's
+ * bodies in body order, then k_disp_finish, a thread per bin that adds the
+ * workgroups' partial sums in workgroup order and applies nbCalcDisp.  The
+ * sums are therefore deterministic (independent of timing and batching) and
+ * differ from the reference's single running sum only in association.
+ * Round 0 takes every binned body; each later round keeps a body when
+ * |mean - value| < sigma * dispersion of the previous round. */
 #define NBH_DISP_TILE 1024
-#define NBH_DISP_W 7          /* per bin: beta sum, sq, removed; vlos sum, sq, removed; count */
-#define NBH_DISP_S 12         /* per bin state: for beta and vlos: sum sq removed var err pad */
+#define NBH_DISP_Q 3
+#define NBH_DISP_W (3 * NBH_DISP_Q + 1)   /* per q: sum, sq, removed; then count */
+#define NBH_DISP_S 20                     /* per q: sum sq removed var err -; [18] raw count */
+
+struct DispRound {
+    double sigma[NBH_DISP_Q];
+    double correct[NBH_DISP_Q];
+};
 
 __global__ __launch_bounds__(256) void k_disp_values(int off, int cnt,
                                                      const double* __restrict__ px,
@@ -122,76 +131,67 @@ __global__ __launch_bounds__(256) void k_disp_values(int off, int cnt,
                                                      const double* __restrict__ vz,
                                                      const unsigned char* __restrict__ ignore,
                                                      HistParams h, int* __restrict__ dbin,
-                                                     double* __restrict__ dbeta,
-                                                     double* __restrict__ dvlos)
+                                                     double* __restrict__ dval)
 {
     const int k = blockIdx.x * 256 + threadIdx.x;
     if (k >= cnt) return;
     const int i = off + k;
     int bin = -1;
-    double beta = 0.0, vl = 0.0;
+    double beta = 0.0, vl = 0.0, dist = 0.0;
     if (!(ignore && ignore[i])) {
         const double x = px[i], y = py[i], z = pz[i];
         bin = nbh_lambda_beta_bin(h, x, y, z, beta);
         const double xs = x + h.sun;
         const double mag = sqrt(xs * xs + y * y + z * z);
         vl = (xs * vx[i] + y * vy[i] + z * vz[i]) / mag;
+        dist = mag;
     }
     dbin[k] = bin;
-    dbeta[k] = beta;
-    dvlos[k] = vl;
+    dval[k] = beta;
+    dval[(size_t) cnt + k] = vl;
+    dval[2 * (size_t) cnt + k] = dist;
 }
 
-__global__ __launch_bounds__(256) void k_disp_part(int cnt, int nbin, int initial,
-                                                   double beta_sigma, double vel_sigma,
+__global__ __launch_bounds__(256) void k_disp_part(int cnt, int nbin, int initial, DispRound r,
                                                    const int* __restrict__ dbin,
-                                                   const double* __restrict__ dbeta,
-                                                   const double* __restrict__ dvlos,
+                                                   const double* __restrict__ dval,
                                                    const double* __restrict__ state,
                                                    double* __restrict__ part)
 {
     __shared__ int sb[NBH_DISP_TILE];
-    __shared__ double sbeta[NBH_DISP_TILE];
-    __shared__ double svl[NBH_DISP_TILE];
+    __shared__ double sv[NBH_DISP_Q][NBH_DISP_TILE];
     const int base = blockIdx.x * NBH_DISP_TILE;
     const int m = min(NBH_DISP_TILE, cnt - base);
     for (int j = threadIdx.x; j < m; j += 256) {
         sb[j] = dbin[base + j];
-        sbeta[j] = dbeta[base + j];
-        svl[j] = dvlos[base + j];
+        for (int q = 0; q < NBH_DISP_Q; ++q) sv[q][j] = dval[(size_t) q * cnt + base + j];
     }
     __syncthreads();
     for (int b = threadIdx.x; b < nbin; b += 256) {
         const double* st = state + (size_t) b * NBH_DISP_S;
         /* previous round: mean = sum / (count - removed), sigma = dispersion */
-        double cB = 0.0, aveB = 0.0, sigB = 0.0, aveV = 0.0, sigV = 0.0;
-        if (!initial) {
-            cB = st[11];
-            aveB = st[0] / (cB - st[2]);
-            sigB = st[3];
-            aveV = st[5] / (cB - st[7]);
-            sigV = st[8];
+        double ave[NBH_DISP_Q], cut[NBH_DISP_Q];
+        for (int q = 0; q < NBH_DISP_Q; ++q) {
+            ave[q] = initial ? 0.0 : st[6 * q] / (st[18] - st[6 * q + 2]);
+            cut[q] = initial ? 0.0 : r.sigma[q] * st[6 * q + 3];
         }
-        double s0 = 0.0, q0 = 0.0, r0 = 0.0, s1 = 0.0, q1 = 0.0, r1 = 0.0, c = 0.0;
+        double acc[NBH_DISP_W];
+        for (int w = 0; w < NBH_DISP_W; ++w) acc[w] = 0.0;
         for (int j = 0; j < m; ++j) {
             if (sb[j] != b) continue;
-            c += 1.0;
-            const double vb = sbeta[j], vv = svl[j];
-            if (initial || fabs(aveB - vb) < beta_sigma * sigB) {
-                s0 += vb;
-                q0 += vb * vb;
-            } else {
-                r0 += 1.0;
-            }
-            if (initial || fabs(aveV - vv) < vel_sigma * sigV) {
-                s1 += vv;
-                q1 += vv * vv;
-            } else {
-                r1 += 1.0;
+            acc[NBH_DISP_W - 1] += 1.0;
+            for (int q = 0; q < NBH_DISP_Q; ++q) {
+                const double v = sv[q][j];
+                if (initial || fabs(ave[q] - v) < cut[q]) {
+                    acc[3 * q] += v;
+                    acc[3 * q + 1] += v * v;
+                } else {
+                    acc[3 * q + 2] += 1.0;
+                }
             }
         }
         double* o = part + ((size_t) blockIdx.x * nbin + b) * NBH_DISP_W;
-        o[0] = s0; o[1] = q0; o[2] = r0; o[3] = s1; o[4] = q1; o[5] = r1; o[6] = c;
+        for (int w = 0; w < NBH_DISP_W; ++w) o[w] = acc[w];
     }
 }
 
@@ -213,8 +213,7 @@ __device__ __forceinline__ void nbh_calc_disp(double sum, double sq, double remo
     }
 }
 
-__global__ __launch_bounds__(256) void k_disp_finish(int nbin, int nblk, int initial,
-                                                     double beta_correct, double vel_correct,
+__global__ __launch_bounds__(256) void k_disp_finish(int nbin, int nblk, int initial, DispRound r,
                                                      const double* __restrict__ part,
                                                      double* __restrict__ state)
 {
@@ -227,15 +226,45 @@ __global__ __launch_bounds__(256) void k_disp_finish(int nbin, int nblk, int ini
         for (int w = 0; w < NBH_DISP_W; ++w) t[w] += o[w];
     }
     double* st = state + (size_t) b * NBH_DISP_S;
-    if (initial) {
+    st[18] = t[NBH_DISP_W - 1];
+    for (int q = 0; q < NBH_DISP_Q; ++q) {
+        double* sq = st + 6 * q;
         /* every bin starts at dispersion 0 (nbody_histogram.c:846-854) */
-        st[3] = st[4] = st[8] = st[9] = 0.0;
+        if (initial) sq[3] = sq[4] = 0.0;
+        sq[0] = t[3 * q];
+        sq[1] = t[3 * q + 1];
+        sq[2] = t[3 * q + 2];
+        nbh_calc_disp(sq[0], sq[1], sq[2], st[18], initial, r.correct[q], sq[3], sq[4]);
     }
-    st[0] = t[0]; st[1] = t[1]; st[2] = t[2];
-    st[5] = t[3]; st[6] = t[4]; st[7] = t[5];
-    st[11] = t[6];
-    nbh_calc_disp(t[0], t[1], t[2], t[6], initial, beta_correct, st[3], st[4]);
-    nbh_calc_disp(t[3], t[4], t[5], t[6], initial, vel_correct, st[8], st[9]);
+}
+
+/* Histograms 1-5 from the final state: dispersions of beta and v_los, and
+ * the averages of v_los, beta and distance with error dispersion / sqrt(n)
+ * (nbody/src/nbody_histogram.c:996-1070; -1 and 0 below 11 bodies).
+ * out[(2 (k - 1) + {0: value, 1: error}) * nbin + b] for histogram k. */
+__global__ __launch_bounds__(256) void k_disp_out(int nbin, const double* __restrict__ state,
+                                                  double* __restrict__ out)
+{
+    const int b = blockIdx.x * 256 + threadIdx.x;
+    if (b >= nbin) return;
+    const double* st = state + (size_t) b * NBH_DISP_S;
+    const size_t N = (size_t) nbin;
+    out[0 * N + b] = st[3];                         /* 1: beta dispersion */
+    out[1 * N + b] = st[4];
+    out[2 * N + b] = st[6 + 3];                     /* 2: v_los dispersion */
+    out[3 * N + b] = st[6 + 4];
+    const int avg_q[3] = { 1, 0, 2 };               /* 3: v_los, 4: beta, 5: distance */
+    for (int k = 0; k < 3; ++k) {
+        const double* sq = st + 6 * avg_q[k];
+        const int denom = (int) (st[18] - sq[2]);
+        double v = 0.0, e = -1.0;
+        if (denom > 10) {
+            e = sq[3] / sqrt((double) denom);
+            v = sq[0] / denom;
+        }
+        out[(4 + 2 * k) * N + b] = v;
+        out[(5 + 2 * k) * N + b] = e;
+    }
 }
 
 static inline int nbh_blocks(int n, int per) { return n > 0 ? (n + per - 1) / per : 1; }
@@ -243,23 +272,32 @@ static inline int nbh_blocks(int n, int per) { return n > 0 ? (n + per - 1) / pe
 static HistParams nbh_hist_params(const nbhip_histogram_params* hp);
 
 void nbh_dispersion(nbhip_state* st, int off, int cnt, const nbhip_histogram_params* hp,
-                    const nbhip_disp_params* dp, int* dbin, double* dbeta, double* dvlos,
-                    double* part, double* state)
+                    const nbhip_disp_params* dp, int* dbin, double* dval, double* part,
+                    double* state, double* out)
 {
     const HistParams h = nbh_hist_params(hp);
     const int nbin = hp->lambda_bins * hp->beta_bins;
     const int nblk = nbh_blocks(cnt, NBH_DISP_TILE);
+    DispRound r;
+    r.sigma[0] = dp->beta_sigma;
+    r.sigma[1] = dp->vel_sigma;
+    r.sigma[2] = dp->dist_sigma;
+    r.correct[0] = dp->beta_correct;
+    r.correct[1] = dp->vel_correct;
+    r.correct[2] = dp->dist_correct;
     hipStream_t s = st->stream;
     hipLaunchKernelGGL(k_disp_values, dim3(nbh_blocks(cnt, 256)), dim3(256), 0, s, off, cnt,
                        st->px, st->py, st->pz, st->vx, st->vy, st->vz, st->ignore, h, dbin,
-                       dbeta, dvlos);
+                       dval);
     for (int round = 0; round <= dp->iter_max; ++round) {
         const int initial = round == 0;
-        hipLaunchKernelGGL(k_disp_part, dim3(nblk), dim3(256), 0, s, cnt, nbin, initial,
-                           dp->beta_sigma, dp->vel_sigma, dbin, dbeta, dvlos, state, part);
+        hipLaunchKernelGGL(k_disp_part, dim3(nblk), dim3(256), 0, s, cnt, nbin, initial, r,
+                           dbin, dval, state, part);
         hipLaunchKernelGGL(k_disp_finish, dim3(nbh_blocks(nbin, 256)), dim3(256), 0, s, nbin,
-                           nblk, initial, dp->beta_correct, dp->vel_correct, part, state);
+                           nblk, initial, r, part, state);
     }
+    hipLaunchKernelGGL(k_disp_out, dim3(nbh_blocks(nbin, 256)), dim3(256), 0, s, nbin, state,
+                       out);
 }
 
 static HistParams nbh_hist_params(const nbhip_histogram_params* hp)

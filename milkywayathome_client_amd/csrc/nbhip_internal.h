@@ -182,8 +182,8 @@ void nbh_direct(nbhip_state* st, int nsample, const int* d_idx, double* d_acc);
 void nbh_histogram(nbhip_state* st, const nbhip_histogram_params* hp, unsigned* d_counts,
                    unsigned* d_totals);
 void nbh_dispersion(nbhip_state* st, int off, int cnt, const nbhip_histogram_params* hp,
-                    const nbhip_disp_params* dp, int* dbin, double* dbeta, double* dvlos,
-                    double* part, double* state);
+                    const nbhip_disp_params* dp, int* dbin, double* dval, double* part,
+                    double* state, double* out);
 /* launchers (adjoint.hip) */
 void nbh_adj_kick_end(nbhip_state* st, const double* a_next);
 void nbh_adj_force(nbhip_state* st);

@@ -18,11 +18,11 @@ the output layer that consumes the GPU histogram (nbhip_histogram).
   (nbSystemLikelihood, nbody/src/nbody_likelihood.c:155-323): the geometry,
   mass and dispersion terms, with the reference's NaN / infinity conventions.
 
-The beta and line-of-sight velocity dispersions of a simulation come from
+The per-bin histograms of a simulation (beta and line-of-sight velocity
+dispersions, average line-of-sight velocity, beta and distance) come from
 the GPU (nbhip_histogram_dispersion, with the reference's outlier
-rejection).  The average line-of-sight velocity, beta, distance and proper
-motion components are not computed; asking for them raises
-NotImplementedError.
+rejection).  The proper-motion components are not computed; asking for them
+raises NotImplementedError.
 """
 from __future__ import annotations
 
@@ -86,7 +86,9 @@ def read_histogram(path: str) -> Histogram:
     hdr = {}
     rows = []
     orbit_cols = False
-    for raw in open(path):
+    with open(path) as f:
+        lines = f.readlines()
+    for raw in lines:
         line = raw.rstrip("\n")
         if not line or line.startswith("#"):
             continue
@@ -167,9 +169,8 @@ def from_counts(counts, hp: HistogramParams, total_simulated: int,
     (lambda-major, as nbhip_histogram returns them): bin centres, normalised
     counts and errors as nbNormalizeHistogram computes them.  Like the
     reference, the error of bin (i, j) is computed from the raw count of bin
-    i (identical for one beta bin).  `dispersions` = (beta_disp, beta_err,
-    vlos_disp, vlos_err) per bin, as NBodyState.histogram_dispersion returns
-    them, fills histograms 1 and 2."""
+    i (identical for one beta bin).  `dispersions` = {k: (value, error)} for
+    histograms 1-5, as NBodyState.histogram_dispersion returns it."""
     c = np.asarray(counts, dtype=np.int64).reshape(-1)
     lb, bb = hp.lambdaBins, hp.betaBins
     if c.size != lb * bb:
@@ -194,9 +195,9 @@ def from_counts(counts, hp: HistogramParams, total_simulated: int,
                   use_bin=np.ones(c.size, dtype=np.int64), lam=lam, beta=beta, variable=var,
                   err=err, beta_disp_bins=1)
     if dispersions is not None:
-        bd, be, vd, ve = (np.asarray(x, dtype=np.float64).reshape(-1) for x in dispersions)
-        h.comps[1] = (bd, be)
-        h.comps[2] = (vd, ve)
+        for k, (v, e) in dispersions.items():
+            h.comps[int(k)] = (np.asarray(v, dtype=np.float64).reshape(-1),
+                               np.asarray(e, dtype=np.float64).reshape(-1))
     return h
 
 
@@ -375,13 +376,15 @@ def system_likelihood(data: Histogram, hist: Histogram, nbody: int, method: str 
     out = np.full(10, math.nan)
     if data.lambda_bins != hist.lambda_bins:
         return out
-    if any((use_vlos, use_beta_comp, use_dist, use_prop_mot)):
-        raise NotImplementedError("the average line-of-sight velocity, beta, distance and "
-                                  "proper motion components are not computed")
-    for flag, k in ((use_beta_disp, 1), (use_vel_disp, 2)):
+    if use_prop_mot:
+        raise NotImplementedError("the proper-motion components are not computed")
+    # (flag, histogram, output slot) in the reference's order of addition
+    terms = ((use_beta_disp, 1, 3), (use_vel_disp, 2, 4), (use_vlos, 3, 6),
+             (use_beta_comp, 4, 5), (use_dist, 5, 7))
+    for flag, k, _ in terms:
         if flag and (k not in data.comps or k not in hist.comps):
-            raise ValueError(f"histogram {k} (dispersion) missing from the data or the "
-                             "simulation (from_counts(..., dispersions=...))")
+            raise ValueError(f"histogram {k} missing from the data or the simulation "
+                             "(from_counts(..., dispersions=...))")
     if method != "emd":
         raise NotImplementedError("only the EMD geometry term is implemented")
     if hist.total_num < 0.0001 * float(nbody):
@@ -391,13 +394,13 @@ def system_likelihood(data: Histogram, hist: Histogram, nbody: int, method: str 
     geometry = match_emd(data, hist)
     cost = cost_component(data, hist)
     like = geometry + cost
-    if use_beta_disp:
-        out[3] = nb_likelihood(*data.comps[1], data.use_bin, *hist.comps[1],
-                               data.beta_disp_bins)
-        like += out[3]
-    if use_vel_disp:
-        out[4] = nb_likelihood(*data.comps[2], data.use_bin, *hist.comps[2], 1)
-        like += out[4]
+    # [likelihood, geometry, cost, beta disp, vel disp, beta, LOS velocity,
+    # distance, dec PM, RA PM]: the reference's order of components
+    for flag, k, slot in terms:
+        if flag:
+            avg = data.beta_disp_bins if k == 1 else 1
+            out[slot] = nb_likelihood(*data.comps[k], data.use_bin, *hist.comps[k], avg)
+            like += out[slot]
     out[0] = like
     out[1] = geometry
     out[2] = cost

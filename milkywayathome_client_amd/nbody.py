@@ -77,7 +77,8 @@ class _TreeInfo(C.Structure):
 
 class _DispParams(C.Structure):
     _fields_ = [("iter_max", C.c_int), ("beta_sigma", C.c_double), ("vel_sigma", C.c_double),
-                ("beta_correct", C.c_double), ("vel_correct", C.c_double)]
+                ("dist_sigma", C.c_double), ("beta_correct", C.c_double),
+                ("vel_correct", C.c_double), ("dist_correct", C.c_double)]
 
 
 class _HistParams(C.Structure):
@@ -157,7 +158,7 @@ def load_library() -> C.CDLL:
                                     _dp, _dp, _dp]),
         "nbhip_restore": (u, [st, C.c_int, C.c_uint, C.c_double, C.c_uint]),
         "nbhip_histogram_dispersion": (u, [st, C.c_int, C.POINTER(_HistParams),
-                                           C.POINTER(_DispParams), _dp, _dp, _dp, _dp]),
+                                           C.POINTER(_DispParams), _dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -491,23 +492,25 @@ class NBodyState:
         return counts, totals
 
     def histogram_dispersion(self, hp, sim: int = 0, iter_max: int = 6, beta_sigma: float = 2.5,
-                             vel_sigma: float = 2.5, beta_correct: float = 1.111,
-                             vel_correct: float = 1.111):
-        """Beta and line-of-sight velocity dispersions per bin with their
-        errors (nbCreateHistogram's histograms 1 and 2 after outlier
-        rejection), for simulation `sim`.  The defaults are the sample
-        workunit's (SigmaCutoff 2.5, SigmaIter 6, Correction 1.111,
-        nbody/sample_workunits/for_developers.lua:93-95).  Returns
-        (beta_disp, beta_err, vlos_disp, vlos_err), each [bins]."""
+                             vel_sigma: float = 2.5, dist_sigma: float = 2.5,
+                             beta_correct: float = 1.111, vel_correct: float = 1.111,
+                             dist_correct: float = 1.111) -> dict:
+        """nbCreateHistogram's histograms 1-5 for simulation `sim` after the
+        outlier rejection: {1: beta dispersion, 2: v_los dispersion, 3: v_los
+        average, 4: beta average, 5: distance average}, each (value, error)
+        per bin.  The defaults are the sample workunit's (SigmaCutoff 2.5,
+        SigmaIter 6, Correction 1.111, nbody/sample_workunits/
+        for_developers.lua:93-95)."""
         h = _HistParams(hp.phi, hp.theta, hp.psi, hp.lambdaStart, hp.lambdaEnd, hp.betaStart,
                         hp.betaEnd, hp.lambdaBins, hp.betaBins, hp.sunGCDist)
-        d = _DispParams(int(iter_max), float(beta_sigma), float(vel_sigma), float(beta_correct),
-                        float(vel_correct))
+        d = _DispParams(int(iter_max), float(beta_sigma), float(vel_sigma), float(dist_sigma),
+                        float(beta_correct), float(vel_correct), float(dist_correct))
         nbin = hp.lambdaBins * hp.betaBins
-        out = [np.empty(nbin) for _ in range(4)]
+        out = np.empty(10 * nbin)
         self._check(self.lib.nbhip_histogram_dispersion(self._h, int(sim), C.byref(h),
-                                                        C.byref(d), *[_ptr(o) for o in out]))
-        return tuple(out)
+                                                        C.byref(d), _ptr(out)))
+        out = out.reshape(10, nbin)
+        return {k: (out[2 * (k - 1)].copy(), out[2 * (k - 1) + 1].copy()) for k in range(1, 6)}
 
     def gravity_vjp(self, gacc):
         """(dL/dpos [n,3], dL/dmass [n]) of one force evaluation at the current

@@ -1,19 +1,21 @@
 """Test infrastructure only (the checker; the product never imports this):
-a numpy restatement of nbCreateHistogram's dispersion histograms, beta
-(histogram 1) and line-of-sight velocity (histogram 2), with the reference's
-outlier rejection, summed in body order as the reference does.
+a numpy restatement of nbCreateHistogram's per-bin histograms 1-5 (beta and
+line-of-sight velocity dispersions; average line-of-sight velocity, beta and
+distance) with the reference's outlier rejection, summed in body order as
+the reference does.
 
   * binning and (lambda, beta): nbody/src/nbody_histogram.c:858-903, through
     the host lambda_beta (milkywayathome_client_amd/histogram.py, pinned by
     tests/test_histogram.py);
-  * calc_vLOS: nbody/src/nbody_mass.c:171-179;
+  * calc_vLOS and calc_distance: nbody/src/nbody_mass.c:171-187;
   * per-bin sums: nbody/src/nbody_histogram.c:905-917 (np.add.at adds in
     index order, i.e. body order);
   * nbCalcDisp: nbody/src/nbody_mass.c:195-245 (only bins with more than 10
     bodies change; the correction factor on every call after the first);
   * nbRemoveOutliers: nbody/src/nbody_mass.c:247-316 (mean and dispersion of
     the previous round, keep |mean - value| < sigma * dispersion);
-  * the rounds: nbody/src/nbody_histogram.c:964-999.
+  * the rounds: nbody/src/nbody_histogram.c:964-999; the averages and their
+    errors (dispersion / sqrt(n), -1 and 0 below 11 bodies): :996-1070.
 
 Pinned by closed forms in tests/test_dispersion_ref.py (symmetric bins,
 a planted outlier, bins of 10 or fewer bodies, the line-of-sight velocity of
@@ -24,8 +26,9 @@ import numpy as np
 from milkywayathome_client_amd import histogram as H
 
 
-def body_values(pos, vel, hp: H.HistogramParams, ignore=None):
-    """Bin index (-1: ignored or outside), beta (degrees), v_los per body."""
+def body_values(pos, vel, hp: H.HistogramParams, ignore=None, distance=False):
+    """Bin index (-1: ignored or outside), beta (degrees), v_los per body
+    (and the heliocentric distance with distance=True)."""
     pos = np.asarray(pos, dtype=np.float64)
     vel = np.asarray(vel, dtype=np.float64)
     lam, beta = H.lambda_beta(pos, hp)
@@ -40,6 +43,8 @@ def body_values(pos, vel, hp: H.HistogramParams, ignore=None):
     xs = pos[:, 0] + hp.sunGCDist
     mag = np.sqrt(xs * xs + pos[:, 1] ** 2 + pos[:, 2] ** 2)
     vlos = (xs * vel[:, 0] + pos[:, 1] * vel[:, 1] + pos[:, 2] * vel[:, 2]) / mag
+    if distance:
+        return b, beta, vlos, mag
     return b, beta, vlos
 
 
@@ -52,7 +57,10 @@ def _calc_disp(sum_, sq, removed, raw, initial, correction, var, err):
             dispsq = (sq[i] / n_new) - n_ratio * (sum_[i] / count) ** 2
             if not initial:
                 dispsq *= correction
-            var[i] = np.sqrt(dispsq)
+            # a spread far below the mean can cancel to a tiny negative: NaN,
+            # as the reference's sqrt gives
+            with np.errstate(invalid="ignore"):
+                var[i] = np.sqrt(dispsq)
             err[i] = np.sqrt((count + 1) / (count * n_new)) * var[i]
 
 
@@ -67,8 +75,9 @@ def _sums(b, v, nbin, keep):
     return s, q, r
 
 
-def dispersion(b, v, nbin, iter_max, sigma, correction):
-    """(dispersion, error) per bin for the values v of binned bodies b."""
+def dispersion(b, v, nbin, iter_max, sigma, correction, average=False):
+    """(dispersion, error) per bin for the values v of binned bodies b; with
+    average=True the average and its error instead."""
     raw = np.bincount(b[b >= 0], minlength=nbin).astype(np.float64)
     var = np.zeros(nbin)
     err = np.zeros(nbin)
@@ -81,14 +90,24 @@ def dispersion(b, v, nbin, iter_max, sigma, correction):
         keep = np.abs(ave[bb] - v) < sigma * var[bb]
         s, q, r = _sums(b, v, nbin, keep)
         _calc_disp(s, q, r, raw, False, correction, var, err)
+    if average:
+        denom = (raw - r).astype(np.int64)
+        big = denom > 10
+        avg = np.where(big, s / np.where(big, denom, 1), 0.0)
+        aerr = np.where(big, var / np.sqrt(np.where(big, denom, 1)), -1.0)
+        return avg, aerr
     return var, err
 
 
 def dispersions(pos, vel, hp, ignore=None, iter_max=6, beta_sigma=2.5, vel_sigma=2.5,
-                beta_correct=1.111, vel_correct=1.111):
-    """(beta_disp, beta_err, vlos_disp, vlos_err) as nbhip_histogram_dispersion."""
-    b, beta, vlos = body_values(pos, vel, hp, ignore)
+                dist_sigma=2.5, beta_correct=1.111, vel_correct=1.111, dist_correct=1.111):
+    """{1: beta dispersion, 2: v_los dispersion, 3: v_los average, 4: beta
+    average, 5: distance average}, each (value, error) per bin, as
+    nbhip_histogram_dispersion returns them."""
+    b, beta, vlos, dist = body_values(pos, vel, hp, ignore, distance=True)
     nbin = hp.lambdaBins * hp.betaBins
-    bd, be = dispersion(b, beta, nbin, iter_max, beta_sigma, beta_correct)
-    vd, ve = dispersion(b, vlos, nbin, iter_max, vel_sigma, vel_correct)
-    return bd, be, vd, ve
+    return {1: dispersion(b, beta, nbin, iter_max, beta_sigma, beta_correct),
+            2: dispersion(b, vlos, nbin, iter_max, vel_sigma, vel_correct),
+            3: dispersion(b, vlos, nbin, iter_max, vel_sigma, vel_correct, average=True),
+            4: dispersion(b, beta, nbin, iter_max, beta_sigma, beta_correct, average=True),
+            5: dispersion(b, dist, nbin, iter_max, dist_sigma, dist_correct, average=True)}

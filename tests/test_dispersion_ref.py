@@ -52,7 +52,7 @@ def _bodies_at(hp, lam_deg, beta_deg, dist=20.0):
     bet = np.radians(np.asarray(beta_deg, dtype=float))
     # the frame's third axis is minus the beta direction (beta = asin(-z'/r))
     u = np.stack([np.cos(bet) * np.cos(lam), np.cos(bet) * np.sin(lam), -np.sin(bet)], 1)
-    helio = dist * u @ R_
+    helio = (np.reshape(np.asarray(dist, dtype=float), (-1, 1)) * u) @ R_
     return helio - np.array([hp.sunGCDist, 0.0, 0.0])
 
 
@@ -68,9 +68,11 @@ def test_symmetric_bin_closed_form():
     hp = H.HistogramParams(betaStart=-15.0, betaEnd=15.0, betaBins=1)
     n, a = 40, 1.5
     beta = np.where(np.arange(n) % 2 == 0, a, -a)
-    pos = _bodies_at(hp, np.full(n, 3.1), beta)
+    dist = np.where(np.arange(n) % 4 < 2, 19.0, 21.0)        # mean 20, spread 1
+    pos = _bodies_at(hp, np.full(n, 3.1), beta, dist=dist)
     vel = np.zeros_like(pos)
-    bd, be, vd, ve = R.dispersions(pos, vel, hp)
+    d = R.dispersions(pos, vel, hp)
+    (bd, be), (vd, ve) = d[1], d[2]
     k = int(np.floor((3.1 - hp.lambdaStart) / 6.0))
     # no body is rejected: the variance of +-a with n - 1 weighting, times the
     # correction of the later rounds
@@ -80,6 +82,13 @@ def test_symmetric_bin_closed_form():
     others = np.arange(bd.size) != k
     assert np.all(bd[others] == 0.0) and np.all(be[others] == 0.0)
     assert vd[k] == 0.0                      # every v_los is 0
+    # the averages: beta 0 (symmetric), error dispersion / sqrt(n); distance 20
+    assert d[4][0][k] == pytest.approx(0.0, abs=1e-12)
+    assert d[4][1][k] == pytest.approx(disp / math.sqrt(n), rel=1e-12)
+    assert d[5][0][k] == pytest.approx(20.0, rel=1e-13)
+    assert d[5][1][k] == pytest.approx(math.sqrt(n / (n - 1.0) * 1.111) / math.sqrt(n),
+                                       rel=1e-10)
+    assert np.all(d[3][1][others] == -1.0) and np.all(d[3][0][others] == 0.0)
 
 
 def test_planted_outlier_is_rejected():
@@ -87,22 +96,37 @@ def test_planted_outlier_is_rejected():
     beta = np.concatenate([np.where(np.arange(40) % 2 == 0, 1.0, -1.0), [12.0]])
     pos = _bodies_at(hp, np.full(41, -60.5), beta)
     vel = np.zeros_like(pos)
-    bd, be, _, _ = R.dispersions(pos, vel, hp)
+    d = R.dispersions(pos, vel, hp)
+    bd = d[1][0]
     k = int(np.floor((-60.5 - hp.lambdaStart) / 6.0))
     # round 0 sees the outlier: mean 12/41, dispersion sqrt(184/40 - 41/40
     # (12/41)^2) = 2.12; 12 is 11.7 > 2.5 * 2.12 from the mean, so every later
     # round leaves it out and keeps the 40 bodies at +-1
     assert bd[k] == pytest.approx(math.sqrt(40.0 / 39.0 * 1.111), rel=1e-12)
+    assert d[4][0][k] == pytest.approx(0.0, abs=1e-12)          # mean without the outlier
     b, vals, _ = R.body_values(pos, vel, hp)
     v0, _ = R.dispersion(b, vals, 50, 0, 2.5, 1.111)       # no rejection round
     assert v0[k] > 2.0
 
 
+def test_zero_spread_rejects_every_body():
+    """A quantity with no spread in a bin has dispersion 0, so the first
+    rejection round (|mean - v| < sigma * 0) leaves every body out and the
+    average reports 0 with error -1, as the reference's loop does."""
+    hp = H.HistogramParams()
+    pos = _bodies_at(hp, np.full(30, 20.0), np.linspace(-2, 2, 30), dist=20.0)
+    d = R.dispersions(pos, np.zeros_like(pos), hp)
+    k = int(np.floor((20.0 - hp.lambdaStart) / 6.0))
+    assert d[5][0][k] == 0.0 and d[5][1][k] == -1.0      # distances all equal
+    assert d[4][1][k] > 0.0                              # beta has a spread
+
+
 def test_small_bins_keep_zero():
     hp = H.HistogramParams()
     pos = _bodies_at(hp, np.full(10, 40.0), np.linspace(-3, 3, 10))
-    bd, be, _, _ = R.dispersions(pos, np.zeros_like(pos), hp)
-    assert np.all(bd == 0.0) and np.all(be == 0.0)
+    d = R.dispersions(pos, np.zeros_like(pos), hp)
+    assert np.all(d[1][0] == 0.0) and np.all(d[1][1] == 0.0)
+    assert np.all(d[4][0] == 0.0) and np.all(d[4][1] == -1.0)
 
 
 def test_line_of_sight_velocity_of_radial_motion():
@@ -125,8 +149,9 @@ def test_likelihood_with_dispersions_against_itself_is_zero():
     counts = np.bincount(R.body_values(pos, vel, hp)[0].clip(0), minlength=50)
     disp = R.dispersions(pos, vel, hp)
     sim = L.from_counts(counts, hp, 30000, 1e-3, dispersions=disp)
-    out = L.system_likelihood(sim, sim, 30000, use_beta_disp=True, use_vel_disp=True)
-    assert out[0] == 0.0 and out[3] == 0.0 and out[4] == 0.0
+    out = L.system_likelihood(sim, sim, 30000, use_beta_disp=True, use_vel_disp=True,
+                              use_vlos=True, use_beta_comp=True, use_dist=True)
+    assert out[0] == 0.0 and np.all(out[3:8] == 0.0) and np.all(np.isnan(out[8:]))
     # a dwarf with a wider beta spread scores a positive beta-dispersion term
     wide = L.from_counts(counts, hp, 30000, 1e-3,
                          dispersions=R.dispersions(_bodies_at(hp, lam, beta * 1.5), vel, hp))

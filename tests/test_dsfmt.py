@@ -8,7 +8,8 @@ import pytest
 
 from milkywayathome_client_amd.models import DSFMT
 
-KAT = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "dsfmt_19937_kat.json")))
+with open(os.path.join(os.path.dirname(__file__), "golden", "dsfmt_19937_kat.json")) as _f:
+    KAT = json.load(_f)
 
 
 @pytest.mark.parametrize("block", KAT["blocks"], ids=lambda b: f"seed{b['seed']}-{b['kind']}")

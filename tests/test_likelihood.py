@@ -125,9 +125,10 @@ def test_worst_case_and_mismatch():
     assert out[0] == out[1] == L.DEFAULT_WORST_CASE and np.all(out[2:] == 0.0)
     assert np.all(np.isnan(L.system_likelihood(data, _hist([1.0], [0.0], 50, 100, 1.0), 100)))
     with pytest.raises(NotImplementedError):
-        L.system_likelihood(data, data, 100, use_vlos=True)
-    with pytest.raises(ValueError):                      # no dispersion columns
-        L.system_likelihood(data, data, 100, use_vel_disp=True)
+        L.system_likelihood(data, data, 100, use_prop_mot=True)
+    for flag in ("use_beta_disp", "use_vel_disp", "use_vlos", "use_beta_comp", "use_dist"):
+        with pytest.raises(ValueError):                  # no per-bin columns
+            L.system_likelihood(data, data, 100, **{flag: True})
     # an EMD above 50 (the maximum distance) is NaN
     far = _hist([1.0, 0.0], [0.0, 0.0], 100, 100, 1.0)
     near = _hist([0.0, 1.0], [0.0, 0.0], 100, 100, 1.0)
