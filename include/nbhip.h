@@ -240,22 +240,28 @@ unsigned nbhip_set_ignore(nbhip_state* st, const unsigned char* ignore);
 unsigned nbhip_histogram(nbhip_state* st, const nbhip_histogram_params* hp, unsigned* counts,
                          unsigned* totals);
 
-/* The outlier-rejection settings of the per-bin histograms (NBodyCtx's
- * IterMax, BetaSigma, VelSigma, DistSigma, BetaCorrect, VelCorrect,
- * DistCorrect; nbody/src/nbody_lua_types/nbody_lua_nbodyctx.c:127-135). */
+/* The settings of the per-bin histograms: outlier rejection (NBodyCtx's
+ * IterMax, BetaSigma, VelSigma, DistSigma, PMSigma, BetaCorrect, VelCorrect,
+ * DistCorrect, PMCorrect; nbody/src/nbody_lua_types/nbody_lua_nbodyctx.c:
+ * 127-137) and the proper-motion frame (sunVelx/y/z in the model's velocity
+ * units, NGPdec, NGPra, lNCP in radians; nbody/include/nbody_defaults.h:40-45). */
 typedef struct nbhip_disp_params {
     int iter_max;
-    double beta_sigma, vel_sigma, dist_sigma;
-    double beta_correct, vel_correct, dist_correct;
+    double beta_sigma, vel_sigma, dist_sigma, pm_sigma;
+    double beta_correct, vel_correct, dist_correct, pm_correct;
+    double sun_vel[3];
+    double ngp_dec, ngp_ra, l_ncp;
 } nbhip_disp_params;
 
-/* Histograms 1-5 of nbCreateHistogram (nbody/src/nbody_histogram.c:858-1070)
+/* Histograms 1-7 of nbCreateHistogram (nbody/src/nbody_histogram.c:858-1110)
  * for simulation `sim` on the current state, after nbCalcDisp and iter_max
  * rounds of nbRemoveOutliers (nbody/src/nbody_mass.c:195-316): 1 the beta
  * dispersion, 2 the line-of-sight velocity dispersion (calc_vLOS,
  * nbody/src/nbody_mass.c:171-179), 3 the average line-of-sight velocity,
  * 4 the average beta, 5 the average heliocentric distance (calc_distance,
- * :181-187).  out holds 10 * lambda_bins * beta_bins doubles:
+ * :181-187), 6 and 7 the average proper motions in declination and right
+ * ascension (nbVXVYVZtomuDec / RA, nbody/src/nbody_coordinates.c:226-287).
+ * out holds 14 * lambda_bins * beta_bins doubles:
  * out[(2 (k - 1) + 0) * bins + b] the value of histogram k in bin b
  * (lambda-major), out[(2 (k - 1) + 1) * bins + b] its error (-1 for an
  * average over 10 or fewer bodies).  Ignored bodies are left out as in

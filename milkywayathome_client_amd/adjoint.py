@@ -311,7 +311,7 @@ class StreamModel:
         pt = torch.tensor(np.asarray(p, dtype=np.float64), requires_grad=True)
         L = self.loss(pt, target, grad=True)
         L.backward()
-        return float(L), pt.grad.numpy().copy()
+        return float(L.detach()), pt.grad.numpy().copy()
 
 
 def make_model(n: int = 20000, p_ref=(1.0, 1.0, 0.2, 0.2, 12.0, 0.2), seed: int = 34086709,

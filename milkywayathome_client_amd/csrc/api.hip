@@ -1125,7 +1125,7 @@ extern "C" unsigned nbhip_histogram_dispersion(nbhip_state* st, int sim,
     const size_t nbin = (size_t) hp->lambda_bins * (size_t) hp->beta_bins;
     if (cnt <= 0) {
         /* no bodies: every dispersion 0, every average 0 with error -1 */
-        for (size_t k = 0; k < 10; ++k)
+        for (size_t k = 0; k < 14; ++k)
             for (size_t b = 0; b < nbin; ++b) out[k * nbin + b] = (k >= 4 && k % 2) ? -1.0 : 0.0;
         return NBHIP_SUCCESS;
     }
@@ -1133,16 +1133,16 @@ extern "C" unsigned nbhip_histogram_dispersion(nbhip_state* st, int sim,
     int* dbin = nullptr;
     double *dval = nullptr, *part = nullptr, *state = nullptr, *dout = nullptr;
     hipError_t e = nbh_alloc(&dbin, (size_t) cnt);
-    if (e == hipSuccess) e = nbh_alloc(&dval, 3 * (size_t) cnt);
-    if (e == hipSuccess) e = nbh_alloc(&part, 10 * nblk * nbin);
-    if (e == hipSuccess) e = nbh_alloc(&state, 20 * nbin);
-    if (e == hipSuccess) e = nbh_alloc(&dout, 10 * nbin);
+    if (e == hipSuccess) e = nbh_alloc(&dval, 5 * (size_t) cnt);
+    if (e == hipSuccess) e = nbh_alloc(&part, 16 * nblk * nbin);
+    if (e == hipSuccess) e = nbh_alloc(&state, 32 * nbin);
+    if (e == hipSuccess) e = nbh_alloc(&dout, 14 * nbin);
     if (e == hipSuccess) {
         nbh_dispersion(st, off, cnt, hp, dp, dbin, dval, part, state, dout);
         e = hipGetLastError();
     }
     if (e == hipSuccess)
-        e = hipMemcpyAsync(out, dout, 10 * nbin * sizeof(double), hipMemcpyDeviceToHost,
+        e = hipMemcpyAsync(out, dout, 14 * nbin * sizeof(double), hipMemcpyDeviceToHost,
                            st->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(st->stream);
     for (void* p : { (void*) dbin, (void*) dval, (void*) part, (void*) state, (void*) dout })

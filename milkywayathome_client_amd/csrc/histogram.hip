@@ -94,14 +94,16 @@ __global__ __launch_bounds__(256) void k_histogram(int n, const double* __restri
 /* ---------------------------------------------------------------------- */
 /* per-bin dispersions and averages of beta, v_los and distance            */
 /* ---------------------------------------------------------------------- */
-/* nbCreateHistogram's histograms 1-5 (nbody/src/nbody_histogram.c:858-1070):
+/* nbCreateHistogram's histograms 1-7 (nbody/src/nbody_histogram.c:858-1110):
  * the beta and line-of-sight velocity dispersions, the average line-of-sight
- * velocity, beta and distance, each after nbCalcDisp and the rounds of
- * nbRemoveOutliers (nbody/src/nbody_mass.c:195-316; calc_vLOS and
- * calc_distance :171-187), for the bodies of one simulation.  The averages go
- * through the same rounds as the dispersions of their quantity (histograms 3
- * and 2 see the same values, sigma and correction; so do 4 and 1), so three
- * quantities suffice: q = 0 beta, 1 v_los, 2 distance.
+ * velocity, beta, distance and proper motions in declination and right
+ * ascension, each after nbCalcDisp and the rounds of nbRemoveOutliers
+ * (nbody/src/nbody_mass.c:195-316; calc_vLOS and calc_distance :171-187;
+ * nbVXVYVZtomuDec / RA, nbody/src/nbody_coordinates.c:192-287), for the
+ * bodies of one simulation.  The averages go through the same rounds as the
+ * dispersions of their quantity (histograms 3 and 2 see the same values,
+ * sigma and correction; so do 4 and 1), so five quantities suffice: q = 0
+ * beta, 1 v_los, 2 distance, 3 mu_dec, 4 mu_ra.
  *
  * k_disp_values records each body's bin (-1: ignored or outside) and the
  * three values once.  Each round of sums is k_disp_part, one workgroup per
@@ -113,14 +115,55 @@ __global__ __launch_bounds__(256) void k_histogram(int n, const double* __restri
  * Round 0 takes every binned body; each later round keeps a body when
  * |mean - value| < sigma * dispersion of the previous round. */
 #define NBH_DISP_TILE 1024
-#define NBH_DISP_Q 3
+#define NBH_DISP_Q 5
 #define NBH_DISP_W (3 * NBH_DISP_Q + 1)   /* per q: sum, sq, removed; then count */
-#define NBH_DISP_S 20                     /* per q: sum sq removed var err -; [18] raw count */
+#define NBH_DISP_S 32                     /* per q: sum sq removed var err -; [30] raw count */
+#define NBH_DISP_RAW 30
 
 struct DispRound {
     double sigma[NBH_DISP_Q];
     double correct[NBH_DISP_Q];
 };
+
+/* The proper-motion frame: the Galactic -> equatorial rotation of
+ * cartesianalign (nbody/src/nbody_coordinates.c:192-224) and the Sun's
+ * velocity. */
+struct PMFrame {
+    double rot[3][3];
+    double sun_vel[3];
+};
+
+/* nbVXVYVZtomuDec and nbVXVYVZtomuRA (nbody/src/nbody_coordinates.c:226-287):
+ * heliocentric position and velocity relative to the Sun rotated into the
+ * equatorial frame, the velocity's components along the unit vectors of
+ * increasing latitude and longitude over the distance, in milliarcseconds
+ * per year (degrees * 3600 / 10^6 of the model's units). */
+__device__ __forceinline__ void nbh_proper_motion(const PMFrame& f, double sun, double x,
+                                                  double y, double z, double vx, double vy,
+                                                  double vz, double& mudec, double& mura)
+{
+    const double px = x + sun, py = y, pz = z;
+    const double ux = vx - f.sun_vel[0], uy = vy - f.sun_vel[1], uz = vz - f.sun_vel[2];
+    const double X = f.rot[0][0] * px + f.rot[0][1] * py + f.rot[0][2] * pz;
+    const double Y = f.rot[1][0] * px + f.rot[1][1] * py + f.rot[1][2] * pz;
+    const double Z = f.rot[2][0] * px + f.rot[2][1] * py + f.rot[2][2] * pz;
+    const double VX = f.rot[0][0] * ux + f.rot[0][1] * uy + f.rot[0][2] * uz;
+    const double VY = f.rot[1][0] * ux + f.rot[1][1] * uy + f.rot[1][2] * uz;
+    const double VZ = f.rot[2][0] * ux + f.rot[2][1] * uy + f.rot[2][2] * uz;
+    double l = atan2(Y, X);
+    const double b = atan2(Z, sqrt(X * X + Y * Y));
+    if (l < 0.0) l += 2.0 * M_PI;
+    /* findLatUnitVec goes through degrees: b' = 90 - b (degrees) */
+    const double bdeg = b * 180.0 / M_PI;
+    const double b2 = (90.0 - bdeg) * M_PI / 180.0;
+    const double edx = -cos(l) * cos(b2), edy = -cos(b2) * sin(l), edz = sin(b2);
+    const double erx = -sin(l), ery = cos(l);
+    const double dist = sqrt(X * X + Y * Y + Z * Z);
+    const double cdec = VX * edx + VY * edy + VZ * edz;
+    const double cra = VX * erx + VY * ery;   /* e_l has no z component */
+    mudec = (cdec / dist) * 180.0 / M_PI * 3600.0 / 1000000.0;
+    mura = (cra / dist) * 180.0 / M_PI * 3600.0 / 1000000.0;
+}
 
 __global__ __launch_bounds__(256) void k_disp_values(int off, int cnt,
                                                      const double* __restrict__ px,
@@ -130,14 +173,15 @@ __global__ __launch_bounds__(256) void k_disp_values(int off, int cnt,
                                                      const double* __restrict__ vy,
                                                      const double* __restrict__ vz,
                                                      const unsigned char* __restrict__ ignore,
-                                                     HistParams h, int* __restrict__ dbin,
+                                                     HistParams h, PMFrame f,
+                                                     int* __restrict__ dbin,
                                                      double* __restrict__ dval)
 {
     const int k = blockIdx.x * 256 + threadIdx.x;
     if (k >= cnt) return;
     const int i = off + k;
     int bin = -1;
-    double beta = 0.0, vl = 0.0, dist = 0.0;
+    double beta = 0.0, vl = 0.0, dist = 0.0, mudec = 0.0, mura = 0.0;
     if (!(ignore && ignore[i])) {
         const double x = px[i], y = py[i], z = pz[i];
         bin = nbh_lambda_beta_bin(h, x, y, z, beta);
@@ -145,11 +189,14 @@ __global__ __launch_bounds__(256) void k_disp_values(int off, int cnt,
         const double mag = sqrt(xs * xs + y * y + z * z);
         vl = (xs * vx[i] + y * vy[i] + z * vz[i]) / mag;
         dist = mag;
+        nbh_proper_motion(f, h.sun, x, y, z, vx[i], vy[i], vz[i], mudec, mura);
     }
     dbin[k] = bin;
     dval[k] = beta;
     dval[(size_t) cnt + k] = vl;
     dval[2 * (size_t) cnt + k] = dist;
+    dval[3 * (size_t) cnt + k] = mudec;
+    dval[4 * (size_t) cnt + k] = mura;
 }
 
 __global__ __launch_bounds__(256) void k_disp_part(int cnt, int nbin, int initial, DispRound r,
@@ -172,7 +219,7 @@ __global__ __launch_bounds__(256) void k_disp_part(int cnt, int nbin, int initia
         /* previous round: mean = sum / (count - removed), sigma = dispersion */
         double ave[NBH_DISP_Q], cut[NBH_DISP_Q];
         for (int q = 0; q < NBH_DISP_Q; ++q) {
-            ave[q] = initial ? 0.0 : st[6 * q] / (st[18] - st[6 * q + 2]);
+            ave[q] = initial ? 0.0 : st[6 * q] / (st[NBH_DISP_RAW] - st[6 * q + 2]);
             cut[q] = initial ? 0.0 : r.sigma[q] * st[6 * q + 3];
         }
         double acc[NBH_DISP_W];
@@ -226,7 +273,7 @@ __global__ __launch_bounds__(256) void k_disp_finish(int nbin, int nblk, int ini
         for (int w = 0; w < NBH_DISP_W; ++w) t[w] += o[w];
     }
     double* st = state + (size_t) b * NBH_DISP_S;
-    st[18] = t[NBH_DISP_W - 1];
+    st[NBH_DISP_RAW] = t[NBH_DISP_W - 1];
     for (int q = 0; q < NBH_DISP_Q; ++q) {
         double* sq = st + 6 * q;
         /* every bin starts at dispersion 0 (nbody_histogram.c:846-854) */
@@ -234,14 +281,16 @@ __global__ __launch_bounds__(256) void k_disp_finish(int nbin, int nblk, int ini
         sq[0] = t[3 * q];
         sq[1] = t[3 * q + 1];
         sq[2] = t[3 * q + 2];
-        nbh_calc_disp(sq[0], sq[1], sq[2], st[18], initial, r.correct[q], sq[3], sq[4]);
+        nbh_calc_disp(sq[0], sq[1], sq[2], st[NBH_DISP_RAW], initial, r.correct[q], sq[3],
+                      sq[4]);
     }
 }
 
-/* Histograms 1-5 from the final state: dispersions of beta and v_los, and
- * the averages of v_los, beta and distance with error dispersion / sqrt(n)
- * (nbody/src/nbody_histogram.c:996-1070; -1 and 0 below 11 bodies).
- * out[(2 (k - 1) + {0: value, 1: error}) * nbin + b] for histogram k. */
+/* Histograms 1-7 from the final state: dispersions of beta and v_los, and
+ * the averages of v_los, beta, distance, mu_dec and mu_ra with error
+ * dispersion / sqrt(n) (nbody/src/nbody_histogram.c:996-1110; -1 and 0 below
+ * 11 bodies).  out[(2 (k - 1) + {0: value, 1: error}) * nbin + b] for
+ * histogram k. */
 __global__ __launch_bounds__(256) void k_disp_out(int nbin, const double* __restrict__ state,
                                                   double* __restrict__ out)
 {
@@ -253,10 +302,11 @@ __global__ __launch_bounds__(256) void k_disp_out(int nbin, const double* __rest
     out[1 * N + b] = st[4];
     out[2 * N + b] = st[6 + 3];                     /* 2: v_los dispersion */
     out[3 * N + b] = st[6 + 4];
-    const int avg_q[3] = { 1, 0, 2 };               /* 3: v_los, 4: beta, 5: distance */
-    for (int k = 0; k < 3; ++k) {
+    /* 3: v_los, 4: beta, 5: distance, 6: mu_dec, 7: mu_ra */
+    const int avg_q[5] = { 1, 0, 2, 3, 4 };
+    for (int k = 0; k < 5; ++k) {
         const double* sq = st + 6 * avg_q[k];
-        const int denom = (int) (st[18] - sq[2]);
+        const int denom = (int) (st[NBH_DISP_RAW] - sq[2]);
         double v = 0.0, e = -1.0;
         if (denom > 10) {
             e = sq[3] / sqrt((double) denom);
@@ -282,12 +332,23 @@ void nbh_dispersion(nbhip_state* st, int off, int cnt, const nbhip_histogram_par
     r.sigma[0] = dp->beta_sigma;
     r.sigma[1] = dp->vel_sigma;
     r.sigma[2] = dp->dist_sigma;
+    r.sigma[3] = r.sigma[4] = dp->pm_sigma;
     r.correct[0] = dp->beta_correct;
     r.correct[1] = dp->vel_correct;
     r.correct[2] = dp->dist_correct;
+    r.correct[3] = r.correct[4] = dp->pm_correct;
+    /* cartesianalign's rotation (nbody/src/nbody_coordinates.c:192-224) */
+    PMFrame f;
+    const double cl = cos(M_PI - dp->l_ncp), sl = sin(M_PI - dp->l_ncp);
+    const double cd = cos(M_PI_2 - dp->ngp_dec), sd = sin(M_PI_2 - dp->ngp_dec);
+    const double ca = cos(dp->ngp_ra), sa = sin(dp->ngp_ra);
+    f.rot[0][0] = ca * cd * cl - sa * sl; f.rot[0][1] = -ca * cd * sl - sa * cl; f.rot[0][2] = ca * sd;
+    f.rot[1][0] = sa * cd * cl + ca * sl; f.rot[1][1] = -sa * cd * sl + ca * cl; f.rot[1][2] = sa * sd;
+    f.rot[2][0] = -sd * cl; f.rot[2][1] = sd * sl; f.rot[2][2] = cd;
+    for (int k = 0; k < 3; ++k) f.sun_vel[k] = dp->sun_vel[k];
     hipStream_t s = st->stream;
     hipLaunchKernelGGL(k_disp_values, dim3(nbh_blocks(cnt, 256)), dim3(256), 0, s, off, cnt,
-                       st->px, st->py, st->pz, st->vx, st->vy, st->vz, st->ignore, h, dbin,
+                       st->px, st->py, st->pz, st->vx, st->vy, st->vz, st->ignore, h, f, dbin,
                        dval);
     for (int round = 0; round <= dp->iter_max; ++round) {
         const int initial = round == 0;

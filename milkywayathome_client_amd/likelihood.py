@@ -19,10 +19,9 @@ the output layer that consumes the GPU histogram (nbhip_histogram).
   mass and dispersion terms, with the reference's NaN / infinity conventions.
 
 The per-bin histograms of a simulation (beta and line-of-sight velocity
-dispersions, average line-of-sight velocity, beta and distance) come from
-the GPU (nbhip_histogram_dispersion, with the reference's outlier
-rejection).  The proper-motion components are not computed; asking for them
-raises NotImplementedError.
+dispersions, average line-of-sight velocity, beta, distance and proper
+motions) come from the GPU (nbhip_histogram_dispersion, with the reference's
+outlier rejection).
 """
 from __future__ import annotations
 
@@ -108,9 +107,10 @@ def read_histogram(path: str) -> Histogram:
     lb = int(hdr["lambdaBins"])
     bb = int(hdr["betaBins"])
     ncol = (19 if int(hdr.get("hasPM", "0")) == 1 else 15) if orbit_cols else 9
-    if len(rows) != lb * bb or any(len(r) != ncol for r in rows):
+    # like the reference's sscanf, columns beyond the format's are ignored
+    if len(rows) != lb * bb or any(len(r) < ncol for r in rows):
         raise ValueError(f"{path}: expected {lb * bb} rows of {ncol} columns")
-    a = np.array([[float(x) for x in r] for r in rows])
+    a = np.array([[float(x) for x in r[:ncol]] for r in rows])
     h = Histogram(lambda_bins=lb, beta_bins=bb, total_num=int(hdr.get("n", "0")),
                   total_simulated=int(hdr.get("totalSimulated", "0")),
                   mass_per_particle=float(hdr.get("massPerParticle", "0")),
@@ -130,8 +130,10 @@ def read_histogram(path: str) -> Histogram:
 
 
 def write_histogram(path: str, h: Histogram, comment: str = "") -> None:
-    """nbPrintHistogram's 9-column layout: the counts and their errors, then
-    the beta and velocity dispersion columns (-1 where not computed)."""
+    """nbPrintHistogram (nbody/src/nbody_histogram.c:515-600): the counts and
+    their errors, then (value, error) of histograms 1-2 (9 columns), or of
+    1-7 behind a '!' line when histogram 3 is present (19 columns; 'hasPM = 1'
+    when histogram 6 is); -1 where a histogram was not computed."""
     with open(path, "w") as f:
         for c in comment.splitlines():
             f.write(f"# {c}\n")
@@ -143,9 +145,14 @@ def write_histogram(path: str, h: Histogram, comment: str = "") -> None:
         f.write(f"betaBins = {h.beta_bins}\n")
         if h.beta_disp_bins:
             f.write(f"betaDispBins = {h.beta_disp_bins}\n")
+        orbit = 3 in h.comps
+        if 6 in h.comps:
+            f.write("hasPM = 1\n")
+        if orbit:
+            f.write("!\n")
         for i in range(h.nbin):
             cols = []
-            for k in (1, 2):
+            for k in (range(1, 8) if orbit else (1, 2)):
                 if k in h.comps:
                     cols += [f"{h.comps[k][0][i]:12.15f}", f"{h.comps[k][1][i]:12.15f}"]
                 else:
@@ -170,7 +177,7 @@ def from_counts(counts, hp: HistogramParams, total_simulated: int,
     counts and errors as nbNormalizeHistogram computes them.  Like the
     reference, the error of bin (i, j) is computed from the raw count of bin
     i (identical for one beta bin).  `dispersions` = {k: (value, error)} for
-    histograms 1-5, as NBodyState.histogram_dispersion returns it."""
+    histograms 1-7, as NBodyState.histogram_dispersion returns it."""
     c = np.asarray(counts, dtype=np.int64).reshape(-1)
     lb, bb = hp.lambdaBins, hp.betaBins
     if c.size != lb * bb:
@@ -376,11 +383,10 @@ def system_likelihood(data: Histogram, hist: Histogram, nbody: int, method: str 
     out = np.full(10, math.nan)
     if data.lambda_bins != hist.lambda_bins:
         return out
-    if use_prop_mot:
-        raise NotImplementedError("the proper-motion components are not computed")
     # (flag, histogram, output slot) in the reference's order of addition
     terms = ((use_beta_disp, 1, 3), (use_vel_disp, 2, 4), (use_vlos, 3, 6),
-             (use_beta_comp, 4, 5), (use_dist, 5, 7))
+             (use_beta_comp, 4, 5), (use_dist, 5, 7), (use_prop_mot, 6, 8),
+             (use_prop_mot, 7, 9))
     for flag, k, _ in terms:
         if flag and (k not in data.comps or k not in hist.comps):
             raise ValueError(f"histogram {k} missing from the data or the simulation "

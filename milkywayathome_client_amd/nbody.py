@@ -77,8 +77,17 @@ class _TreeInfo(C.Structure):
 
 class _DispParams(C.Structure):
     _fields_ = [("iter_max", C.c_int), ("beta_sigma", C.c_double), ("vel_sigma", C.c_double),
-                ("dist_sigma", C.c_double), ("beta_correct", C.c_double),
-                ("vel_correct", C.c_double), ("dist_correct", C.c_double)]
+                ("dist_sigma", C.c_double), ("pm_sigma", C.c_double),
+                ("beta_correct", C.c_double), ("vel_correct", C.c_double),
+                ("dist_correct", C.c_double), ("pm_correct", C.c_double),
+                ("sun_vel", C.c_double * 3), ("ngp_dec", C.c_double), ("ngp_ra", C.c_double),
+                ("l_ncp", C.c_double)]
+
+
+# The proper-motion frame's defaults (nbody/include/nbody_defaults.h:40-45):
+# the Sun's velocity in the model's units and the NGP / NCP angles in degrees.
+SUN_VEL = (10.3, 229.2, 6.9)
+NGP_DEC, NGP_RA, L_NCP = 27.4, 192.0, 123.0
 
 
 class _HistParams(C.Structure):
@@ -493,24 +502,31 @@ class NBodyState:
 
     def histogram_dispersion(self, hp, sim: int = 0, iter_max: int = 6, beta_sigma: float = 2.5,
                              vel_sigma: float = 2.5, dist_sigma: float = 2.5,
-                             beta_correct: float = 1.111, vel_correct: float = 1.111,
-                             dist_correct: float = 1.111) -> dict:
-        """nbCreateHistogram's histograms 1-5 for simulation `sim` after the
+                             pm_sigma: float = 2.5, beta_correct: float = 1.111,
+                             vel_correct: float = 1.111, dist_correct: float = 1.111,
+                             pm_correct: float = 1.111, sun_vel=SUN_VEL, ngp_dec: float = NGP_DEC,
+                             ngp_ra: float = NGP_RA, l_ncp: float = L_NCP) -> dict:
+        """nbCreateHistogram's histograms 1-7 for simulation `sim` after the
         outlier rejection: {1: beta dispersion, 2: v_los dispersion, 3: v_los
-        average, 4: beta average, 5: distance average}, each (value, error)
-        per bin.  The defaults are the sample workunit's (SigmaCutoff 2.5,
+        average, 4: beta average, 5: distance average, 6: mu_dec average,
+        7: mu_ra average}, each (value, error) per bin.  The sigmas and
+        corrections default to the sample workunit's (SigmaCutoff 2.5,
         SigmaIter 6, Correction 1.111, nbody/sample_workunits/
-        for_developers.lua:93-95)."""
+        for_developers.lua:93-95); the angles are in degrees."""
         h = _HistParams(hp.phi, hp.theta, hp.psi, hp.lambdaStart, hp.lambdaEnd, hp.betaStart,
                         hp.betaEnd, hp.lambdaBins, hp.betaBins, hp.sunGCDist)
+        d2r = math.pi / 180.0
         d = _DispParams(int(iter_max), float(beta_sigma), float(vel_sigma), float(dist_sigma),
-                        float(beta_correct), float(vel_correct), float(dist_correct))
+                        float(pm_sigma), float(beta_correct), float(vel_correct),
+                        float(dist_correct), float(pm_correct),
+                        (C.c_double * 3)(*[float(v) for v in sun_vel]), ngp_dec * d2r,
+                        ngp_ra * d2r, l_ncp * d2r)
         nbin = hp.lambdaBins * hp.betaBins
-        out = np.empty(10 * nbin)
+        out = np.empty(14 * nbin)
         self._check(self.lib.nbhip_histogram_dispersion(self._h, int(sim), C.byref(h),
                                                         C.byref(d), _ptr(out)))
-        out = out.reshape(10, nbin)
-        return {k: (out[2 * (k - 1)].copy(), out[2 * (k - 1) + 1].copy()) for k in range(1, 6)}
+        out = out.reshape(14, nbin)
+        return {k: (out[2 * (k - 1)].copy(), out[2 * (k - 1) + 1].copy()) for k in range(1, 8)}
 
     def gravity_vjp(self, gacc):
         """(dL/dpos [n,3], dL/dmass [n]) of one force evaluation at the current

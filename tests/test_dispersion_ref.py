@@ -8,7 +8,11 @@
 * the numpy restatement of the dispersion histograms (oracle/histogram_ref.py,
   the checker of nbhip_histogram_dispersion) against closed forms: a bin of
   bodies at beta = +-a, a planted outlier that the rejection removes, bins of
-  ten or fewer bodies, and the line-of-sight velocity of a radial orbit."""
+  ten or fewer bodies, and the line-of-sight velocity of a radial orbit;
+* its proper motions: the frame rotation by where it sends the Galactic and
+  celestial poles (and the Galactic centre), the motions by finite
+  differences of the equatorial angles of bodies moving along straight
+  lines, zero for radial motion relative to the Sun."""
 import math
 import os
 
@@ -156,3 +160,98 @@ def test_likelihood_with_dispersions_against_itself_is_zero():
     wide = L.from_counts(counts, hp, 30000, 1e-3,
                          dispersions=R.dispersions(_bodies_at(hp, lam, beta * 1.5), vel, hp))
     assert L.system_likelihood(sim, wide, 30000, use_beta_disp=True)[3] > 1.0
+
+
+def _unit(lon_deg, lat_deg):
+    lo, la = np.radians(lon_deg), np.radians(lat_deg)
+    return np.array([np.cos(la) * np.cos(lo), np.cos(la) * np.sin(lo), np.sin(la)])
+
+
+def test_frame_rotation_sends_the_poles():
+    """The Galactic -> equatorial rotation is fixed by two directions: the
+    north Galactic pole lands at (RA, Dec) = (NGPra, NGPdec), and the
+    celestial pole sits at Galactic (l, b) = (lNCP, NGPdec)."""
+    for dec, ra, lncp in [(27.4, 192.0, 123.0), (27.13, 192.86, 122.93), (-10.0, 30.0, 200.0)]:
+        rot = R.galactic_to_equatorial(dec, ra, lncp)
+        np.testing.assert_allclose(rot @ rot.T, np.eye(3), atol=1e-15)
+        assert np.linalg.det(rot) == pytest.approx(1.0, abs=1e-14)
+        np.testing.assert_allclose(rot @ np.array([0.0, 0.0, 1.0]), _unit(ra, dec), atol=1e-15)
+        np.testing.assert_allclose(rot @ _unit(lncp, dec), [0.0, 0.0, 1.0], atol=1e-15)
+    # with the default angles the Galactic centre is near its catalogued
+    # B1950 place, RA 265.6 deg, Dec -28.92 deg (the NGP's RA is rounded to 192)
+    gc = R.galactic_to_equatorial() @ np.array([1.0, 0.0, 0.0])
+    assert np.degrees(np.arctan2(gc[1], gc[0])) % 360.0 == pytest.approx(265.6, abs=0.5)
+    assert np.degrees(np.arcsin(gc[2])) == pytest.approx(-28.92, abs=0.1)
+
+
+def _equatorial_angles(helio):
+    P = helio @ R.galactic_to_equatorial().T
+    return (np.arctan2(P[:, 1], P[:, 0]),
+            np.arctan2(P[:, 2], np.hypot(P[:, 0], P[:, 1])))
+
+
+def test_proper_motions_are_angle_rates():
+    """mu_dec = dDec/dt and mu_ra = cos(Dec) dRA/dt of a body moving with its
+    velocity relative to the Sun, in degrees * 3600 / 10^6 per time unit, by
+    central differences of the angles along the straight-line motion."""
+    rng = np.random.default_rng(11)
+    n = 200
+    pos = rng.normal(size=(n, 3)) * 15.0
+    vel = rng.normal(size=(n, 3)) * 150.0
+    mud, mur = R.proper_motions(pos, vel, 8.0)
+    helio = pos + np.array([8.0, 0.0, 0.0])
+    rel = vel - np.array(R.SUN_VEL)
+    h = 1e-6
+    ra1, de1 = _equatorial_angles(helio + h * rel)
+    ra0, de0 = _equatorial_angles(helio - h * rel)
+    _, dec = _equatorial_angles(helio)
+    dra = np.angle(np.exp(1j * (ra1 - ra0)))
+    conv = np.degrees(1.0) * 3600.0 / 1e6
+    np.testing.assert_allclose(mud, (de1 - de0) / (2 * h) * conv, rtol=1e-6, atol=1e-9)
+    np.testing.assert_allclose(mur, np.cos(dec) * dra / (2 * h) * conv, rtol=1e-6, atol=1e-9)
+
+
+def test_radial_motion_has_no_proper_motion():
+    rng = np.random.default_rng(12)
+    pos = rng.normal(size=(50, 3)) * 10.0
+    helio = pos + np.array([8.0, 0.0, 0.0])
+    vel = np.array(R.SUN_VEL) + rng.normal(size=(50, 1)) * 100.0 * helio
+    mud, mur = R.proper_motions(pos, vel, 8.0)
+    scale = np.degrees(1.0) * 3600.0 / 1e6 * 100.0
+    assert np.max(np.abs(mud)) < 1e-12 * scale and np.max(np.abs(mur)) < 1e-12 * scale
+    # and the Sun's own velocity is what is subtracted
+    mud2, mur2 = R.proper_motions(pos, vel - np.array(R.SUN_VEL), 8.0, sun_vel=(0.0, 0.0, 0.0))
+    np.testing.assert_allclose(mud2, mud, atol=1e-12)
+    np.testing.assert_allclose(mur2, mur, atol=1e-12)
+
+
+def test_proper_motion_averages_without_rejection():
+    """With no rejection round the averages of histograms 6 and 7 are the
+    means of the bin's proper motions."""
+    hp = H.HistogramParams()
+    pos = _bodies_at(hp, np.full(30, -20.0), np.linspace(-1, 1, 30), dist=20.0)
+    vel = np.zeros_like(pos)
+    d = R.dispersions(pos, vel, hp, iter_max=0)
+    mud, mur = R.proper_motions(pos, vel, hp.sunGCDist)
+    k = int(np.floor((-20.0 - hp.lambdaStart) / 6.0))
+    assert d[6][0][k] == pytest.approx(np.mean(mud), rel=1e-12)
+    assert d[7][0][k] == pytest.approx(np.mean(mur), rel=1e-12)
+    # a body at rest in the Galactic frame shows the reflex of the Sun's motion
+    assert abs(np.mean(mur)) > 0.1
+
+
+def test_likelihood_with_proper_motions():
+    hp = H.HistogramParams()
+    rng = np.random.default_rng(13)
+    lam = rng.uniform(-100, 100, 30000)
+    beta = rng.normal(0.0, 2.0, 30000)
+    pos = _bodies_at(hp, lam, beta)
+    vel = rng.normal(0.0, 30.0, pos.shape)
+    counts = np.bincount(R.body_values(pos, vel, hp)[0].clip(0), minlength=50)
+    sim = L.from_counts(counts, hp, 30000, 1e-3, dispersions=R.dispersions(pos, vel, hp))
+    out = L.system_likelihood(sim, sim, 30000, use_prop_mot=True)
+    assert out[0] == 0.0 and out[8] == 0.0 and out[9] == 0.0 and np.all(np.isnan(out[3:8]))
+    moved = L.from_counts(counts, hp, 30000, 1e-3,
+                          dispersions=R.dispersions(pos, vel + np.array([0.0, 0.0, 40.0]), hp))
+    out = L.system_likelihood(sim, moved, 30000, use_prop_mot=True)
+    assert out[8] > 0.0 and out[9] > 0.0 and out[0] == out[1] + out[2] + out[8] + out[9]

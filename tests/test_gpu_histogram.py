@@ -95,9 +95,9 @@ def test_likelihood_from_gpu_counts():
 
 
 def test_dispersions_match_host_restatement():
-    """nbhip_histogram_dispersion (histograms 1-5: beta and line-of-sight
-    velocity dispersions, average line-of-sight velocity, beta and distance,
-    with outlier rejection) against the numpy restatement of the reference's
+    """nbhip_histogram_dispersion (histograms 1-7: beta and line-of-sight
+    velocity dispersions, average line-of-sight velocity, beta, distance and
+    proper motions, with outlier rejection) against the numpy restatement of the reference's
     loop (oracle/histogram_ref.py): same bins, same rejections, the sums
     differing only in association (1e-12); for one state with ignored bodies
     and for each member of a batch."""
@@ -114,18 +114,19 @@ def test_dispersions_match_host_restatement():
     st.set_ignore(ignore)
     got = st.histogram_dispersion(hp)
     ref = R.dispersions(pos, vel, hp, ignore=ignore)
-    for k in range(1, 6):
+    for k in range(1, 8):
         for g, r in zip(got[k], ref[k]):
             np.testing.assert_allclose(g, r, rtol=1e-12, atol=1e-12)
     assert np.count_nonzero(got[1][0]) >= 15 and np.count_nonzero(got[2][0]) >= 15
     assert np.count_nonzero(got[5][1] > 0) >= 15
+    assert np.count_nonzero(got[6][1] > 0) >= 15 and np.count_nonzero(got[7][1] > 0) >= 15
     st.close()
     members = [_stream(k, 20 + i, (4.0 * i, 0.0, 0.0)) for i, k in enumerate([3000, 25000, 8000])]
     bt = nb.NBodyBatch(ctx, None, members, prestep=False)
     for k, (p, v, _) in enumerate(members):
         got = bt.histogram_dispersion(hp, sim=k)
         ref = R.dispersions(p, v, hp)
-        for j in range(1, 6):
+        for j in range(1, 8):
             for g, r in zip(got[j], ref[j]):
                 np.testing.assert_allclose(g, r, rtol=1e-12, atol=1e-12)
     bt.close()

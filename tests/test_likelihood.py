@@ -124,9 +124,8 @@ def test_worst_case_and_mismatch():
     out = L.system_likelihood(data, few, nbody=100000)  # 2 < 1e-4 * 1e5
     assert out[0] == out[1] == L.DEFAULT_WORST_CASE and np.all(out[2:] == 0.0)
     assert np.all(np.isnan(L.system_likelihood(data, _hist([1.0], [0.0], 50, 100, 1.0), 100)))
-    with pytest.raises(NotImplementedError):
-        L.system_likelihood(data, data, 100, use_prop_mot=True)
-    for flag in ("use_beta_disp", "use_vel_disp", "use_vlos", "use_beta_comp", "use_dist"):
+    for flag in ("use_beta_disp", "use_vel_disp", "use_vlos", "use_beta_comp", "use_dist",
+                 "use_prop_mot"):
         with pytest.raises(ValueError):                  # no per-bin columns
             L.system_likelihood(data, data, 100, **{flag: True})
     # an EMD above 50 (the maximum distance) is NaN
@@ -162,3 +161,26 @@ def test_from_counts_file_round_trip(tmp_path):
     assert (back.total_num, back.total_simulated) == (h.total_num, h.total_simulated)
     assert back.mass_per_particle == pytest.approx(h.mass_per_particle, rel=1e-12)
     assert L.match_emd(back, h) == 0.0
+
+
+def test_nineteen_column_round_trip(tmp_path):
+    """With histograms 1-7 the file carries 'hasPM = 1', the '!' line and 19
+    columns (nbPrintHistogram); reading it back gives every column, and a
+    reader of the 15-column format ignores the proper-motion columns, as the
+    reference's sscanf does."""
+    rng = np.random.default_rng(6)
+    h = _hist(rng.uniform(size=6), rng.uniform(size=6) * 0.01, 600, 1000, 1e-3)
+    for k in range(1, 8):
+        h.comps[k] = (rng.normal(size=6), rng.uniform(size=6))
+    path = tmp_path / "pm.hist"
+    L.write_histogram(str(path), h)
+    text = path.read_text()
+    assert "hasPM = 1\n!\n" in text
+    back = L.read_histogram(str(path))
+    for k in range(1, 8):
+        np.testing.assert_allclose(back.comps[k][0], h.comps[k][0], rtol=1e-14, atol=1e-15)
+        np.testing.assert_allclose(back.comps[k][1], h.comps[k][1], rtol=1e-14, atol=1e-15)
+    path.write_text(text.replace("hasPM = 1\n", ""))
+    back = L.read_histogram(str(path))
+    assert sorted(back.comps) == [1, 2, 3, 4, 5]
+    np.testing.assert_allclose(back.comps[5][0], h.comps[5][0], rtol=1e-14, atol=1e-15)
