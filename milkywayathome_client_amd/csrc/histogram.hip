@@ -92,7 +92,7 @@ __global__ __launch_bounds__(256) void k_histogram(int n, const double* __restri
 }
 
 /* ---------------------------------------------------------------------- */
-/* per-bin dispersions and averages of beta, v_los and distance            */
+/* per-bin dispersions and averages of beta, v_los, distance, proper motion */
 /* ---------------------------------------------------------------------- */
 /* nbCreateHistogram's histograms 1-7 (nbody/src/nbody_histogram.c:858-1110):
  * the beta and line-of-sight velocity dispersions, the average line-of-sight
