@@ -239,6 +239,7 @@ __global__ __launch_bounds__(BS) void k_walk32(const NodeF* __restrict__ node,
     unsigned metv = ~0u;   /* 0 once the body met its own record */
     unsigned nbb = 0, nbc = 0, nop = 0, steps = 0;
     int q = L.q0;
+    const unsigned long long valid_m = __builtin_amdgcn_ballot_w64(valid);
 
     /* One node step: test, vote, successor (fetched now, used next step),
      * interaction.  The successor lies in [q + 1, qend], so every step
@@ -253,7 +254,12 @@ __global__ __launch_bounds__(BS) void k_walk32(const NodeF* __restrict__ node,
         bool accept, any_open;
         if (COH) {
             const bool active = one_root ? valid : (q >= my_root && q < my_end);
-            any_open = __builtin_amdgcn_ballot_w64(active && !(drSq >= nd.rcrit2)) != 0ull;
+            const unsigned long long act = one_root ? valid_m : __builtin_amdgcn_ballot_w64(active);
+            /* the lanes whose criterion accepts, straight from the comparison's
+             * lane mask (ordered >=, so NaN opens); a ballot of the lane bool
+             * would round-trip it through a vector register */
+            const unsigned long long pass = __builtin_amdgcn_fcmpf(drSq, nd.rcrit2, 3);
+            any_open = (act & ~pass) != 0ull;
             accept = active && !any_open;
         } else {
             const bool active = lp == q;
