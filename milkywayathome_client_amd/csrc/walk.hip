@@ -278,20 +278,20 @@ __global__ __launch_bounds__(BS) void k_walk32(const NodeF* __restrict__ node,
             const float d2 = (EPS0 && nd.body == self) ? 1.0f : drSq + eps2;
             const float rinv = __builtin_amdgcn_rsqf(d2);
             const float rinv2 = rinv * rinv;
-            const float mor3 = nd.m * rinv * rinv2;
             if (QUAD) {
-                /* acc += (mor3 + 2.5 dQd / r^7) d - Qd / r^5; a body's record
-                 * carries Q = 0, which leaves exactly the monopole */
+                /* acc += r^-5 ((m r^2 + 2.5 dQd / r^2) d - Qd); a body's
+                 * record carries Q = 0, which leaves the monopole */
                 const float qx = __builtin_fmaf(nd.q[0], dx, __builtin_fmaf(nd.q[1], dy, nd.q[2] * dz));
                 const float qy = __builtin_fmaf(nd.q[1], dx, __builtin_fmaf(nd.q[3], dy, nd.q[4] * dz));
                 const float qz = __builtin_fmaf(nd.q[2], dx, __builtin_fmaf(nd.q[4], dy, nd.q[5] * dz));
                 const float dQd = __builtin_fmaf(qx, dx, __builtin_fmaf(qy, dy, qz * dz));
                 const float dr5 = rinv2 * rinv2 * rinv;
-                const float s1 = __builtin_fmaf(dr5 * 2.5f, dQd * rinv2, mor3);
-                sx = __builtin_fmaf(s1, dx, __builtin_fmaf(-dr5, qx, sx));
-                sy = __builtin_fmaf(s1, dy, __builtin_fmaf(-dr5, qy, sy));
-                sz = __builtin_fmaf(s1, dz, __builtin_fmaf(-dr5, qz, sz));
+                const float S = __builtin_fmaf(dQd, 2.5f * rinv2, nd.m * d2);
+                sx = __builtin_fmaf(dr5, __builtin_fmaf(S, dx, -qx), sx);
+                sy = __builtin_fmaf(dr5, __builtin_fmaf(S, dy, -qy), sy);
+                sz = __builtin_fmaf(dr5, __builtin_fmaf(S, dz, -qz), sz);
             } else {
+                const float mor3 = nd.m * rinv * rinv2;
                 sx = __builtin_fmaf(mor3, dx, sx);
                 sy = __builtin_fmaf(mor3, dy, sy);
                 sz = __builtin_fmaf(mor3, dz, sz);
