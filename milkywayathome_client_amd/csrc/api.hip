@@ -98,7 +98,7 @@ static void nbh_free_all(nbhip_state* st)
                      st->simid_s, st->su,  st->dadj,  st->cincl,   st->scan_part, st->blo,
                      st->bhi,  st->bdelta, st->bparent, st->node,  st->nquad,  st->nmid,
                      st->nlevel, st->nparent, st->nsim_node, st->level_list, st->nodef,
-                     st->scal, st->sims, st->ignore, st->hist_buf, st->pos4,
+                     st->scal, st->sims, st->ignore, st->hist_buf, st->disp_buf, st->pos4,
                      st->exact_part };
     for (void* p : ptrs)
         if (p) (void) hipFree(p);
@@ -1130,23 +1130,40 @@ extern "C" unsigned nbhip_histogram_dispersion(nbhip_state* st, int sim,
         return NBHIP_SUCCESS;
     }
     const size_t nblk = ((size_t) cnt + 1023) / 1024;
-    int* dbin = nullptr;
-    double *dval = nullptr, *part = nullptr, *state = nullptr, *dout = nullptr;
-    hipError_t e = nbh_alloc(&dbin, (size_t) cnt);
-    if (e == hipSuccess) e = nbh_alloc(&dval, 5 * (size_t) cnt);
-    if (e == hipSuccess) e = nbh_alloc(&part, 16 * nblk * nbin);
-    if (e == hipSuccess) e = nbh_alloc(&state, 32 * nbin);
-    if (e == hipSuccess) e = nbh_alloc(&dout, 14 * nbin);
-    if (e == hipSuccess) {
-        nbh_dispersion(st, off, cnt, hp, dp, dbin, dval, part, state, dout);
-        e = hipGetLastError();
+    /* one scratch block, kept by the state and grown on demand: body bins,
+     * five values per body, per-workgroup partial sums, per-bin state and
+     * the 14 output rows, each section 256-byte aligned */
+    auto up = [](size_t bytes) { return (bytes + 255) & ~(size_t) 255; };
+    const size_t o_val = up((size_t) cnt * sizeof(int));
+    const size_t o_part = o_val + up(5 * (size_t) cnt * sizeof(double));
+    const size_t n_part = 16 * nblk * nbin;
+    if (n_part > ((size_t) 1 << 31)) {
+        nbh_set_error("nbhip_histogram_dispersion: %zu bins x %d bodies is too large", nbin, cnt);
+        return NBHIP_USER_ERROR;
     }
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(out, dout, 14 * nbin * sizeof(double), hipMemcpyDeviceToHost,
-                           st->stream);
+    const size_t o_state = o_part + up(n_part * sizeof(double));
+    const size_t o_out = o_state + up(32 * nbin * sizeof(double));
+    const size_t need = o_out + up(14 * nbin * sizeof(double));
+    hipError_t e = hipSuccess;
+    if (need > st->disp_cap) {
+        if (st->disp_buf) e = hipFree(st->disp_buf);
+        st->disp_buf = nullptr;
+        st->disp_cap = 0;
+        if (e == hipSuccess) e = nbh_alloc(&st->disp_buf, need);
+        if (e == hipSuccess) st->disp_cap = need;
+    }
+    if (e == hipSuccess) {
+        char* b = st->disp_buf;
+        double* dout = reinterpret_cast<double*>(b + o_out);
+        nbh_dispersion(st, off, cnt, hp, dp, reinterpret_cast<int*>(b),
+                       reinterpret_cast<double*>(b + o_val), reinterpret_cast<double*>(b + o_part),
+                       reinterpret_cast<double*>(b + o_state), dout);
+        e = hipGetLastError();
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(out, dout, 14 * nbin * sizeof(double), hipMemcpyDeviceToHost,
+                               st->stream);
+    }
     if (e == hipSuccess) e = hipStreamSynchronize(st->stream);
-    for (void* p : { (void*) dbin, (void*) dval, (void*) part, (void*) state, (void*) dout })
-        if (p) (void) hipFree(p);
     if (e != hipSuccess) {
         nbh_set_error("nbhip_histogram_dispersion: %s", hipGetErrorString(e));
         return NBHIP_GPU_ERROR;

@@ -163,6 +163,8 @@ struct nbhip_state {
     unsigned char* ignore;        /* per body, original order: left out of histograms */
     unsigned* hist_buf;           /* device counts + totals of nbhip_histogram */
     size_t hist_cap;              /* unsigned entries in hist_buf */
+    char* disp_buf;               /* scratch of nbhip_histogram_dispersion (grow-only) */
+    size_t disp_cap;              /* bytes in disp_buf */
 
     WalkTune tune;
     int use_graph;

@@ -130,3 +130,23 @@ def test_dispersions_match_host_restatement():
             for g, r in zip(got[j], ref[j]):
                 np.testing.assert_allclose(g, r, rtol=1e-12, atol=1e-12)
     bt.close()
+
+
+def test_dispersion_scratch_reuse_across_shapes():
+    """The dispersion scratch is kept by the state and grown on demand: calls
+    with a small, a larger and again the small histogram shape on one state
+    each match the restatement."""
+    from oracle import histogram_ref as R
+    pos, vel, m = _stream(20000, 11, (0.0, 0.0, 0.0))
+    ctx = nb.NBodyCtx.create(timestep=1e-3, timeEvolve=1.0, eps2=1e-3, theta=0.5)
+    st = nb.NBodyState(ctx, None, pos, vel, m, prestep=False)
+    shapes = [H.HistogramParams(), H.HistogramParams(lambdaBins=100, betaBins=3),
+              H.HistogramParams()]
+    for hp in shapes:
+        assert _edges_clear(pos, hp)
+        got = st.histogram_dispersion(hp)
+        ref = R.dispersions(pos, vel, hp)
+        for k in range(1, 8):
+            for g, r in zip(got[k], ref[k]):
+                np.testing.assert_allclose(g, r, rtol=1e-12, atol=1e-12)
+    st.close()
