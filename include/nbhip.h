@@ -265,7 +265,11 @@ typedef struct nbhip_disp_params {
  * out[(2 (k - 1) + 0) * bins + b] the value of histogram k in bin b
  * (lambda-major), out[(2 (k - 1) + 1) * bins + b] its error (-1 for an
  * average over 10 or fewer bodies).  Ignored bodies are left out as in
- * nbhip_histogram. */
+ * nbhip_histogram.  Device scratch (about 48 bytes per body of the
+ * simulation plus 128 bytes per bin per 1024 bodies) is kept by the state,
+ * grown when a call needs more and released by nbhip_release;
+ * NBHIP_USER_ERROR when the per-workgroup partial sums would exceed 2^31
+ * doubles. */
 unsigned nbhip_histogram_dispersion(nbhip_state* st, int sim, const nbhip_histogram_params* hp,
                                     const nbhip_disp_params* dp, double* out);
 
