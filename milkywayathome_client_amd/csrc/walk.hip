@@ -70,6 +70,10 @@ __device__ __forceinline__ void nbh_walk_counters(DevScalars* scal, unsigned nbb
     }
 }
 
+struct Quad6 {
+    double q[6];
+};
+
 /* Plain form: FP64 end to end, reference operation order. */
 template <int BS, bool QUAD, bool COUNT>
 __global__ __launch_bounds__(BS) void k_walk64(const NodeW* __restrict__ node,
@@ -107,6 +111,9 @@ __global__ __launch_bounds__(BS) void k_walk64(const NodeW* __restrict__ node,
     bool bad = false;
     int q = L.q0;
     NodeW nd = node[max(0, min(q, scal->nnode - 1))];
+    /* the node's quadrupole moments ride along with its record (a cell's;
+     * a body's slot is never read) */
+    Quad6 nq = *reinterpret_cast<const Quad6*>(nquad + 6 * (size_t) max(0, min(q, scal->nnode - 1)));
     while (q < L.qend) {
         if (COUNT) ++steps;
         const int nd_next = __builtin_amdgcn_readfirstlane(nd.next);
@@ -126,6 +133,7 @@ __global__ __launch_bounds__(BS) void k_walk64(const NodeW* __restrict__ node,
             qn = L.qend;
         }
         const NodeW nxt = node[min(qn, L.qend - 1)];
+        const Quad6 nqx = *reinterpret_cast<const Quad6*>(nquad + 6 * (size_t) min(qn, L.qend - 1));
         if (accept) {
             /* with eps2 > 0 the self term adds exactly +0 (dx = dy = dz = 0),
              * so lanes need not branch around it */
@@ -138,9 +146,8 @@ __global__ __launch_bounds__(BS) void k_walk64(const NodeW* __restrict__ node,
                 ay += mor3 * dy;
                 az += mor3 * dz;
                 if (QUAD && nd_body < 0) {
-                    const double* Q = nquad + 6 * q;
-                    const double qxx = Q[0], qxy = Q[1], qxz = Q[2];
-                    const double qyy = Q[3], qyz = Q[4], qzz = Q[5];
+                    const double qxx = nq.q[0], qxy = nq.q[1], qxz = nq.q[2];
+                    const double qyy = nq.q[3], qyz = nq.q[4], qzz = nq.q[5];
                     const double qx = qxx * dx + qxy * dy + qxz * dz;
                     const double qy = qxy * dx + qyy * dy + qyz * dz;
                     const double qz = qxz * dx + qyz * dy + qzz * dz;
@@ -167,6 +174,7 @@ __global__ __launch_bounds__(BS) void k_walk64(const NodeW* __restrict__ node,
         if (o.multi && lp >= my_end) lp = INT_MAX;
         q = qn;
         nd = nxt;
+        nq = nqx;
     }
 
     if (bad && __lane_id() == 0) atomicOr(&scal->status, NBHIP_ERROR);
