@@ -392,6 +392,8 @@ def system_likelihood(data: Histogram, hist: Histogram, nbody: int, method: str 
             raise ValueError(f"histogram {k} missing from the data or the simulation "
                              "(from_counts(..., dispersions=...))")
     if method != "emd":
+        # nbCalcChisq (nbody/src/nbody_chisq.c:131-232) iterates lambdaBins^2
+        # bins over lambdaBins * betaBins entries: no defined result to match
         raise NotImplementedError("only the EMD geometry term is implemented")
     if hist.total_num < 0.0001 * float(nbody):
         out[:] = 0.0
