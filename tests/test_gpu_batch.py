@@ -154,3 +154,37 @@ def test_coincident_bodies_flag_only_their_simulation():
     with pytest.raises(nb.NBodyError) as e:
         nb.NBodyState(ctx, None, pos, vel, m)
     assert e.value.status & nb.NBODY_TREE_STRUCTURE_ERROR
+
+
+def test_coherent_batch_refines_each_simulation():
+    """The wave-coherent walk in a batch, where waves hold bodies of several
+    simulations (the 1-, 2- and 37-body members share one wave with the start
+    of the next): each simulation's error against its own FP64 direct sum is
+    no larger than its FP64 tree's, the one- and two-body members get their
+    exact forces, results do not depend on the workgroup size, and every
+    status word stays clean."""
+    members, eps2 = _members(), _eps2()
+    ctx = nb.NBodyCtx.create(timestep=1e-4, timeEvolve=1e-2, eps2=1e-4, theta=0.6)
+    print(f"coherent batch of {len(members)} sims {SIZES}", flush=True)
+    bt = nb.NBodyBatch(ctx, None, members, eps2=eps2, precision="mixed_coherent")
+    assert bt.statuses() == [0] * len(members)
+    for k, (pos, vel, m) in enumerate(members):
+        ck = nb.NBodyCtx.create(timestep=1e-4, timeEvolve=1e-2, eps2=eps2[k], theta=0.6)
+        one = nb.NBodyState(ck, None, pos, vel, m)
+        ab = bt.sim_bodies(k)[2]
+        a64 = one.accelerations()
+        if len(m) <= 2:
+            np.testing.assert_allclose(ab, a64, rtol=1e-6, atol=1e-12)
+        else:
+            idx = np.arange(0, len(m), max(1, len(m) // 2000), dtype=np.int32)
+            d = one.direct_accel(idx)
+            e64 = np.median(np.linalg.norm(a64[idx] - d, axis=1) / np.linalg.norm(d, axis=1))
+            ecoh = np.median(np.linalg.norm(ab[idx] - d, axis=1) / np.linalg.norm(d, axis=1))
+            print(k, len(m), "median vs direct: fp64 tree", e64, "coherent batch", ecoh)
+            assert ecoh <= 1.02 * e64, k
+        one.close()
+    bt.set_tune(64, 1024)
+    a = bt.walk_accel()
+    bt.set_tune(256, 1024)
+    assert np.array_equal(bt.walk_accel(), a)
+    bt.close()
