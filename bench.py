@@ -8,7 +8,8 @@ Hernquist bulge, logarithmic halo); BH86 theta = 0.5 with quadrupoles,
 dt = 1e-3 Gyr, eps2 from calculateEps2.  Inputs are resident in HBM before
 the timed region; one step = drift + tree build + walk + kicks for all bodies.
 
-Prints one JSON line (see the --help of each field in DESIGN.md).
+Prints one JSON line: the metric and its value, the per-phase device times,
+the walk's roofline figures and the CPU baseline.
 """
 from __future__ import annotations
 

@@ -2,7 +2,7 @@
 sums across batches, FP64 state) against the plain FP64 path on the same
 tree and against the FP64 direct sum.  Tolerances: median relative
 acceleration error <= 1e-6 and 99th percentile <= 1e-5 versus the FP64 tree
-(the north-star bound is 1e-5 median)."""
+(the accuracy target for the mixed walk is 1e-5 median against the direct sum)."""
 import numpy as np
 import pytest
 
