@@ -966,6 +966,107 @@ __global__ __launch_bounds__(256) void k_level_scatter(const int* __restrict__ n
  * children of a level-l cell are bodies or level-(l+1) cells, finished by an
  * earlier launch.  Grid-stride over the level's cells, so a fixed launch
  * shape serves any count (graph capture). */
+/* One cell's summary (see below); false on a broken Next link. */
+__device__ __forceinline__ bool nbh_summarize_cell(const int* __restrict__ level_list, int level,
+                                                   int i, NodeW* __restrict__ node,
+                                                   double* __restrict__ nquad,
+                                                   const double* __restrict__ nmid,
+                                                   const int* __restrict__ nsim,
+                                                   SimTab* __restrict__ sims,
+                                                   NodeF* __restrict__ nodef,
+                                                   const nbhip_ctx& ctx, DevScalars* scal, int nn)
+{
+    const int p = level_list[scal->level_off[level] + i];
+    const int sim = nsim[p];
+    const double rsize = sims[sim].rsize;
+    const int end = node[p].next;
+    if (p < 0 || end <= p || end > nn) {
+        return false;
+    }
+    double mass = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
+    for (int c = p + 1; c < end;) {
+        const NodeW ch = node[c];
+        mass += ch.m;
+        cx += ch.m * ch.x;
+        cy += ch.m * ch.y;
+        cz += ch.m * ch.z;
+        if (ch.next <= c || ch.next > end) {
+            return false;
+        }
+        c = ch.next;
+    }
+    const double mx = nmid[3 * p + 0], my = nmid[3 * p + 1], mz = nmid[3 * p + 2];
+    if (mass > 0.0) {
+        cx /= mass;
+        cy /= mass;
+        cz /= mass;
+    } else {
+        cx = mx; cy = my; cz = mz;
+    }
+    const double psize = ldexp(rsize, -level);
+    const double half = 0.5 * psize;
+    if (cx < mx - half || cx > mx + half || cy < my - half || cy > my + half ||
+        cz < mz - half || cz > mz + half)
+        atomicOr(&sims[sim].flags, NBHIP_TREE_STRUCTURE_ERROR);
+    const double rc2 = nbh_rcrit2(ctx, mx, my, mz, rsize, cx, cy, cz, psize);
+
+    if (ctx.use_quad) {
+        double q0 = 0.0, q1 = 0.0, q2 = 0.0, q3 = 0.0, q4 = 0.0, q5 = 0.0;
+        for (int c = p + 1; c < end;) {
+            const NodeW ch = node[c];
+            const double dx = ch.x - cx, dy = ch.y - cy, dz = ch.z - cz;
+            const double drsq = dz * dz + (dy * dy + dx * dx);
+            const double m = ch.m;
+            double e0 = m * (3.0 * (dx * dx) - drsq);
+            double e1 = m * (3.0 * (dx * dy));
+            double e2 = m * (3.0 * (dx * dz));
+            double e3 = m * (3.0 * (dy * dy) - drsq);
+            double e4 = m * (3.0 * (dy * dz));
+            double e5 = m * (3.0 * (dz * dz) - drsq);
+            if (ch.body < 0) {
+                const double* cq = &nquad[6 * c];
+                e0 += cq[0]; e1 += cq[1]; e2 += cq[2];
+                e3 += cq[3]; e4 += cq[4]; e5 += cq[5];
+            }
+            q0 += e0; q1 += e1; q2 += e2; q3 += e3; q4 += e4; q5 += e5;
+            c = ch.next;
+        }
+        double* pq = &nquad[6 * p];
+        pq[0] = q0; pq[1] = q1; pq[2] = q2; pq[3] = q3; pq[4] = q4; pq[5] = q5;
+        if (nodef) {
+            NodeF& f = nodef[p];
+            f.q[0] = (float) q0; f.q[1] = (float) q1; f.q[2] = (float) q2;
+            f.q[3] = (float) q3; f.q[4] = (float) q4; f.q[5] = (float) q5;
+        }
+    } else {
+        double* pq = &nquad[6 * p];
+        pq[0] = pq[1] = pq[2] = pq[3] = pq[4] = pq[5] = 0.0;
+        if (nodef)
+            for (int k = 0; k < 6; ++k) nodef[p].q[k] = 0.0f;
+    }
+    NodeW w;
+    w.x = cx;
+    w.y = cy;
+    w.z = cz;
+    w.m = mass;
+    w.rcrit2 = rc2;
+    w.next = end;
+    w.body = -1;
+    node[p] = w;
+    if (nodef) {
+        /* the mixed walk's record: FP64 position, the rest in FP32 */
+        NodeF& f = nodef[p];
+        f.x = cx;
+        f.y = cy;
+        f.z = cz;
+        f.m = (float) mass;
+        f.rcrit2 = (float) rc2;
+        f.next = end;
+        f.body = -1;
+    }
+    return true;
+}
+
 __global__ __launch_bounds__(256) void k_summarize_level(const int* __restrict__ level_list,
                                                          int level, NodeW* __restrict__ node,
                                                          double* __restrict__ nquad,
@@ -979,96 +1080,36 @@ __global__ __launch_bounds__(256) void k_summarize_level(const int* __restrict__
     const int cnt = scal->level_count[level];
     const int nn = scal->nnode;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
-        const int p = level_list[scal->level_off[level] + i];
-        const int sim = nsim[p];
-        const double rsize = sims[sim].rsize;
-        const int end = node[p].next;
-        if (p < 0 || end <= p || end > nn) {
+        if (!nbh_summarize_cell(level_list, level, i, node, nquad, nmid, nsim, sims, nodef, ctx,
+                                scal, nn)) {
             atomicOr(&scal->status, NBHIP_ERROR);
             return;
         }
-        double mass = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
-        for (int c = p + 1; c < end;) {
-            const NodeW ch = node[c];
-            mass += ch.m;
-            cx += ch.m * ch.x;
-            cy += ch.m * ch.y;
-            cz += ch.m * ch.z;
-            if (ch.next <= c || ch.next > end) {
-                atomicOr(&scal->status, NBHIP_ERROR);
-                return;
-            }
-            c = ch.next;
-        }
-        const double mx = nmid[3 * p + 0], my = nmid[3 * p + 1], mz = nmid[3 * p + 2];
-        if (mass > 0.0) {
-            cx /= mass;
-            cy /= mass;
-            cz /= mass;
-        } else {
-            cx = mx; cy = my; cz = mz;
-        }
-        const double psize = ldexp(rsize, -level);
-        const double half = 0.5 * psize;
-        if (cx < mx - half || cx > mx + half || cy < my - half || cy > my + half ||
-            cz < mz - half || cz > mz + half)
-            atomicOr(&sims[sim].flags, NBHIP_TREE_STRUCTURE_ERROR);
-        const double rc2 = nbh_rcrit2(ctx, mx, my, mz, rsize, cx, cy, cz, psize);
+    }
+}
 
-        if (ctx.use_quad) {
-            double q0 = 0.0, q1 = 0.0, q2 = 0.0, q3 = 0.0, q4 = 0.0, q5 = 0.0;
-            for (int c = p + 1; c < end;) {
-                const NodeW ch = node[c];
-                const double dx = ch.x - cx, dy = ch.y - cy, dz = ch.z - cz;
-                const double drsq = dz * dz + (dy * dy + dx * dx);
-                const double m = ch.m;
-                double e0 = m * (3.0 * (dx * dx) - drsq);
-                double e1 = m * (3.0 * (dx * dy));
-                double e2 = m * (3.0 * (dx * dz));
-                double e3 = m * (3.0 * (dy * dy) - drsq);
-                double e4 = m * (3.0 * (dy * dz));
-                double e5 = m * (3.0 * (dz * dz) - drsq);
-                if (ch.body < 0) {
-                    const double* cq = &nquad[6 * c];
-                    e0 += cq[0]; e1 += cq[1]; e2 += cq[2];
-                    e3 += cq[3]; e4 += cq[4]; e5 += cq[5];
-                }
-                q0 += e0; q1 += e1; q2 += e2; q3 += e3; q4 += e4; q5 += e5;
-                c = ch.next;
-            }
-            double* pq = &nquad[6 * p];
-            pq[0] = q0; pq[1] = q1; pq[2] = q2; pq[3] = q3; pq[4] = q4; pq[5] = q5;
-            if (nodef) {
-                NodeF& f = nodef[p];
-                f.q[0] = (float) q0; f.q[1] = (float) q1; f.q[2] = (float) q2;
-                f.q[3] = (float) q3; f.q[4] = (float) q4; f.q[5] = (float) q5;
-            }
-        } else {
-            double* pq = &nquad[6 * p];
-            pq[0] = pq[1] = pq[2] = pq[3] = pq[4] = pq[5] = 0.0;
-            if (nodef)
-                for (int k = 0; k < 6; ++k) nodef[p].q[k] = 0.0f;
-        }
-        NodeW w;
-        w.x = cx;
-        w.y = cy;
-        w.z = cz;
-        w.m = mass;
-        w.rcrit2 = rc2;
-        w.next = end;
-        w.body = -1;
-        node[p] = w;
-        if (nodef) {
-            /* the mixed walk's record: FP64 position, the rest in FP32 */
-            NodeF& f = nodef[p];
-            f.x = cx;
-            f.y = cy;
-            f.z = cz;
-            f.m = (float) mass;
-            f.rcrit2 = (float) rc2;
-            f.next = end;
-            f.body = -1;
-        }
+/* Levels [lo, 63], deepest first, in one workgroup with a barrier between
+ * levels: the deep end of a tree has few cells per level, so one launch
+ * replaces up to 64 - lo launches that mostly find their level empty (the
+ * graph-captured step, which cannot read the level counts back). */
+#define NBH_TAIL_THREADS 1024
+#define NBH_TAIL_LEVEL 24
+__global__ __launch_bounds__(NBH_TAIL_THREADS) void k_summarize_tail(
+    const int* __restrict__ level_list, int lo, NodeW* __restrict__ node,
+    double* __restrict__ nquad, const double* __restrict__ nmid, const int* __restrict__ nsim,
+    SimTab* __restrict__ sims, NodeF* __restrict__ nodef, nbhip_ctx ctx, DevScalars* scal)
+{
+    const int nn = scal->nnode;
+    for (int level = 63; level >= lo; --level) {
+        /* uniform: read by every thread after the previous level's barrier */
+        if (scal->status & (NBHIP_CELL_OVERFLOW_ERROR | NBHIP_ERROR)) return;
+        const int cnt = scal->level_count[level];
+        if (cnt == 0) continue;
+        for (int i = threadIdx.x; i < cnt; i += NBH_TAIL_THREADS)
+            if (!nbh_summarize_cell(level_list, level, i, node, nquad, nmid, nsim, sims, nodef,
+                                    ctx, scal, nn))
+                atomicOr(&scal->status, NBHIP_ERROR);
+        __syncthreads();
     }
 }
 
@@ -1168,6 +1209,13 @@ void nbh_build(nbhip_state* st, int graph_safe)
     if (have) memcpy(st->h_level_count, counts, sizeof(counts));
     /* 64-lane workgroups: a level's cells spread over more CUs */
     const int gmax = nbh_blocks(n, 64 * 8) + 1;
+    if (!have) {
+        /* without the counts: the deep levels in one single-workgroup launch */
+        hipLaunchKernelGGL(k_summarize_tail, dim3(1), dim3(NBH_TAIL_THREADS), 0, s,
+                           st->level_list, NBH_TAIL_LEVEL, st->node, st->nquad, st->nmid,
+                           st->nsim_node, st->sims, st->nodef, st->ctx, st->scal);
+        top = NBH_TAIL_LEVEL - 1;
+    }
     for (int l = top; l >= 0; --l) {
         if (have && counts[l] == 0) continue;
         const int g = have ? nbh_blocks(counts[l], 64) : gmax;

@@ -97,6 +97,33 @@ def test_graph_steps_equal_eager_steps(batch):
     graph.close()
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_graph_steps_with_a_deep_tree(precision):
+    """A close pair inside a Plummer sphere pushes the tree past depth 24,
+    where the graph-captured step summarises the deep levels in one
+    single-workgroup launch: graph steps still equal eager steps."""
+    ctx = nb.NBodyCtx.create(timestep=1e-4, timeEvolve=1e-2, eps2=1e-4, theta=0.5)
+    pos, vel, m = plummer_bodies(3000, seed=9)
+    pos = np.concatenate([pos, pos[:1] + np.array([[1e-7, -1e-7, 5e-8]])])
+    vel = np.concatenate([vel, vel[:1]])
+    m = np.concatenate([m, m[:1]])
+    eager = nb.NBodyState(ctx, None, pos, vel, m, precision=precision)
+    graph = nb.NBodyState(ctx, None, pos, vel, m, precision=precision)
+    depth = eager.tree_info().max_depth
+    print("max depth", depth, flush=True)
+    assert depth >= 26
+    graph.set_graph(True)
+    for rep in range(3):
+        assert eager.steps(4) == 0
+        assert graph.steps(4) == 0
+        pe, ve, ae = eager.bodies()
+        pg, vg, ag = graph.bodies()
+        assert np.array_equal(pg, pe) and np.array_equal(vg, ve) and np.array_equal(ag, ae)
+    assert graph.tree_info().max_depth >= 26
+    eager.close()
+    graph.close()
+
+
 def test_autotune_settings_keep_fp64_results():
     pos, vel, m = plummer_bodies(50000, seed=11)
     ctx = nb.NBodyCtx.create(timestep=1e-4, timeEvolve=1e-2, eps2=1e-4, theta=0.5)
