@@ -579,25 +579,24 @@ __global__ __launch_bounds__(1024) void k_scan_add(unsigned* __restrict__ data,
 /* ---------------------------------------------------------------------- */
 /* node counts, per-simulation segments                                    */
 /* ---------------------------------------------------------------------- */
-__global__ void k_node_counts(const unsigned* __restrict__ cincl, int node_cap,
-                              DevScalars* __restrict__ scal)
-{
-    const int M = scal->nmassive;
-    const int C = M > 0 ? (int) cincl[M - 1] : 0;
-    const int nn = M + C;
-    scal->ncell = C;
-    scal->nnode = nn;
-    if (nn > node_cap) scal->status |= NBHIP_CELL_OVERFLOW_ERROR;
-}
 
-/* The first massive body of each simulation owns its root cell. */
+/* Each simulation's first body (which owns its root cell) and root; thread 0 also finalises the cell
+ * and node counts (cells = the inclusive scan's last entry) and flags a node
+ * table overflow. */
 __global__ __launch_bounds__(256) void k_sim_roots(const int* __restrict__ simid_s,
                                                    const unsigned* __restrict__ cincl,
-                                                   const unsigned* __restrict__ ccount_unused,
-                                                   SimTab* __restrict__ sims,
-                                                   const DevScalars* __restrict__ scal)
+                                                   int node_cap, SimTab* __restrict__ sims,
+                                                   DevScalars* __restrict__ scal)
 {
     const int a = blockIdx.x * 256 + threadIdx.x;
+    if (a == 0) {
+        const int M = scal->nmassive;
+        const int C = M > 0 ? (int) cincl[M - 1] : 0;
+        const int nn = M + C;
+        scal->ncell = C;
+        scal->nnode = nn;
+        if (nn > node_cap) scal->status |= NBHIP_CELL_OVERFLOW_ERROR;
+    }
     if (a >= scal->nmassive) return;
     if (!nbh_first_of_sim(simid_s, a)) return;
     const int k = simid_s[a];
@@ -906,18 +905,10 @@ __global__ __launch_bounds__(256) void k_emit_leaves(const int* __restrict__ blo
     }
 }
 
-/* Cells grouped by level: exclusive scan of the per-level counts, then
- * each cell takes a slot in its level's segment (order within a level is
- * irrelevant: cells of one level are independent). */
-__global__ void k_level_scan(DevScalars* scal)
-{
-    int acc = 0;
-    for (int l = 0; l < 64; ++l) {
-        scal->level_off[l] = acc;
-        acc += scal->level_count[l];
-    }
-    scal->level_off[64] = acc;
-}
+/* Cells grouped by level: each workgroup forms the exclusive scan of the
+ * per-level counts itself (64 entries; workgroup 0 also stores it for the
+ * summaries), then each cell takes a slot in its level's segment (order
+ * within a level is irrelevant: cells of one level are independent). */
 
 /* Each workgroup takes 4096 consecutive nodes at a time (grid-stride over
  * the nodes actually built): it counts their cells per level in LDS, claims
@@ -931,7 +922,18 @@ __global__ __launch_bounds__(256) void k_level_scatter(const int* __restrict__ n
 {
     __shared__ int lh[64];
     __shared__ int base[64];
+    __shared__ int off[65];
     const int tid = threadIdx.x;
+    if (tid == 0) {
+        int acc = 0;
+        for (int l = 0; l < 64; ++l) {
+            off[l] = acc;
+            acc += scal->level_count[l];
+        }
+        off[64] = acc;
+    }
+    __syncthreads();
+    if (blockIdx.x == 0 && tid <= 64) scal->level_off[tid] = off[tid];
     const int nn = (scal->status & (NBHIP_CELL_OVERFLOW_ERROR | NBHIP_ERROR)) ? 0 : scal->nnode;
     for (int c0 = blockIdx.x * NBH_LS_CHUNK; c0 < nn; c0 += gridDim.x * NBH_LS_CHUNK) {
         const int cend = min(c0 + NBH_LS_CHUNK, nn);
@@ -951,7 +953,7 @@ __global__ __launch_bounds__(256) void k_level_scatter(const int* __restrict__ n
             const int l = nlevel[i];
             if (l >= 0 && l <= 63) {
                 const int r = atomicAdd(&lh[l], 1);
-                level_list[scal->level_off[l] + base[l] + r] = i;
+                level_list[off[l] + base[l] + r] = i;
             }
         }
         __syncthreads();
@@ -1173,9 +1175,8 @@ void nbh_build(nbhip_state* st, int graph_safe)
     hipLaunchKernelGGL(k_scan_part, dim3(1), dim3(1024), 0, s, st->scan_part, st->scal);
     hipLaunchKernelGGL(k_scan_add, dim3(nscan), dim3(1024), 0, s, st->cincl, st->scan_part,
                        st->scal);
-    hipLaunchKernelGGL(k_node_counts, dim3(1), dim3(1), 0, s, st->cincl, st->node_cap, st->scal);
     hipLaunchKernelGGL(k_sim_roots, dim3(g256), dim3(256), 0, s, st->simid_s, st->cincl,
-                       st->cincl, st->sims, st->scal);
+                       st->node_cap, st->sims, st->scal);
     hipLaunchKernelGGL(k_sim_ends, dim3(1), dim3(256), 0, s, st->nsim, st->sims, st->scal);
     hipLaunchKernelGGL(k_karras, dim3(g256), dim3(256), 0, s, n, st->su, st->simid_s, st->blo,
                        st->bhi, st->bdelta, st->bparent, st->scal);
@@ -1192,7 +1193,6 @@ void nbh_build(nbhip_state* st, int graph_safe)
         hipMemcpyAsync(st->h_counts_pinned, st->scal->level_count, 64 * sizeof(int),
                        hipMemcpyDeviceToHost, s) == hipSuccess &&
         hipEventRecord(st->ev_counts, s) == hipSuccess;
-    hipLaunchKernelGGL(k_level_scan, dim3(1), dim3(1), 0, s, st->scal);
     hipLaunchKernelGGL(k_level_scatter, dim3(nbh_blocks(2 * n + 1024, NBH_LS_CHUNK)), dim3(256),
                        0, s, st->nlevel, st->level_list, st->scal);
     nbh_mark(st, 3);
