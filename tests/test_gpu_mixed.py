@@ -102,3 +102,26 @@ def test_coherent_mixed_is_at_least_as_accurate(dwarf_case):
     a = coh.walk_accel()
     coh.set_tune(256, 1024)
     assert np.array_equal(coh.walk_accel(), a)
+
+
+@pytest.mark.parametrize("criterion,theta", [("sw93", 0.6), ("treecode", 0.5), ("bh86", 0.0)])
+def test_mixed_other_criteria_match_fp64(dwarf_case, criterion, theta):
+    """The mixed walk reads each cell's critical radius from its record, so
+    every opening criterion (SW93, TreeCode, theta = 0) follows the FP64
+    walk's lists and forces to the same tolerance as BH86."""
+    _, pot, pos, vel, m = dwarf_case
+    n = 20000
+    pos, vel, m = pos[:n], vel[:n], m[:n]
+    eps2 = models.calculate_eps2(n, 0.2, 0.8, 12.0, 60.0)
+    ctx = nb.NBodyCtx.create(timestep=1e-3, timeEvolve=1.0, eps2=eps2, theta=theta,
+                             criterion=criterion)
+    s64 = nb.NBodyState(ctx, pot, pos, vel, m)
+    s32 = nb.NBodyState(ctx, pot, pos, vel, m, precision="mixed")
+    e = rel_err(s32.accelerations(), s64.accelerations(), 0)
+    print(criterion, theta, "median", np.median(e), "p99", np.percentile(e, 99))
+    assert np.median(e) <= 1e-6 and np.percentile(e, 99) <= 1e-5
+    c64, c32 = s64.walk_counters(), s32.walk_counters()
+    for a, b in zip(c64[:3], c32[:3]):
+        assert abs(a - b) <= 1e-4 * max(a, 1)
+    s64.close()
+    s32.close()
