@@ -488,7 +488,10 @@ __global__ __launch_bounds__(256) void k_delta_count(int n, const unsigned long 
     }
     __syncthreads();
     if (sim >= 0 && sim != bsim) mixed = 1;
-    if (sim >= 0) atomicMax(&bmax, depth);
+    /* the workgroup's deepest level: a wave maximum, then one LDS atomic per
+     * wave (not one per lane on a single address) */
+    const int wmax = nbh_wave_max_i(sim >= 0 ? depth : -1);
+    if (__lane_id() == 0 && wmax >= 0) atomicMax(&bmax, wmax);
     __syncthreads();
     if (sim >= 0 && mixed) atomicMax(&sims[sim].maxdepth, depth);
     if (threadIdx.x == 0 && !mixed && bsim >= 0) atomicMax(&sims[bsim].maxdepth, bmax);
