@@ -197,6 +197,15 @@ unsigned nbhip_batch_tree_info(nbhip_state* st, int sim, nbhip_tree_info* info);
  * once per call). */
 unsigned nbhip_set_graph(nbhip_state* st, int enable);
 
+/* No counterpart in the reference.  Coherent mixed walk only
+ * (NBHIP_PRECISION_MIXED_COHERENT): scale every cell's squared critical
+ * radius by `scale` (>= 1) in the walk's records, so the walk opens more and
+ * each body's interaction list refines its reference list further (more
+ * accurate, more interactions).  Takes effect at the next tree build; 1 is
+ * the default.  NBHIP_USER_ERROR for another precision or a scale outside
+ * [1, 1e6]. */
+unsigned nbhip_set_open_scale(nbhip_state* st, double scale);
+
 /* Walk launch parameters: workgroup size (64, 128, 256 threads) and, for the
  * mixed walk, the node block (a power of two, in pre-order node indices)
  * over which a body's FP32 partial sum runs before the FP64 Kahan add.

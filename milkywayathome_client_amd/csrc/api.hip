@@ -157,6 +157,7 @@ static unsigned nbh_create(const nbhip_ctx* ctx, int device, int nsim, const int
     st->top_levels = (63 - sb) / 3;
     st->sim_shift = nsim > 1 ? 3 * st->top_levels : 64;
     st->node_cap = 5 * st->n + 1024 * nsim;
+    st->open_scale = 1.0;
     st->sort_blocks = st->n > 0 ? (st->n + NBH_SORT_TILE - 1) / NBH_SORT_TILE : 1;
     st->tune.block = 256;
     st->tune.batch = 1024;
@@ -749,6 +750,22 @@ extern "C" unsigned nbhip_set_tune(nbhip_state* st, const nbhip_tune* tune)
     st->tune.block = tune->block;
     st->tune.batch = tune->batch;
     st->graph_ready = 0;
+    return NBHIP_SUCCESS;
+}
+
+extern "C" unsigned nbhip_set_open_scale(nbhip_state* st, double scale)
+{
+    if (!st) return NBHIP_USER_ERROR;
+    if (st->ctx.precision != NBHIP_PRECISION_MIXED_COHERENT) {
+        nbh_set_error("nbhip_set_open_scale: only the coherent mixed walk opens more");
+        return NBHIP_USER_ERROR;
+    }
+    if (!(scale >= 1.0) || !(scale <= 1.0e6)) {
+        nbh_set_error("nbhip_set_open_scale: scale %.17g is outside [1, 1e6]", scale);
+        return NBHIP_USER_ERROR;
+    }
+    st->open_scale = scale;
+    st->graph_ready = 0;          /* the captured step carries the old factor */
     return NBHIP_SUCCESS;
 }
 

@@ -163,6 +163,7 @@ struct nbhip_state {
     unsigned char* ignore;        /* per body, original order: left out of histograms */
     unsigned* hist_buf;           /* device counts + totals of nbhip_histogram */
     size_t hist_cap;              /* unsigned entries in hist_buf */
+    double open_scale;            /* coherent walk: critical radius^2 factor (>= 1) */
     char* disp_buf;               /* scratch of nbhip_histogram_dispersion (grow-only) */
     size_t disp_cap;              /* bytes in disp_buf */
 

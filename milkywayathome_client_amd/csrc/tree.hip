@@ -979,7 +979,8 @@ __device__ __forceinline__ bool nbh_summarize_cell(const int* __restrict__ level
                                                    const int* __restrict__ nsim,
                                                    SimTab* __restrict__ sims,
                                                    NodeF* __restrict__ nodef,
-                                                   const nbhip_ctx& ctx, DevScalars* scal, int nn)
+                                                   const nbhip_ctx& ctx, double rc_scale,
+                                                   DevScalars* scal, int nn)
 {
     const int p = level_list[scal->level_off[level] + i];
     const int sim = nsim[p];
@@ -1065,7 +1066,8 @@ __device__ __forceinline__ bool nbh_summarize_cell(const int* __restrict__ level
         f.y = cy;
         f.z = cz;
         f.m = (float) mass;
-        f.rcrit2 = (float) rc2;
+        /* the coherent walk may open more (rc_scale >= 1; 1 otherwise) */
+        f.rcrit2 = (float) (rc2 * rc_scale);
         f.next = end;
         f.body = -1;
     }
@@ -1079,14 +1081,15 @@ __global__ __launch_bounds__(256) void k_summarize_level(const int* __restrict__
                                                          const int* __restrict__ nsim,
                                                          SimTab* __restrict__ sims,
                                                          NodeF* __restrict__ nodef,
-                                                         nbhip_ctx ctx, DevScalars* scal)
+                                                         nbhip_ctx ctx, double rc_scale,
+                                                         DevScalars* scal)
 {
     if (scal->status & (NBHIP_CELL_OVERFLOW_ERROR | NBHIP_ERROR)) return;
     const int cnt = scal->level_count[level];
     const int nn = scal->nnode;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
         if (!nbh_summarize_cell(level_list, level, i, node, nquad, nmid, nsim, sims, nodef, ctx,
-                                scal, nn)) {
+                                rc_scale, scal, nn)) {
             atomicOr(&scal->status, NBHIP_ERROR);
             return;
         }
@@ -1102,7 +1105,8 @@ __global__ __launch_bounds__(256) void k_summarize_level(const int* __restrict__
 __global__ __launch_bounds__(NBH_TAIL_THREADS) void k_summarize_tail(
     const int* __restrict__ level_list, int lo, NodeW* __restrict__ node,
     double* __restrict__ nquad, const double* __restrict__ nmid, const int* __restrict__ nsim,
-    SimTab* __restrict__ sims, NodeF* __restrict__ nodef, nbhip_ctx ctx, DevScalars* scal)
+    SimTab* __restrict__ sims, NodeF* __restrict__ nodef, nbhip_ctx ctx, double rc_scale,
+    DevScalars* scal)
 {
     const int nn = scal->nnode;
     for (int level = 63; level >= lo; --level) {
@@ -1112,7 +1116,7 @@ __global__ __launch_bounds__(NBH_TAIL_THREADS) void k_summarize_tail(
         if (cnt == 0) continue;
         for (int i = threadIdx.x; i < cnt; i += NBH_TAIL_THREADS)
             if (!nbh_summarize_cell(level_list, level, i, node, nquad, nmid, nsim, sims, nodef,
-                                    ctx, scal, nn))
+                                    ctx, rc_scale, scal, nn))
                 atomicOr(&scal->status, NBHIP_ERROR);
         __syncthreads();
     }
@@ -1216,7 +1220,7 @@ void nbh_build(nbhip_state* st, int graph_safe)
         /* without the counts: the deep levels in one single-workgroup launch */
         hipLaunchKernelGGL(k_summarize_tail, dim3(1), dim3(NBH_TAIL_THREADS), 0, s,
                            st->level_list, NBH_TAIL_LEVEL, st->node, st->nquad, st->nmid,
-                           st->nsim_node, st->sims, st->nodef, st->ctx, st->scal);
+                           st->nsim_node, st->sims, st->nodef, st->ctx, st->open_scale, st->scal);
         top = NBH_TAIL_LEVEL - 1;
     }
     for (int l = top; l >= 0; --l) {
@@ -1224,7 +1228,7 @@ void nbh_build(nbhip_state* st, int graph_safe)
         const int g = have ? nbh_blocks(counts[l], 64) : gmax;
         hipLaunchKernelGGL(k_summarize_level, dim3(g), dim3(64), 0, s, st->level_list, l,
                            st->node, st->nquad, st->nmid, st->nsim_node, st->sims, st->nodef,
-                           st->ctx, st->scal);
+                           st->ctx, st->open_scale, st->scal);
     }
     nbh_mark(st, 4);
 }

@@ -110,6 +110,7 @@ EXPORTED = [
     "nbhip_batch_tree_info", "nbhip_set_graph", "nbhip_autotune", "nbhip_set_tune",
     "nbhip_adjoint_forward", "nbhip_adjoint_backward", "nbhip_gravity_vjp", "nbhip_set_ignore",
     "nbhip_histogram", "nbhip_orbit_tangent", "nbhip_restore", "nbhip_histogram_dispersion",
+    "nbhip_set_open_scale",
 ]
 
 _LIB: Optional[C.CDLL] = None
@@ -154,6 +155,7 @@ def load_library() -> C.CDLL:
         "nbhip_batch_status": (u, [st, C.POINTER(C.c_uint)]),
         "nbhip_batch_tree_info": (u, [st, C.c_int, C.POINTER(_TreeInfo)]),
         "nbhip_set_graph": (u, [st, C.c_int]),
+        "nbhip_set_open_scale": (u, [st, C.c_double]),
         "nbhip_autotune": (u, [st, C.POINTER(_Tune), C.POINTER(_Tune), C.c_int,
                                C.POINTER(C.c_int)]),
         "nbhip_set_tune": (u, [st, C.POINTER(_Tune)]),
@@ -473,6 +475,12 @@ class NBodyState:
         self._check(self.lib.nbhip_autotune(self._h, C.byref(best), trials, 32, C.byref(cnt)))
         return ((best.block, best.batch, best.ms),
                 [(trials[i].block, trials[i].batch, trials[i].ms) for i in range(cnt.value)])
+
+    def set_open_scale(self, scale: float) -> None:
+        """Coherent mixed walk only: scale every cell's squared critical
+        radius by `scale` (>= 1), so the walk opens more and each body's list
+        refines its reference list further; from the next tree build."""
+        self._check(self.lib.nbhip_set_open_scale(self._h, float(scale)))
 
     def set_tune(self, block: int, batch: int = 1024) -> None:
         t = _Tune(block, batch, 0.0)

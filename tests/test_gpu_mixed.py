@@ -125,3 +125,33 @@ def test_mixed_other_criteria_match_fp64(dwarf_case, criterion, theta):
         assert abs(a - b) <= 1e-4 * max(a, 1)
     s64.close()
     s32.close()
+
+
+def test_coherent_open_scale_refines(dwarf_case):
+    """nbhip_set_open_scale: scaling the critical radii of the coherent walk
+    opens more cells, so it does more interactions and its error against the
+    direct sum falls; scale 1 restores the default forces bit for bit; other
+    walks and scales below 1 are rejected."""
+    ctx, pot, pos, vel, m = dwarf_case
+    idx = np.arange(0, pos.shape[0], 10, dtype=np.int32)
+    coh = nb.NBodyState(ctx, None, pos, vel, m, precision="mixed_coherent")
+    d = coh.direct_accel(idx)
+    a1 = coh.accelerations()
+    e1 = np.median(rel_err(a1[idx], d, 0))
+    c1 = coh.walk_counters()
+    coh.set_open_scale(2.0)
+    coh.gravity()
+    e2 = np.median(rel_err(coh.accelerations()[idx], d, 0))
+    c2 = coh.walk_counters()
+    print("median vs direct: scale 1", e1, "scale 2", e2, "interactions", c1[:2], c2[:2])
+    assert e2 < e1 and c2[0] + c2[1] > c1[0] + c1[1]
+    coh.set_open_scale(1.0)
+    coh.gravity()
+    assert np.array_equal(coh.accelerations(), a1)
+    with pytest.raises(nb.NBodyError):
+        coh.set_open_scale(0.5)
+    coh.close()
+    mixed = nb.NBodyState(ctx, None, pos, vel, m, precision="mixed", prestep=False)
+    with pytest.raises(nb.NBodyError):
+        mixed.set_open_scale(2.0)
+    mixed.close()
