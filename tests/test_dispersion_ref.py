@@ -9,10 +9,12 @@
   the checker of nbhip_histogram_dispersion) against closed forms: a bin of
   bodies at beta = +-a, a planted outlier that the rejection removes, bins of
   ten or fewer bodies, and the line-of-sight velocity of a radial orbit;
-* its proper motions: the frame rotation by where it sends the Galactic and
-  celestial poles (and the Galactic centre), the motions by finite
-  differences of the equatorial angles of bodies moving along straight
-  lines, zero for radial motion relative to the Sun."""
+* its proper motions: the known answers of the reference's
+  nbody/tests/propermotion_test.c (nine test objects, 0.2 mas/yr), the frame
+  rotation by where it sends the Galactic and celestial poles (and the
+  Galactic centre), the motions by finite differences of the equatorial
+  angles of bodies moving along straight lines, zero for radial motion
+  relative to the Sun."""
 import math
 import os
 
@@ -255,3 +257,31 @@ def test_likelihood_with_proper_motions():
                           dispersions=R.dispersions(pos, vel + np.array([0.0, 0.0, 40.0]), hp))
     out = L.system_likelihood(sim, moved, 30000, use_prop_mot=True)
     assert out[8] > 0.0 and out[9] > 0.0 and out[0] == out[1] + out[2] + out[8] + out[9]
+
+
+# nbody/tests/propermotion_test.c: test objects (position kpc, velocity, both
+# Galactocentric) and the (mu_dec, mu_ra) it expects within 0.2 mas/yr with the
+# default context (Sun 8 kpc from the centre, default Sun velocity and angles)
+PROPER_MOTION_KATS = [
+    ((-10.0, 0.0, 0.0), (10.3, 229.2, 6.9), (0.0, 0.0)),        # co-moving with the Sun
+    ((3.3, 2.8, -3.1), (10.3, 229.2, 6.9), (0.0, 0.0)),
+    ((-10.0, 0.0, 0.0), (40.1, 229.2, 6.9), (0.0, 0.0)),        # along the line of sight
+    ((0.0, 0.0, 0.0), (-10.3, -229.2, -6.9), (-10.28, -5.8)),   # centre, twice the reflex
+    ((-16.0, 0.0, 0.0), (0.0, 0.0, 0.0), (-5.14, 2.9)),         # behind the Sun
+    ((8.0, 0.0, 0.0), (0.0, 0.0, 0.0), (-2.57, -1.45)),         # twice as far
+    ((18.3, 229.2, 6.9), (0.0, 0.0, 0.0), (0.0, 0.0)),          # on the Sun's path
+    ((-2.3, -229.2, -6.9), (0.0, 0.0, 0.0), (0.0, 0.0)),
+    ((-1.1, -41.1, -27.9), (-57.0, -226.0, 221.0), (0.4, 1.9)),  # the LMC
+]
+
+
+@pytest.mark.parametrize("pos,vel,expect", PROPER_MOTION_KATS)
+def test_proper_motion_known_answers(pos, vel, expect):
+    mud, mur = R.proper_motions(np.array([pos]), np.array([vel]), 8.0)
+    assert abs(mud[0] - expect[0]) < 0.2 and abs(mur[0] - expect[1]) < 0.2
+
+
+def test_frame_rotation_is_unitary():
+    """propermotion_test.c's rotation check: a unit vector stays a unit vector."""
+    v = R.galactic_to_equatorial() @ np.array([1.0, 0.0, 0.0])
+    assert abs(np.linalg.norm(v) - 1.0) < 1e-15
