@@ -153,6 +153,12 @@ def ext_accel(potential, pos) -> np.ndarray:
     return out
 
 
+def virial_radius(a_b: float, a_d: float, m_b: float, m_d: float) -> float:
+    """nbCalculateVirial (nbody/src/nbody_virial.c:38-102): the virial radius
+    of a two-Plummer dwarf."""
+    return float(load_models().nbm_virial_radius(a_b, a_d, m_b, m_d))
+
+
 def calculate_eps2(nbody: float, a_b: float, a_d: float, m_b: float, m_d: float) -> float:
     """calculateEps2(nbody, a_b, a_d, M_b, M_d, 0) (nbody/src/nbody_lua_models.c:158-169)."""
     return load_models().nbm_calculate_eps2(float(nbody), a_b, a_d, m_b, m_d)

@@ -103,3 +103,18 @@ def test_softening_and_timestep_helpers():
     assert math.isclose(eps2, (rv * 0.98 * 10000 ** -0.26) ** 2, rel_tol=1e-12)
     dt = models.calculate_timestep(16.0, 0.2)
     assert math.isclose(dt, 0.01 * math.sqrt(4.0 / 3.0 * math.pi * 0.008 / 16.0))
+
+
+def test_virial_radius_of_one_plummer():
+    """nbody/tests/virial_test.c: with one component massless the virial
+    radius is the single Plummer sphere's, 16 a / (3 pi), to 1e-5 (1000 random
+    scale radii in [0.1, 5] and masses in [1, 20])."""
+    rng = np.random.default_rng(17)
+    for _ in range(1000):
+        a_b, a_d = rng.uniform(0.1, 5.0, 2)
+        m_b, m_d = rng.uniform(1.0, 20.0, 2)
+        if rng.uniform() < 0.5:
+            m_b, lhs = 0.0, 16.0 * a_d / 3.0 / np.pi
+        else:
+            m_d, lhs = 0.0, 16.0 * a_b / 3.0 / np.pi
+        assert abs(models.virial_radius(a_b, a_d, m_b, m_d) - lhs) <= 1e-5
