@@ -312,7 +312,8 @@ def run_adjoint(args):
 
 def run_precision(args):
     """Precision workload: the N = 10^6 two-component dwarf advanced 1000
-    steps with each form (FP64, mixed, coherent mixed).  At steps 0, 100
+    steps with each form (FP64, mixed, coherent mixed, and the coherent walk
+    with its squared critical radii scaled by 1.5 via nbhip_set_open_scale).  At steps 0, 100
     and 1000, 4096 sampled bodies are checked against the FP64 direct sum
     (median and 99th percentile); the mixed forms' step-0 accelerations are
     compared body by body with the FP64 tree's; the final lambda histograms
@@ -327,11 +328,17 @@ def run_precision(args):
     hp = H.HistogramParams()
     checkpoints = [0, 100, 1000]
     out, hists, acc0, raw = {}, {}, {}, {}
-    for prec in ("fp64", "mixed", "mixed_coherent"):
+    forms = {"fp64": ("fp64", 1.0), "mixed": ("mixed", 1.0),
+             "mixed_coherent": ("mixed_coherent", 1.0),
+             "mixed_coherent_open1.5": ("mixed_coherent", 1.5)}
+    for prec, (kind, scale) in forms.items():
         ctx = nb.NBodyCtx.create(timestep=dt, timeEvolve=dt * checkpoints[-1], eps2=eps2,
                                  theta=args.theta, criterion="BH86", useQuad=True,
                                  treeRSize=4.0)
-        st = nb.NBodyState(ctx, pot, pos, vel, m, precision=prec)
+        st = nb.NBodyState(ctx, pot, pos, vel, m, precision=kind, prestep=scale == 1.0)
+        if scale != 1.0:
+            st.set_open_scale(scale)
+            st.gravity()
         tune = None
         if prec != "fp64" and not args.no_autotune:
             best, _ = st.autotune()
@@ -359,7 +366,7 @@ def run_precision(args):
                      "ms_per_step": 1e3 * t_run / checkpoints[-1],
                      "status": st.last_status}
         st.close()
-    for prec in ("mixed", "mixed_coherent"):
+    for prec in ("mixed", "mixed_coherent", "mixed_coherent_open1.5"):
         a, b = acc0[prec], acc0["fp64"]
         rel = np.linalg.norm(a - b, axis=1) / np.linalg.norm(b, axis=1)
         out[prec]["step0_vs_fp64_tree"] = {"median": float(np.median(rel)),
