@@ -1225,7 +1225,12 @@ void nbh_build(nbhip_state* st, int graph_safe)
     }
     for (int l = top; l >= 0; --l) {
         if (have && counts[l] == 0) continue;
-        const int g = have ? nbh_blocks(counts[l], 64) : gmax;
+        int g = have ? nbh_blocks(counts[l], 64) : gmax;
+        if (!have && 3 * l < 40) {
+            /* a level holds at most 8^l cells per simulation: no larger grid */
+            const long long cap = (long long) st->nsim << (3 * l);
+            if (cap < (long long) g * 64) g = nbh_blocks((int) cap, 64);
+        }
         hipLaunchKernelGGL(k_summarize_level, dim3(g), dim3(64), 0, s, st->level_list, l,
                            st->node, st->nquad, st->nmid, st->nsim_node, st->sims, st->nodef,
                            st->ctx, st->open_scale, st->scal);
