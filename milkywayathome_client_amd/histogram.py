@@ -74,14 +74,23 @@ def create_histogram(pos: np.ndarray, hp: HistogramParams, ignore=None):
 
 
 def emd(a: np.ndarray, b: np.ndarray, hp: HistogramParams) -> float:
-    """EMD (degrees of lambda) between two normalised single-beta-bin
-    histograms, rounded to 1e-9."""
-    if hp.betaBins != 1:
-        raise NotImplementedError("EMD is implemented for one beta bin")
+    """EMD (degrees) between two histograms [lambdaBins * betaBins]
+    (lambda-major), each normalised to unit mass, rounded to 1e-9: the
+    cumulative-difference closed form for one beta bin, the transport linear
+    program over the bin centres otherwise."""
     a = np.asarray(a, dtype=np.float64)
     b = np.asarray(b, dtype=np.float64)
     if a.sum() <= 0 or b.sum() <= 0:
         return math.inf
+    if hp.betaBins != 1:
+        from . import likelihood as L
+        ls = (hp.lambdaEnd - hp.lambdaStart) / hp.lambdaBins
+        bs = (hp.betaEnd - hp.betaStart) / hp.betaBins
+        i, j = np.meshgrid(np.arange(hp.lambdaBins), np.arange(hp.betaBins), indexing="ij")
+        lam = ((i + 0.5) * ls + hp.lambdaStart).ravel()
+        beta = ((j + 0.5) * bs + hp.betaStart).ravel()
+        d = L.emd_calc(a / a.sum(), lam, beta, b / b.sum(), lam, beta)
+        return round(d * 1e9) * 1e-9
     ca = np.cumsum(a / a.sum())
     cb = np.cumsum(b / b.sum())
     w = (hp.lambdaEnd - hp.lambdaStart) / hp.lambdaBins

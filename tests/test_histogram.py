@@ -5,6 +5,7 @@ closed form for a shifted delta."""
 import math
 
 import numpy as np
+import pytest
 from scipy.stats import wasserstein_distance
 
 from milkywayathome_client_amd import histogram as H
@@ -67,3 +68,28 @@ def test_emd_against_scipy_and_closed_form():
     d2[17] = 1.0
     assert abs(H.emd(d1, d2, hp) - 7 * w) < 1e-9
     assert H.emd(a, a, hp) == 0.0
+
+
+def test_emd_over_several_beta_bins():
+    """With several beta bins the EMD is the transport distance between bin
+    centres: a unit mass moved from one bin to another costs their Euclidean
+    distance in (lambda, beta) degrees; a histogram against itself costs 0;
+    one beta bin keeps the closed form."""
+    hp = H.HistogramParams(betaBins=3)
+    nb_ = hp.lambdaBins * hp.betaBins
+    a = np.zeros(nb_)
+    b = np.zeros(nb_)
+    a[10 * 3 + 0] = 1.0           # lambda bin 10, beta bin 0
+    b[13 * 3 + 2] = 1.0           # lambda bin 13, beta bin 2
+    ls = (hp.lambdaEnd - hp.lambdaStart) / hp.lambdaBins
+    bs = (hp.betaEnd - hp.betaStart) / hp.betaBins
+    assert H.emd(a, b, hp) == pytest.approx(np.hypot(3 * ls, 2 * bs), abs=1e-8)
+    rng = np.random.default_rng(2)
+    w = rng.uniform(size=nb_)
+    assert H.emd(w, w, hp) == pytest.approx(0.0, abs=1e-8)
+    one = H.HistogramParams()
+    x = np.zeros(one.lambdaBins)
+    y = np.zeros(one.lambdaBins)
+    x[5], y[9] = 1.0, 1.0
+    assert H.emd(x, y, one) == pytest.approx(4 * (one.lambdaEnd - one.lambdaStart)
+                                             / one.lambdaBins, abs=1e-9)
