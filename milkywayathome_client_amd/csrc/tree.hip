@@ -659,6 +659,7 @@ __device__ __forceinline__ double nbh_rcrit2(const nbhip_ctx& c, double mx, doub
 /* ---------------------------------------------------------------------- */
 __global__ __launch_bounds__(256) void k_karras(int n, const unsigned long long* __restrict__ su,
                                                 const int* __restrict__ simid_s,
+                                                const int* __restrict__ dadj,
                                                 int* __restrict__ blo, int* __restrict__ bhi,
                                                 int* __restrict__ bdelta, int* __restrict__ bparent,
                                                 const DevScalars* __restrict__ scal)
@@ -669,8 +670,9 @@ __global__ __launch_bounds__(256) void k_karras(int n, const unsigned long long*
     if (scal->status & NBHIP_CELL_OVERFLOW_ERROR) return;
     const int sb = scal->sim_bits;
 #define KD(a, b) nbh_kdelta(su, simid_s, n, M, sb, (a), (b))
-    const int dr = KD(i, i + 1);
-    const int dl = KD(i, i - 1);
+    /* the neighbours' prefixes were formed by k_delta_count */
+    const int dr = dadj[i];
+    const int dl = i > 0 ? dadj[i - 1] : -1;
     const int d = dr > dl ? 1 : -1;
     const int dmin = d > 0 ? dl : dr;
     int lmax = 2;
@@ -1185,8 +1187,8 @@ void nbh_build(nbhip_state* st, int graph_safe)
     hipLaunchKernelGGL(k_sim_roots, dim3(g256), dim3(256), 0, s, st->simid_s, st->cincl,
                        st->node_cap, st->sims, st->scal);
     hipLaunchKernelGGL(k_sim_ends, dim3(1), dim3(256), 0, s, st->nsim, st->sims, st->scal);
-    hipLaunchKernelGGL(k_karras, dim3(g256), dim3(256), 0, s, n, st->su, st->simid_s, st->blo,
-                       st->bhi, st->bdelta, st->bparent, st->scal);
+    hipLaunchKernelGGL(k_karras, dim3(g256), dim3(256), 0, s, n, st->su, st->simid_s, st->dadj,
+                       st->blo, st->bhi, st->bdelta, st->bparent, st->scal);
     hipLaunchKernelGGL(k_emit_cells, dim3(g256), dim3(256), 0, s, st->blo, st->bhi, st->bdelta,
                        st->bparent, st->dadj, st->cincl, st->su, st->simid_s, n, st->node,
                        st->nmid, st->nlevel, st->nparent, st->nsim_node, st->sims, st->scal);
