@@ -191,6 +191,14 @@ unsigned nbhip_batch_download(nbhip_state* st, int sim, double* pos, double* vel
 /* Sticky status of each simulation (nsim entries). */
 unsigned nbhip_batch_status(nbhip_state* st, unsigned* status);
 unsigned nbhip_batch_tree_info(nbhip_state* st, int sim, nbhip_tree_info* info);
+/* nbhip_tree_download for simulation `sim` of a batch: its own pre-order
+ * segment (nbhip_batch_tree_info's nnode entries), with skip counted from
+ * the segment's first node (its root) and ref from the simulation's first
+ * body, so it compares directly with the simulation's tree built alone.
+ * nbhip_tree_download returns the whole table of every simulation. */
+unsigned nbhip_batch_tree_download(nbhip_state* st, int sim, int* kind, int* ref, int* skip,
+                                   int* level, double* pos, double* mass, double* rcrit2,
+                                   double* quad);
 
 /* enable != 0: nbhip_steps captures one step into a hipGraph on first use and
  * replays it (no host synchronisation inside a step; the status is read
@@ -234,6 +242,20 @@ typedef struct nbhip_histogram_params {
     int lambda_bins, beta_bins;
     double sun_gc_dist;
 } nbhip_histogram_params;
+
+/* No counterpart in the reference.  nbhip_histogram without any device
+ * allocation, for callers that cannot allocate device memory (code under
+ * HIP graph capture, a best-likelihood loop that bins every step, serving
+ * loops that forbid hipMalloc).  The counters live in a fixed device array
+ * of NBHIP_NOALLOC_MAX_COUNTERS unsigned entries (4 MiB), one copy per GPU
+ * held until the process ends; its address is looked up once per device by
+ * nbhip_create / nbhip_batch_create.  nsim * (lambda_bins * beta_bins + 1)
+ * must not exceed the maximum (NBHIP_USER_ERROR otherwise).  All calls on
+ * one GPU share that array: calls that overlap on one GPU (several host
+ * threads, several states) race.  Same outputs as nbhip_histogram. */
+#define NBHIP_NOALLOC_MAX_COUNTERS (1u << 20)
+unsigned nbhip_histogram_noalloc(nbhip_state* st, const nbhip_histogram_params* hp,
+                                 unsigned* counts, unsigned* totals);
 
 /* Per-body flag (n entries, all simulations, upload order); nonzero = the
  * body is left out of histograms, as ignoreBody leaves out the dark matter

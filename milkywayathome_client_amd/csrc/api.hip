@@ -146,6 +146,8 @@ static unsigned nbh_create(const nbhip_ctx* ctx, int device, int nsim, const int
         return NBHIP_CAPABILITY_ERROR;
     }
     NBH_CHECK(hipSetDevice(device));
+    rc = nbh_scratch_init(device);
+    if (rc) return rc;
 
     nbhip_state* st = (nbhip_state*) calloc(1, sizeof(nbhip_state));
     st->ctx = *ctx;
@@ -589,6 +591,41 @@ extern "C" unsigned nbhip_batch_status(nbhip_state* st, unsigned* status)
     return NBHIP_SUCCESS;
 }
 
+/* Node table segment [first, first + nn) of the last tree; skip and ref are
+ * made relative to the segment and to body_off. */
+static unsigned nbh_tree_segment(nbhip_state* st, int first, int nn, int body_off, int* kind,
+                                 int* ref, int* skip, int* level, double* pos, double* mass,
+                                 double* rcrit2, double* quad)
+{
+    if (nn <= 0) return NBHIP_SUCCESS;
+    std::vector<NodeW> nodes(nn);
+    std::vector<double> q(6 * (size_t) nn);
+    std::vector<int> lev(nn);
+    std::vector<unsigned> perm(st->n > 0 ? st->n : 1);
+    NBH_CHECK(hipMemcpy(nodes.data(), st->node + first, nn * sizeof(NodeW),
+                        hipMemcpyDeviceToHost));
+    NBH_CHECK(hipMemcpy(q.data(), st->nquad + 6 * (size_t) first, 6 * (size_t) nn * sizeof(double),
+                        hipMemcpyDeviceToHost));
+    NBH_CHECK(hipMemcpy(lev.data(), st->nlevel + first, nn * sizeof(int), hipMemcpyDeviceToHost));
+    if (st->n > 0)
+        NBH_CHECK(hipMemcpy(perm.data(), st->perm, st->n * sizeof(unsigned),
+                            hipMemcpyDeviceToHost));
+    for (int i = 0; i < nn; ++i) {
+        const NodeW& w = nodes[i];
+        const bool body = w.body >= 0;
+        if (kind) kind[i] = body ? 1 : 0;
+        if (ref) ref[i] = body ? (int) perm[w.body] - body_off : -1;
+        if (skip) skip[i] = w.next - first;
+        if (level) level[i] = body ? -1 : lev[i];
+        if (pos) { pos[3 * i] = w.x; pos[3 * i + 1] = w.y; pos[3 * i + 2] = w.z; }
+        if (mass) mass[i] = w.m;
+        if (rcrit2) rcrit2[i] = body ? 0.0 : w.rcrit2;
+        if (quad)
+            for (int k = 0; k < 6; ++k) quad[6 * i + k] = body ? 0.0 : q[6 * (size_t) i + k];
+    }
+    return NBHIP_SUCCESS;
+}
+
 extern "C" unsigned nbhip_tree_download(nbhip_state* st, int* kind, int* ref, int* skip,
                                         int* level, double* pos, double* mass, double* rcrit2,
                                         double* quad)
@@ -607,31 +644,31 @@ extern "C" unsigned nbhip_tree_download(nbhip_state* st, int* kind, int* ref, in
         nbh_set_error("no valid tree (nnode %d)", nn);
         return NBHIP_ERROR;
     }
-    std::vector<NodeW> nodes(nn);
-    std::vector<double> q(6 * (size_t) nn);
-    std::vector<int> lev(nn);
-    std::vector<unsigned> perm(st->n > 0 ? st->n : 1);
-    NBH_CHECK(hipMemcpy(nodes.data(), st->node, nn * sizeof(NodeW), hipMemcpyDeviceToHost));
-    NBH_CHECK(hipMemcpy(q.data(), st->nquad, 6 * (size_t) nn * sizeof(double),
-                        hipMemcpyDeviceToHost));
-    NBH_CHECK(hipMemcpy(lev.data(), st->nlevel, nn * sizeof(int), hipMemcpyDeviceToHost));
-    if (st->n > 0)
-        NBH_CHECK(hipMemcpy(perm.data(), st->perm, st->n * sizeof(unsigned),
-                            hipMemcpyDeviceToHost));
-    for (int i = 0; i < nn; ++i) {
-        const NodeW& w = nodes[i];
-        const bool body = w.body >= 0;
-        if (kind) kind[i] = body ? 1 : 0;
-        if (ref) ref[i] = body ? (int) perm[w.body] : -1;
-        if (skip) skip[i] = w.next;
-        if (level) level[i] = body ? -1 : lev[i];
-        if (pos) { pos[3 * i] = w.x; pos[3 * i + 1] = w.y; pos[3 * i + 2] = w.z; }
-        if (mass) mass[i] = w.m;
-        if (rcrit2) rcrit2[i] = body ? 0.0 : w.rcrit2;
-        if (quad)
-            for (int k = 0; k < 6; ++k) quad[6 * i + k] = body ? 0.0 : q[6 * (size_t) i + k];
+    return nbh_tree_segment(st, 0, nn, 0, kind, ref, skip, level, pos, mass, rcrit2, quad);
+}
+
+extern "C" unsigned nbhip_batch_tree_download(nbhip_state* st, int sim, int* kind, int* ref,
+                                              int* skip, int* level, double* pos, double* mass,
+                                              double* rcrit2, double* quad)
+{
+    if (!st || sim < 0 || sim >= st->nsim) return NBHIP_USER_ERROR;
+    if (!st->have_tree) {
+        nbh_set_error("no tree: call nbhip_gravity first (and criterion != Exact)");
+        return NBHIP_USER_ERROR;
     }
-    return NBHIP_SUCCESS;
+    NBH_CHECK(hipSetDevice(st->device));
+    SimTab t;
+    DevScalars s;
+    NBH_CHECK(hipMemcpyAsync(&t, st->sims + sim, sizeof(t), hipMemcpyDeviceToHost, st->stream));
+    NBH_CHECK(hipMemcpyAsync(&s, st->scal, sizeof(s), hipMemcpyDeviceToHost, st->stream));
+    NBH_CHECK(hipStreamSynchronize(st->stream));
+    if (t.root < 0) return NBHIP_SUCCESS;            /* no massive body: no tree */
+    if (t.end <= t.root || t.end > s.nnode || s.nnode > st->node_cap) {
+        nbh_set_error("no valid tree segment for simulation %d", sim);
+        return NBHIP_ERROR;
+    }
+    return nbh_tree_segment(st, t.root, t.end - t.root, st->h_body_off[sim], kind, ref, skip,
+                            level, pos, mass, rcrit2, quad);
 }
 
 extern "C" unsigned nbhip_direct_accel(nbhip_state* st, int nsample, const int* idx, double* acc)

@@ -15,6 +15,9 @@
  */
 #include "nbhip_device.h"
 
+#include <algorithm>
+#include <mutex>
+
 struct HistParams {
     double r[3][3];         /* rotation rows */
     double sun;
@@ -387,4 +390,87 @@ void nbh_histogram(nbhip_state* st, const nbhip_histogram_params* hp, unsigned* 
     hipLaunchKernelGGL(k_histogram, dim3(nbh_blocks(st->n, 256)), dim3(256), 0, st->stream,
                        st->n, st->px, st->py, st->pz, st->ignore, st->simid, h, d_counts,
                        d_totals);
+}
+
+/* ---------------------------------------------------------------------- */
+/* allocation-free histogram counts                                        */
+/* ---------------------------------------------------------------------- */
+/* Device scratch of nbhip_histogram_noalloc: the per-simulation bin counters
+ * and totals.  One copy per GPU, held until the process ends; every call on
+ * a GPU shares it, so calls that overlap on one GPU race.  In host code the
+ * array's name stands for the symbol, not a device address: kernels receive
+ * the per-device address that nbh_scratch_init looks up. */
+static __device__ __align__(256) unsigned scratch[NBHIP_NOALLOC_MAX_COUNTERS];
+
+#define NBH_MAX_DEVICES 64
+static std::mutex g_scratch_mu;
+static unsigned* g_scratch_ptr[NBH_MAX_DEVICES];
+
+/* Looks up this GPU's copy of the scratch array once; called by nbhip_create
+ * and nbhip_batch_create with `device` current. */
+unsigned nbh_scratch_init(int device)
+{
+    if (device < 0 || device >= NBH_MAX_DEVICES) {
+        nbh_set_error("device %d: at most %d devices", device, NBH_MAX_DEVICES);
+        return NBHIP_CAPABILITY_ERROR;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return NBHIP_SUCCESS;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+        nbh_set_error("hipGetSymbolAddress(scratch) failed on device %d: %s", device,
+                      hipGetErrorString(e));
+        return NBHIP_GPU_ERROR;
+    }
+    g_scratch_ptr[device] = static_cast<unsigned*>(p);
+    return NBHIP_SUCCESS;
+}
+
+/* Fill kernel (not hipMemsetAsync, so the call also works under capture). */
+__global__ __launch_bounds__(256) void k_fill_u32(unsigned* __restrict__ p, size_t n, unsigned v)
+{
+    for (size_t i = (size_t) blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t) gridDim.x * 256)
+        p[i] = v;
+}
+
+extern "C" unsigned nbhip_histogram_noalloc(nbhip_state* st, const nbhip_histogram_params* hp,
+                                            unsigned* counts, unsigned* totals)
+{
+    if (!st || !hp) return NBHIP_USER_ERROR;
+    if (hp->lambda_bins < 1 || hp->beta_bins < 1 || !(hp->lambda_end > hp->lambda_start) ||
+        !(hp->beta_end > hp->beta_start)) {
+        nbh_set_error("nbhip_histogram_noalloc: empty bin range");
+        return NBHIP_USER_ERROR;
+    }
+    const size_t nbin = (size_t) hp->lambda_bins * (size_t) hp->beta_bins;
+    const size_t need = (size_t) st->nsim * (nbin + 1);
+    if (nbin > NBHIP_NOALLOC_MAX_COUNTERS || need > NBHIP_NOALLOC_MAX_COUNTERS) {
+        nbh_set_error("nbhip_histogram_noalloc: %d simulations x (%zu bins + 1) = %zu counters "
+                      "exceed the scratch of %u",
+                      st->nsim, nbin, need, (unsigned) NBHIP_NOALLOC_MAX_COUNTERS);
+        return NBHIP_USER_ERROR;
+    }
+    NBH_CHECK(hipSetDevice(st->device));
+    unsigned* base = (st->device >= 0 && st->device < NBH_MAX_DEVICES)
+                         ? g_scratch_ptr[st->device] : nullptr;
+    if (!base) {
+        nbh_set_error("nbhip_histogram_noalloc: no scratch address for device %d", st->device);
+        return NBHIP_GPU_ERROR;
+    }
+    unsigned* d_counts = base;
+    unsigned* d_totals = base + (size_t) st->nsim * nbin;
+    /* the previous call's counts are still there: clear what this one uses */
+    const int gfill = (int) std::min<size_t>((need + 255) / 256, 1024);
+    hipLaunchKernelGGL(k_fill_u32, dim3(gfill), dim3(256), 0, st->stream, base, need, 0u);
+    nbh_histogram(st, hp, d_counts, d_totals);
+    NBH_CHECK(hipGetLastError());
+    if (counts)
+        NBH_CHECK(hipMemcpyAsync(counts, d_counts, (size_t) st->nsim * nbin * sizeof(unsigned),
+                                 hipMemcpyDeviceToHost, st->stream));
+    if (totals)
+        NBH_CHECK(hipMemcpyAsync(totals, d_totals, (size_t) st->nsim * sizeof(unsigned),
+                                 hipMemcpyDeviceToHost, st->stream));
+    NBH_CHECK(hipStreamSynchronize(st->stream));
+    return NBHIP_SUCCESS;
 }

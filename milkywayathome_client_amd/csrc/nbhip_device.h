@@ -19,14 +19,16 @@
  * power of two, x * 2^63/R is an exact scaling, so bit (62 - l) of u equals
  * the reference's comparison "midpoint <= x" at level l + 1 (nbSubIndex,
  * nbody/src/nbody_tree.c:36-52): every midpoint is an integer in this
- * fixed-point frame and the comparison is exact. */
-__device__ __forceinline__ unsigned long long nbh_fixed(double x, double key_scale)
+ * fixed-point frame and the comparison is exact.  The root-size rule allows
+ * |x| = R/2 exactly; x = +R/2 is on the upper side of every midpoint, so it
+ * clamps to all ones (the clamp comes before the integer add, which would
+ * overflow at t = 2^62), and x = -R/2 gives 0.  A NaN clamps to all ones. */
+__host__ __device__ __forceinline__ unsigned long long nbh_fixed(double x, double key_scale)
 {
-    double t = floor(x * key_scale);
-    long long it = (long long) t + (1LL << 62);
-    if (it < 0) it = 0;
-    if (it > 0x7FFFFFFFFFFFFFFFLL) it = 0x7FFFFFFFFFFFFFFFLL;
-    return (unsigned long long) it;
+    const double t = floor(x * key_scale);
+    if (!(t < 4611686018427387904.0)) return 0x7FFFFFFFFFFFFFFFull;   /* t >= 2^62 */
+    if (t < -4611686018427387904.0) return 0ull;
+    return (unsigned long long) ((long long) t + (1LL << 62));
 }
 
 /* Spread the low 21 bits of v so that bit i lands on bit 3i. */

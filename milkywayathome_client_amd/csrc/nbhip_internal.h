@@ -187,6 +187,8 @@ void nbh_histogram(nbhip_state* st, const nbhip_histogram_params* hp, unsigned* 
 void nbh_dispersion(nbhip_state* st, int off, int cnt, const nbhip_histogram_params* hp,
                     const nbhip_disp_params* dp, int* dbin, double* dval, double* part,
                     double* state, double* out);
+/* scratch of the allocation-free entry point (histogram.hip) */
+unsigned nbh_scratch_init(int device);
 /* launchers (adjoint.hip) */
 void nbh_adj_kick_end(nbhip_state* st, const double* a_next);
 void nbh_adj_force(nbhip_state* st);

@@ -110,7 +110,7 @@ EXPORTED = [
     "nbhip_batch_tree_info", "nbhip_set_graph", "nbhip_autotune", "nbhip_set_tune",
     "nbhip_adjoint_forward", "nbhip_adjoint_backward", "nbhip_gravity_vjp", "nbhip_set_ignore",
     "nbhip_histogram", "nbhip_orbit_tangent", "nbhip_restore", "nbhip_histogram_dispersion",
-    "nbhip_set_open_scale",
+    "nbhip_set_open_scale", "nbhip_batch_tree_download", "nbhip_histogram_noalloc",
 ]
 
 _LIB: Optional[C.CDLL] = None
@@ -140,6 +140,7 @@ def load_library() -> C.CDLL:
         "nbhip_steps": (u, [st, C.c_uint]),
         "nbhip_tree_info_get": (u, [st, C.POINTER(_TreeInfo)]),
         "nbhip_tree_download": (u, [st, _ip, _ip, _ip, _ip, _dp, _dp, _dp, _dp]),
+        "nbhip_batch_tree_download": (u, [st, C.c_int, _ip, _ip, _ip, _ip, _dp, _dp, _dp, _dp]),
         "nbhip_direct_accel": (u, [st, C.c_int, _ip, _dp]),
         "nbhip_walk_accel": (u, [st, _dp]),
         "nbhip_walk_counters": (u, [st, C.POINTER(C.c_ulonglong)]),
@@ -162,6 +163,8 @@ def load_library() -> C.CDLL:
         "nbhip_set_ignore": (u, [st, C.POINTER(C.c_ubyte)]),
         "nbhip_histogram": (u, [st, C.POINTER(_HistParams), C.POINTER(C.c_uint),
                                 C.POINTER(C.c_uint)]),
+        "nbhip_histogram_noalloc": (u, [st, C.POINTER(_HistParams), C.POINTER(C.c_uint),
+                                        C.POINTER(C.c_uint)]),
         "nbhip_adjoint_forward": (u, [st, C.c_uint, C.c_uint]),
         "nbhip_gravity_vjp": (u, [st, _dp, _dp, _dp]),
         "nbhip_adjoint_backward": (u, [st, _dp, _dp, _dp, _dp, _dp, _dp]),
@@ -637,6 +640,22 @@ class NBodyBatch(NBodyState):
         out = (C.c_uint * self.nsim)()
         self._check(self.lib.nbhip_batch_status(self._h, out))
         return [int(x) for x in out]
+
+    def tree(self, sim: int = 0) -> dict:
+        """The pre-order tree of simulation `sim` alone (skip and ref counted
+        from its own root and first body, as in its single run)."""
+        nn = self.sim_tree_info(sim).nnode
+        out = {
+            "kind": np.empty(nn, np.int32), "ref": np.empty(nn, np.int32),
+            "skip": np.empty(nn, np.int32), "level": np.empty(nn, np.int32),
+            "pos": np.empty((nn, 3)), "mass": np.empty(nn), "rcrit2": np.empty(nn),
+            "quad": np.empty((nn, 6)),
+        }
+        self._check(self.lib.nbhip_batch_tree_download(
+            self._h, int(sim), _ptr(out["kind"], C.c_int), _ptr(out["ref"], C.c_int),
+            _ptr(out["skip"], C.c_int), _ptr(out["level"], C.c_int), _ptr(out["pos"]),
+            _ptr(out["mass"]), _ptr(out["rcrit2"]), _ptr(out["quad"])))
+        return out
 
     def sim_tree_info(self, k: int) -> TreeInfo:
         ti = _TreeInfo()
