@@ -411,7 +411,8 @@ def main():
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--seed", type=int, default=34086709)
     ap.add_argument("--theta", type=float, default=0.5)
-    ap.add_argument("--precision", choices=["all", "both", "fp64", "mixed", "mixed_coherent"],
+    ap.add_argument("--precision", choices=["all", "both", "fp64", "mixed", "mixed_coherent",
+                                            "mixed_group"],
                     default="all",
                     help="all: fp64, mixed and mixed_coherent; both: fp64 and mixed")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -494,7 +495,7 @@ def main():
     if not results:
         print(json.dumps({"metric": METRIC, "error": "no GPU result (see stderr)"}))
         sys.exit(1)
-    mixed_modes = [k for k in ("mixed", "mixed_coherent") if k in results]
+    mixed_modes = [k for k in ("mixed", "mixed_coherent", "mixed_group") if k in results]
     head = (max(mixed_modes, key=lambda k: results[k]["body_steps_per_s"]) if mixed_modes
             else "fp64")
     r = results[head]

@@ -61,6 +61,18 @@ extern "C" {
  * opens a cell when any of them must, so each body's interaction list
  * refines its reference list (more accurate, not the reference's lists) */
 #define NBHIP_PRECISION_MIXED_COHERENT 2
+/* LDS-staged group walk: each wavefront takes 64 consecutive bodies in
+ * Morton order (split into at most four sub-groups where the order jumps
+ * between distant cells), tests the tree's nodes 64 at a time against the
+ * group's bounding box with the context's criterion (a node is accepted
+ * when the box's nearest point passes it, so every body's interaction list
+ * refines its reference list), stages the accepted nodes in LDS as FP32
+ * positions relative to the group's centre and runs a dense FP32
+ * interaction loop over them; each LDS batch is one FP32 partial sum added
+ * into FP64 with Kahan compensation.  More accurate than the reference
+ * lists; results depend on which bodies share a wavefront (not
+ * bit-identical across batching). */
+#define NBHIP_PRECISION_MIXED_GROUP 3
 
 /* The parts of Potential (nbody/include/nbody_potential_types.h) that the
  * supported components read. */
@@ -205,20 +217,22 @@ unsigned nbhip_batch_tree_download(nbhip_state* st, int sim, int* kind, int* ref
  * once per call). */
 unsigned nbhip_set_graph(nbhip_state* st, int enable);
 
-/* No counterpart in the reference.  Coherent mixed walk only
- * (NBHIP_PRECISION_MIXED_COHERENT): scale every cell's squared critical
- * radius by `scale` (>= 1) in the walk's records, so the walk opens more and
- * each body's interaction list refines its reference list further (more
- * accurate, more interactions).  Takes effect at the next tree build; 1 is
- * the default.  NBHIP_USER_ERROR for another precision or a scale outside
- * [1, 1e6]. */
+/* No counterpart in the reference.  Coherent and group mixed walks only
+ * (NBHIP_PRECISION_MIXED_COHERENT, NBHIP_PRECISION_MIXED_GROUP): scale every
+ * cell's squared critical radius by `scale` (>= 1) in the walk's records,
+ * so the walk opens more and each body's interaction list refines its
+ * reference list further (more accurate, more interactions).  Takes effect
+ * at the next tree build; 1 is the default.  NBHIP_USER_ERROR for another
+ * precision or a scale outside [1, 1e6]. */
 unsigned nbhip_set_open_scale(nbhip_state* st, double scale);
 
-/* Walk launch parameters: workgroup size (64, 128, 256 threads) and, for the
- * mixed walk, the node block (a power of two, in pre-order node indices)
- * over which a body's FP32 partial sum runs before the FP64 Kahan add.
- * The block depends only on a body's own interaction list, so results do not
- * change with the workgroup size or with batching. */
+/* Walk launch parameters: workgroup size (64, 128, 256 threads) and
+ * `batch`: for the mixed walks the node block (a power of two, in pre-order
+ * node indices) over which a body's FP32 partial sum runs before the FP64
+ * Kahan add -- it depends only on a body's own interaction list, so results
+ * do not change with the workgroup size or with batching; for the group
+ * walk the depth of its LDS interaction lists (64, 128 or 256 entries; one
+ * list is one FP32 partial sum). */
 typedef struct nbhip_tune {
     int block;
     int batch;

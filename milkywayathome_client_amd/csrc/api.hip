@@ -65,7 +65,8 @@ static unsigned nbh_validate(const nbhip_ctx* c)
         }
     }
     if (c->precision != NBHIP_PRECISION_FP64 && c->precision != NBHIP_PRECISION_MIXED &&
-        c->precision != NBHIP_PRECISION_MIXED_COHERENT) {
+        c->precision != NBHIP_PRECISION_MIXED_COHERENT &&
+        c->precision != NBHIP_PRECISION_MIXED_GROUP) {
         nbh_set_error("invalid precision %d", c->precision);
         return NBHIP_USER_ERROR;
     }
@@ -99,7 +100,7 @@ static void nbh_free_all(nbhip_state* st)
                      st->bhi,  st->bdelta, st->bparent, st->node,  st->nquad,  st->nmid,
                      st->nlevel, st->nparent, st->nsim_node, st->level_list, st->nodef,
                      st->scal, st->sims, st->ignore, st->hist_buf, st->disp_buf, st->pos4,
-                     st->exact_part };
+                     st->exact_part, st->nodeg, st->nkid, st->kbase };
     for (void* p : ptrs)
         if (p) (void) hipFree(p);
     if (st->graph_exec) (void) hipGraphExecDestroy(st->graph_exec);
@@ -196,11 +197,17 @@ static unsigned nbh_create(const nbhip_ctx* ctx, int device, int nsim, const int
     A(hist, 256 * (size_t) st->sort_blocks);
     A(dtot, 256);
     A(pos4, n); A(spos, n); A(simid_s, n); A(su, 3 * n); A(dadj, n); A(cincl, n);
-    A(scan_part, 4096);
+    A(scan_part, std::max<size_t>(4096, (nc + NBH_SCAN_TILE - 1) / NBH_SCAN_TILE));
     A(blo, n); A(bhi, n); A(bdelta, n); A(bparent, 2 * n);
     A(node, nc); A(nquad, 6 * nc); A(nmid, 3 * nc); A(nlevel, nc); A(nparent, nc);
     A(nsim_node, nc); A(level_list, nc);
-    if (ctx->precision != NBHIP_PRECISION_FP64) A(nodef, nc + 1);   /* + 1: walk prefetch */
+    if (ctx->precision == NBHIP_PRECISION_MIXED || ctx->precision == NBHIP_PRECISION_MIXED_COHERENT)
+        A(nodef, nc + 1);   /* + 1: walk prefetch */
+    if (ctx->precision == NBHIP_PRECISION_MIXED_GROUP) {
+        A(nodeg, nc + (size_t) nsim);
+        A(nkid, nc);
+        A(kbase, nc);
+    }
     if (ctx->criterion == NBHIP_EXACT) A(exact_part, 4 * (size_t) st->exact_seg * n);
     A(scal, 1);
     A(sims, (size_t) nsim);
@@ -793,8 +800,9 @@ extern "C" unsigned nbhip_set_tune(nbhip_state* st, const nbhip_tune* tune)
 extern "C" unsigned nbhip_set_open_scale(nbhip_state* st, double scale)
 {
     if (!st) return NBHIP_USER_ERROR;
-    if (st->ctx.precision != NBHIP_PRECISION_MIXED_COHERENT) {
-        nbh_set_error("nbhip_set_open_scale: only the coherent mixed walk opens more");
+    if (st->ctx.precision != NBHIP_PRECISION_MIXED_COHERENT &&
+        st->ctx.precision != NBHIP_PRECISION_MIXED_GROUP) {
+        nbh_set_error("nbhip_set_open_scale: only the coherent and group mixed walks open more");
         return NBHIP_USER_ERROR;
     }
     if (!(scale >= 1.0) || !(scale <= 1.0e6)) {

@@ -39,6 +39,20 @@ struct __attribute__((aligned(16))) NodeF {
     float q[6];
 };
 
+/* Record of the group walk's table, in which every cell's children occupy
+ * one contiguous block (slots 0 .. nsim-1 hold the simulations' roots).
+ * Position FP64 (displacements are formed relative to the group in FP64 and
+ * rounded once); rcrit2 is the squared critical radius times the open scale,
+ * rounded towards +infinity to FP32 (so the group test never accepts what
+ * the FP64 criterion opens), -1 for bodies.  Cells: link = first child slot,
+ * count = children (1..8); bodies: link = sorted body index, count = 0. */
+struct __attribute__((aligned(16))) NodeG {
+    double x, y, z;
+    float m, rcrit2;
+    int link, count;
+    float q[6];
+};
+
 /* Exact criterion: the sources of a simulation of n bodies are summed in
  * this many consecutive segments (one per grid row), 1 below 4096 bodies. */
 __host__ __device__ inline int nbh_exact_segments(int n)
@@ -79,7 +93,8 @@ struct DevScalars {
 /* Walk tuning picked by the autotuner (nbhip_autotune). */
 struct WalkTune {
     int block;                     /* threads per workgroup: 64, 128 or 256 */
-    int batch;                     /* mixed walk: node block per FP32 partial sum (power of 2) */
+    int batch;                     /* mixed walk: node block per FP32 partial sum (power of 2);
+                                      group walk: LDS interaction-list depth (64, 128, 256) */
 };
 
 #define NBH_SORT_TILE 2048   /* keys per radix-sort block: 256 threads x 8 */
@@ -137,6 +152,9 @@ struct nbhip_state {
     int *nlevel, *nparent, *nsim_node;
     int *level_list;              /* cell pre-order indices grouped by level */
     NodeF *nodef;                 /* mixed-precision walk records (precision MIXED) */
+    NodeG *nodeg;                 /* group walk table (precision MIXED_GROUP): nsim + node_cap */
+    unsigned *nkid;               /* children per node (group walk table build) */
+    unsigned *kbase;              /* exclusive scan of nkid: each cell's child block */
     double *exact_part;           /* Exact criterion: per-segment partial sums */
     int exact_seg;                /* Exact criterion: most segments of any simulation */
 
@@ -181,6 +199,10 @@ void nbh_build(nbhip_state* st, int graph_safe);
 void nbh_walk(nbhip_state* st, int kick, int count, double* out_ax, double* out_ay,
               double* out_az, int self_only, const WalkTune* tune);
 void nbh_direct(nbhip_state* st, int nsample, const int* d_idx, double* d_acc);
+/* launcher (walk_group.hip) */
+struct WalkOut;
+void nbh_walk_group(nbhip_state* st, const WalkOut& o, double* out_ax, double* out_ay,
+                    double* out_az, const WalkTune& t, int count);
 /* launcher (histogram.hip) */
 void nbh_histogram(nbhip_state* st, const nbhip_histogram_params* hp, unsigned* d_counts,
                    unsigned* d_totals);

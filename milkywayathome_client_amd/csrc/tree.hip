@@ -579,6 +579,71 @@ __global__ __launch_bounds__(1024) void k_scan_add(unsigned* __restrict__ data,
         if (base + k < M) data[base + k] += add;
 }
 
+/* Exclusive scan of the children per node over the node table (length
+ * scal->nnode, up to the node capacity): each cell's child block in the
+ * group walk table.  Tiles of NBH_SCAN_TILE entries, then the tile totals
+ * (one workgroup, 4096 at a time with a running carry), then the tile
+ * offsets added back. */
+__global__ __launch_bounds__(1024) void k_nscan_tiles(const unsigned* __restrict__ in,
+                                                      unsigned* __restrict__ out,
+                                                      unsigned* __restrict__ part,
+                                                      const DevScalars* __restrict__ scal)
+{
+    __shared__ unsigned wsum[16];
+    const int nn = scal->nnode;
+    if (blockIdx.x * NBH_SCAN_TILE >= nn) return;
+    const int base = blockIdx.x * NBH_SCAN_TILE + threadIdx.x * 4;
+    unsigned v[4];
+    #pragma unroll
+    for (int k = 0; k < 4; ++k) v[k] = (base + k < nn) ? in[base + k] : 0u;
+    const unsigned sum = v[0] + v[1] + v[2] + v[3];
+    unsigned tot;
+    unsigned ex = nbh_block1024_excl(sum, wsum, &tot);
+    #pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (base + k < nn) out[base + k] = ex;
+        ex += v[k];
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(1024) void k_nscan_part(unsigned* __restrict__ part,
+                                                     const DevScalars* __restrict__ scal)
+{
+    __shared__ unsigned wsum[16];
+    const int nb = (scal->nnode + NBH_SCAN_TILE - 1) / NBH_SCAN_TILE;
+    unsigned carry = 0;
+    for (int c0 = 0; c0 < nb; c0 += NBH_SCAN_TILE) {
+        const int base = c0 + threadIdx.x * 4;
+        unsigned v[4];
+        #pragma unroll
+        for (int k = 0; k < 4; ++k) v[k] = (base + k < nb) ? part[base + k] : 0u;
+        const unsigned sum = v[0] + v[1] + v[2] + v[3];
+        unsigned tot;
+        unsigned ex = carry + nbh_block1024_excl(sum, wsum, &tot);
+        #pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (base + k < nb) part[base + k] = ex;      /* exclusive tile offsets */
+            ex += v[k];
+        }
+        carry += tot;
+        __syncthreads();          /* wsum is reused by the next chunk */
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_nscan_add(unsigned* __restrict__ out,
+                                                    const unsigned* __restrict__ part,
+                                                    const DevScalars* __restrict__ scal)
+{
+    const int nn = scal->nnode;
+    if (blockIdx.x == 0 || blockIdx.x * NBH_SCAN_TILE >= nn) return;
+    const int base = blockIdx.x * NBH_SCAN_TILE + threadIdx.x * 4;
+    const unsigned add = part[blockIdx.x];
+    #pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (base + k < nn) out[base + k] += add;
+}
+
 /* ---------------------------------------------------------------------- */
 /* node counts, per-simulation segments                                    */
 /* ---------------------------------------------------------------------- */
@@ -589,6 +654,7 @@ __global__ __launch_bounds__(1024) void k_scan_add(unsigned* __restrict__ data,
 __global__ __launch_bounds__(256) void k_sim_roots(const int* __restrict__ simid_s,
                                                    const unsigned* __restrict__ cincl,
                                                    int node_cap, SimTab* __restrict__ sims,
+                                                   unsigned* __restrict__ nkid,
                                                    DevScalars* __restrict__ scal)
 {
     const int a = blockIdx.x * 256 + threadIdx.x;
@@ -601,6 +667,13 @@ __global__ __launch_bounds__(256) void k_sim_roots(const int* __restrict__ simid
         if (nn > node_cap) scal->status |= NBHIP_CELL_OVERFLOW_ERROR;
     }
     if (a >= scal->nmassive) return;
+    if (nkid) {
+        /* the group walk's child counters of body a's new cells and its leaf
+         * (pre-order a + cincl[a-1] .. a + cincl[a]), counted by the emission */
+        const int lo = a + (a > 0 ? (int) cincl[a - 1] : 0);
+        const int hi = a + (int) cincl[a];
+        for (int i = lo; i <= hi && i < node_cap; ++i) nkid[i] = 0u;
+    }
     if (!nbh_first_of_sim(simid_s, a)) return;
     const int k = simid_s[a];
     sims[k].first = a;
@@ -782,7 +855,8 @@ __device__ __forceinline__ void nbh_emit_cells_body(
     const unsigned* __restrict__ cincl, const unsigned long long* __restrict__ su,
     const int* __restrict__ simid_s, int n, NodeW* __restrict__ node,
     double* __restrict__ nmid, int* __restrict__ nlevel, int* __restrict__ nparent,
-    int* __restrict__ nsim, SimTab* __restrict__ sims, DevScalars* __restrict__ scal, int* lh)
+    int* __restrict__ nsim, SimTab* __restrict__ sims, unsigned* __restrict__ nkid,
+    DevScalars* __restrict__ scal, int* lh)
 {
     const int k = blockIdx.x * 256 + threadIdx.x;
     const int M = scal->nmassive;
@@ -814,8 +888,10 @@ __device__ __forceinline__ void nbh_emit_cells_body(
             atomicOr(&scal->status, NBHIP_ERROR);
             return;
         }
-        nbh_emit_cell(pre, l, next, (l == Lp + 1) ? top_parent : pre - 1, sim, R, ux, uy, uz,
-                      node, nmid, nlevel, nparent, nsim, sims);
+        const int par = (l == Lp + 1) ? top_parent : pre - 1;
+        nbh_emit_cell(pre, l, next, par, sim, R, ux, uy, uz, node, nmid, nlevel, nparent, nsim,
+                      sims);
+        if (nkid && par >= 0) atomicAdd(&nkid[par], 1u);
         atomicAdd(&lh[l], 1);
     }
 }
@@ -834,13 +910,14 @@ __global__ __launch_bounds__(256) void k_emit_cells(const int* __restrict__ blo,
                                                     int* __restrict__ nparent,
                                                     int* __restrict__ nsim,
                                                     SimTab* __restrict__ sims,
+                                                    unsigned* __restrict__ nkid,
                                                     DevScalars* __restrict__ scal)
 {
     __shared__ int lh[64];
     if (threadIdx.x < 64) lh[threadIdx.x] = 0;
     __syncthreads();
     nbh_emit_cells_body(blo, bhi, bdelta, bparent, dadj, cincl, su, simid_s, n, node, nmid,
-                        nlevel, nparent, nsim, sims, scal, lh);
+                        nlevel, nparent, nsim, sims, nkid, scal, lh);
     __syncthreads();
     if (threadIdx.x < 64 && lh[threadIdx.x])
         atomicAdd(&scal->level_count[threadIdx.x], lh[threadIdx.x]);
@@ -862,6 +939,7 @@ __global__ __launch_bounds__(256) void k_emit_leaves(const int* __restrict__ blo
                                                      int* __restrict__ nsim,
                                                      SimTab* __restrict__ sims,
                                                      NodeF* __restrict__ nodef,
+                                                     unsigned* __restrict__ nkid,
                                                      DevScalars* __restrict__ scal)
 {
     const int s = blockIdx.x * 256 + threadIdx.x;
@@ -886,6 +964,7 @@ __global__ __launch_bounds__(256) void k_emit_leaves(const int* __restrict__ blo
         atomicOr(&scal->status, NBHIP_ERROR);
         return;
     }
+    if (nkid) atomicAdd(&nkid[parent], 1u);
     const double4 p = spos[s];
     NodeW b;
     b.x = p.x; b.y = p.y; b.z = p.z; b.m = p.w;
@@ -973,6 +1052,41 @@ __global__ __launch_bounds__(256) void k_level_scatter(const int* __restrict__ n
  * children of a level-l cell are bodies or level-(l+1) cells, finished by an
  * earlier launch.  Grid-stride over the level's cells, so a fixed launch
  * shape serves any count (graph capture). */
+/* The group walk's table (precision MIXED_GROUP; T is null otherwise):
+ * every cell writes its children's records into its child block
+ * (slot nsim + kbase[cell] + rank, rank in octant order), and a root also
+ * writes its own record into slot sim. */
+struct GroupTab {
+    NodeG* T;
+    const unsigned* nkid;
+    const unsigned* kbase;
+    int nsim;
+};
+
+/* Group-walk record of node c (a finished cell or a body). */
+__device__ __forceinline__ NodeG nbh_group_record(const NodeW& w, int c,
+                                                  const double* __restrict__ nquad,
+                                                  const GroupTab& g, double rc_scale, bool quad)
+{
+    NodeG r;
+    r.x = w.x;
+    r.y = w.y;
+    r.z = w.z;
+    r.m = (float) w.m;
+    if (w.body >= 0) {
+        r.rcrit2 = -1.0f;
+        r.link = w.body;
+        r.count = 0;
+        for (int k = 0; k < 6; ++k) r.q[k] = 0.0f;
+    } else {
+        r.rcrit2 = __double2float_ru(w.rcrit2 * rc_scale);
+        r.link = g.nsim + (int) g.kbase[c];
+        r.count = (int) g.nkid[c];
+        for (int k = 0; k < 6; ++k) r.q[k] = quad ? (float) nquad[6 * (size_t) c + k] : 0.0f;
+    }
+    return r;
+}
+
 /* One cell's summary (see below); false on a broken Next link. */
 __device__ __forceinline__ bool nbh_summarize_cell(const int* __restrict__ level_list, int level,
                                                    int i, NodeW* __restrict__ node,
@@ -981,6 +1095,7 @@ __device__ __forceinline__ bool nbh_summarize_cell(const int* __restrict__ level
                                                    const int* __restrict__ nsim,
                                                    SimTab* __restrict__ sims,
                                                    NodeF* __restrict__ nodef,
+                                                   const GroupTab& g,
                                                    const nbhip_ctx& ctx, double rc_scale,
                                                    DevScalars* scal, int nn)
 {
@@ -991,6 +1106,9 @@ __device__ __forceinline__ bool nbh_summarize_cell(const int* __restrict__ level
     if (p < 0 || end <= p || end > nn) {
         return false;
     }
+    const int nk = g.T ? (int) g.nkid[p] : 0;
+    const int kb = g.T ? g.nsim + (int) g.kbase[p] : 0;
+    int rank = 0;
     double mass = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
     for (int c = p + 1; c < end;) {
         const NodeW ch = node[c];
@@ -1001,8 +1119,14 @@ __device__ __forceinline__ bool nbh_summarize_cell(const int* __restrict__ level
         if (ch.next <= c || ch.next > end) {
             return false;
         }
+        if (g.T) {
+            if (rank >= nk) return false;
+            g.T[kb + rank] = nbh_group_record(ch, c, nquad, g, rc_scale, ctx.use_quad);
+        }
+        ++rank;
         c = ch.next;
     }
+    if (g.T && rank != nk) return false;
     const double mx = nmid[3 * p + 0], my = nmid[3 * p + 1], mz = nmid[3 * p + 2];
     if (mass > 0.0) {
         cx /= mass;
@@ -1061,6 +1185,7 @@ __device__ __forceinline__ bool nbh_summarize_cell(const int* __restrict__ level
     w.next = end;
     w.body = -1;
     node[p] = w;
+    if (g.T && level == 0) g.T[sim] = nbh_group_record(w, p, nquad, g, rc_scale, ctx.use_quad);
     if (nodef) {
         /* the mixed walk's record: FP64 position, the rest in FP32 */
         NodeF& f = nodef[p];
@@ -1083,6 +1208,7 @@ __global__ __launch_bounds__(256) void k_summarize_level(const int* __restrict__
                                                          const int* __restrict__ nsim,
                                                          SimTab* __restrict__ sims,
                                                          NodeF* __restrict__ nodef,
+                                                         GroupTab g,
                                                          nbhip_ctx ctx, double rc_scale,
                                                          DevScalars* scal)
 {
@@ -1090,8 +1216,8 @@ __global__ __launch_bounds__(256) void k_summarize_level(const int* __restrict__
     const int cnt = scal->level_count[level];
     const int nn = scal->nnode;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
-        if (!nbh_summarize_cell(level_list, level, i, node, nquad, nmid, nsim, sims, nodef, ctx,
-                                rc_scale, scal, nn)) {
+        if (!nbh_summarize_cell(level_list, level, i, node, nquad, nmid, nsim, sims, nodef, g,
+                                ctx, rc_scale, scal, nn)) {
             atomicOr(&scal->status, NBHIP_ERROR);
             return;
         }
@@ -1107,8 +1233,8 @@ __global__ __launch_bounds__(256) void k_summarize_level(const int* __restrict__
 __global__ __launch_bounds__(NBH_TAIL_THREADS) void k_summarize_tail(
     const int* __restrict__ level_list, int lo, NodeW* __restrict__ node,
     double* __restrict__ nquad, const double* __restrict__ nmid, const int* __restrict__ nsim,
-    SimTab* __restrict__ sims, NodeF* __restrict__ nodef, nbhip_ctx ctx, double rc_scale,
-    DevScalars* scal)
+    SimTab* __restrict__ sims, NodeF* __restrict__ nodef, GroupTab g, nbhip_ctx ctx,
+    double rc_scale, DevScalars* scal)
 {
     const int nn = scal->nnode;
     for (int level = 63; level >= lo; --level) {
@@ -1118,7 +1244,7 @@ __global__ __launch_bounds__(NBH_TAIL_THREADS) void k_summarize_tail(
         if (cnt == 0) continue;
         for (int i = threadIdx.x; i < cnt; i += NBH_TAIL_THREADS)
             if (!nbh_summarize_cell(level_list, level, i, node, nquad, nmid, nsim, sims, nodef,
-                                    ctx, rc_scale, scal, nn))
+                                    g, ctx, rc_scale, scal, nn))
                 atomicOr(&scal->status, NBHIP_ERROR);
         __syncthreads();
     }
@@ -1185,17 +1311,27 @@ void nbh_build(nbhip_state* st, int graph_safe)
     hipLaunchKernelGGL(k_scan_add, dim3(nscan), dim3(1024), 0, s, st->cincl, st->scan_part,
                        st->scal);
     hipLaunchKernelGGL(k_sim_roots, dim3(g256), dim3(256), 0, s, st->simid_s, st->cincl,
-                       st->node_cap, st->sims, st->scal);
+                       st->node_cap, st->sims, st->nkid, st->scal);
     hipLaunchKernelGGL(k_sim_ends, dim3(1), dim3(256), 0, s, st->nsim, st->sims, st->scal);
     hipLaunchKernelGGL(k_karras, dim3(g256), dim3(256), 0, s, n, st->su, st->simid_s, st->dadj,
                        st->blo, st->bhi, st->bdelta, st->bparent, st->scal);
     hipLaunchKernelGGL(k_emit_cells, dim3(g256), dim3(256), 0, s, st->blo, st->bhi, st->bdelta,
                        st->bparent, st->dadj, st->cincl, st->su, st->simid_s, n, st->node,
-                       st->nmid, st->nlevel, st->nparent, st->nsim_node, st->sims, st->scal);
+                       st->nmid, st->nlevel, st->nparent, st->nsim_node, st->sims, st->nkid,
+                       st->scal);
     hipLaunchKernelGGL(k_emit_leaves, dim3(g256), dim3(256), 0, s, st->blo, st->bdelta,
                        st->bparent, st->dadj, st->cincl, st->spos, st->simid_s, st->node,
                        st->nmid, st->nlevel, st->nparent, st->nsim_node, st->sims, st->nodef,
-                       st->scal);
+                       st->nkid, st->scal);
+    if (st->nodeg) {
+        /* each cell's child block in the group walk table */
+        const int nt = nbh_blocks(st->node_cap, NBH_SCAN_TILE);
+        hipLaunchKernelGGL(k_nscan_tiles, dim3(nt), dim3(1024), 0, s, st->nkid, st->kbase,
+                           st->scan_part, st->scal);
+        hipLaunchKernelGGL(k_nscan_part, dim3(1), dim3(1024), 0, s, st->scan_part, st->scal);
+        hipLaunchKernelGGL(k_nscan_add, dim3(nt), dim3(1024), 0, s, st->kbase, st->scan_part,
+                           st->scal);
+    }
     /* the per-level cell counts are final once the cells are emitted: read
      * them back (eager builds) while the level lists are being built */
     const bool readback = !graph_safe &&
@@ -1216,13 +1352,15 @@ void nbh_build(nbhip_state* st, int graph_safe)
     }
     st->h_levels_valid = have ? 1 : 0;
     if (have) memcpy(st->h_level_count, counts, sizeof(counts));
+    const GroupTab gt = { st->nodeg, st->nkid, st->kbase, st->nsim };
     /* 64-lane workgroups: a level's cells spread over more CUs */
     const int gmax = nbh_blocks(n, 64 * 8) + 1;
     if (!have) {
         /* without the counts: the deep levels in one single-workgroup launch */
         hipLaunchKernelGGL(k_summarize_tail, dim3(1), dim3(NBH_TAIL_THREADS), 0, s,
                            st->level_list, NBH_TAIL_LEVEL, st->node, st->nquad, st->nmid,
-                           st->nsim_node, st->sims, st->nodef, st->ctx, st->open_scale, st->scal);
+                           st->nsim_node, st->sims, st->nodef, gt, st->ctx, st->open_scale,
+                           st->scal);
         top = NBH_TAIL_LEVEL - 1;
     }
     for (int l = top; l >= 0; --l) {
@@ -1235,7 +1373,7 @@ void nbh_build(nbhip_state* st, int graph_safe)
         }
         hipLaunchKernelGGL(k_summarize_level, dim3(g), dim3(64), 0, s, st->level_list, l,
                            st->node, st->nquad, st->nmid, st->nsim_node, st->sims, st->nodef,
-                           st->ctx, st->open_scale, st->scal);
+                           gt, st->ctx, st->open_scale, st->scal);
     }
     nbh_mark(st, 4);
 }

@@ -13,62 +13,7 @@
  * reference order.  Node records are wave-uniform, so they arrive through
  * scalar loads; only interactions spend vector ALU.
  */
-#include "nbhip_device.h"
-
-/* Output side of one walk launch. */
-struct WalkOut {
-    int n;
-    int kick;
-    int use_pot;
-    int multi;          /* several simulations: lanes park at their own end */
-    int all_eps2_pos;   /* every simulation has eps2 > 0 */
-    int batch_shift;    /* mixed walk: log2 of the node block per FP32 partial sum */
-    double dtHalf;
-    nbhip_potential pot;
-};
-
-/* Closing half of the force evaluation shared by both walks: external
- * potential (nbMapForceBody, nbody/src/nbody_grav.c:164-182), scatter to
- * the original body order, optional closing half-kick (advanceVelocities,
- * nbody/src/nbody_plain.c:301-315). */
-__device__ __forceinline__ void nbh_walk_finish(const WalkOut& o, double px, double py,
-                                                double pz, double ax, double ay, double az,
-                                                unsigned j, double* __restrict__ oax,
-                                                double* __restrict__ oay,
-                                                double* __restrict__ oaz,
-                                                double* __restrict__ vx, double* __restrict__ vy,
-                                                double* __restrict__ vz)
-{
-    if (o.use_pot) {
-        double ex, ey, ez;
-        nbh_ext_accel(o.pot, px, py, pz, ex, ey, ez);
-        ax = ax + ex;
-        ay = ay + ey;
-        az = az + ez;
-    }
-    oax[j] = ax;
-    oay[j] = ay;
-    oaz[j] = az;
-    if (o.kick) {
-        vx[j] = vx[j] + o.dtHalf * (ax + 0.0);
-        vy[j] = vy[j] + o.dtHalf * (ay + 0.0);
-        vz[j] = vz[j] + o.dtHalf * (az + 0.0);
-    }
-}
-
-__device__ __forceinline__ void nbh_walk_counters(DevScalars* scal, unsigned nbb, unsigned nbc,
-                                                  unsigned nop, unsigned steps)
-{
-    unsigned long long c0 = nbh_wave_sum_u64(nbb);
-    unsigned long long c1 = nbh_wave_sum_u64(nbc);
-    unsigned long long c2 = nbh_wave_sum_u64(nop);
-    if (__lane_id() == 0) {
-        atomicAdd(&scal->counters[0], c0);
-        atomicAdd(&scal->counters[1], c1);
-        atomicAdd(&scal->counters[2], c2);
-        atomicAdd(&scal->counters[3], (unsigned long long) steps);
-    }
-}
+#include "nbhip_walk.h"
 
 struct Quad6 {
     double q[6];
@@ -201,13 +146,6 @@ __global__ __launch_bounds__(BS) void k_walk64(const NodeW* __restrict__ node,
  * a node when any lane's criterion says open and otherwise every lane of the
  * node's simulation interacts with it (each lane's list then refines its
  * reference list: every accepted node passes that lane's own test). */
-__device__ __forceinline__ void nbh_kahan_add(double& sum, double& comp, double v)
-{
-    const double y = v - comp;
-    const double t = sum + y;
-    comp = (t - sum) - y;
-    sum = t;
-}
 
 template <int BS, bool QUAD, bool COUNT, bool COH, bool MULTI, bool EPS0>
 __global__ __launch_bounds__(BS) void k_walk32(const NodeF* __restrict__ node,
@@ -427,6 +365,10 @@ void nbh_walk(nbhip_state* st, int kick, int count, double* oax, double* oay, do
     o.batch_shift = bsh;
     o.dtHalf = 0.5 * c.timestep;
     o.pot = c.pot;
+    if (c.precision == NBHIP_PRECISION_MIXED_GROUP) {
+        nbh_walk_group(st, o, oax, oay, oaz, t, count);
+        return;
+    }
     const int bs = (t.block == 64 || t.block == 128) ? t.block : 256;
     dim3 grid(nbh_blocks(st->n, bs)), block(bs);
     if (c.precision != NBHIP_PRECISION_FP64) {

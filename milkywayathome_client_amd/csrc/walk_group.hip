@@ -1,0 +1,393 @@
+/*
+ * LDS-staged group walk (precision NBHIP_PRECISION_MIXED_GROUP) on gfx950.
+ *
+ * A wavefront owns 64 consecutive bodies in Morton order.  Where the order
+ * jumps between distant cells (the weakest common-prefix link of the wave
+ * lies three or more levels above the wave's median link; applied twice, so
+ * at most four parts) and where a batch changes simulation, the wave is cut
+ * into sub-groups, walked one after another.  For a sub-group:
+ *
+ *  1. its bounding box is reduced across the member lanes (FP64);
+ *  2. the tree is walked breadth-first 64 nodes at a time: the frontier is a
+ *     stack of child ranges in LDS (each cell's children occupy one block of
+ *     the walk table, so an opened cell pushes one entry), and lane l takes
+ *     child (l mod 8) of entry (l / 8);
+ *  3. a node is accepted when the box's nearest point passes the context's
+ *     opening test (squared distance >= the record's rcrit2, which is the
+ *     reference criterion's value rounded up), which implies every member
+ *     body's own test (nbGravity, nbody/src/nbody_grav.c:42-124): each
+ *     body's interaction list refines its reference list.  Bodies are always
+ *     accepted; opened cells push their children;
+ *  4. accepted nodes are appended to two LDS lists (cells with quadrupole,
+ *     bodies) as FP32 positions relative to the group's centre (one FP64
+ *     subtraction rounded once);
+ *  5. a full list is drained by a dense loop in which every member lane
+ *     interacts with every entry (monopole + quadrupole with the reference's
+ *     softened expressions, nbody_grav.c:53-99, in FP32 with v_rsq_f32); one
+ *     drained list is one FP32 partial sum, added into FP64 with Kahan
+ *     compensation.
+ *
+ * The tests and the table are FP64 and deterministic, so a body's list does
+ * not depend on timing, and repeated runs are bit-identical.
+ */
+#include "nbhip_walk.h"
+
+#define NBH_GW_FCAP 1024   /* range-stack entries per wavefront */
+/* A depth-first descent pushes at most 7 entries more than it pops per
+ * level; the stack keeps this much room below its capacity for the deepest
+ * tree (63 levels), so throttled pops can always make progress. */
+#define NBH_GW_SLACK 448
+#define NBH_GW_SPLIT_T 3   /* split when min link level < median link level - 3 */
+
+__device__ __forceinline__ double nbh_uniform_d(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readfirstlane((int) (unsigned) (b & 0xffffffffll));
+    const int hi = __builtin_amdgcn_readfirstlane((int) (b >> 32));
+    return __longlong_as_double(((long long) hi << 32) | (unsigned) lo);
+}
+
+__device__ __forceinline__ double nbh_wave_min_d(double v)
+{
+    for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, 64));
+    return nbh_uniform_d(v);
+}
+
+__device__ __forceinline__ double nbh_wave_max_d(double v)
+{
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    return nbh_uniform_d(v);
+}
+
+__device__ __forceinline__ int nbh_wave_min_u(int v)
+{
+    return __builtin_amdgcn_readfirstlane(nbh_wave_min_i(v));
+}
+
+__device__ __forceinline__ unsigned nbh_mbcnt(unsigned long long m)
+{
+    return __builtin_amdgcn_mbcnt_hi((unsigned) (m >> 32),
+                                     __builtin_amdgcn_mbcnt_lo((unsigned) m, 0u));
+}
+
+/* Cuts of one wave: bit i set = a sub-group ends at lane i.  `lvl` is the
+ * level of the common prefix of this lane's body and the next lane's (-1
+ * where the next lane is in another simulation, massless, invalid or past
+ * the wave).  Runs between hard cuts are split at their weakest link (first
+ * minimum) when that link's level lies NBH_GW_SPLIT_T or more levels below
+ * the lower median of the run's link levels; twice. */
+__device__ __forceinline__ unsigned long long nbh_group_cuts(unsigned long long hard, int lvl,
+                                                             int lane)
+{
+    unsigned long long cuts = hard;
+    for (int pass = 0; pass < 2; ++pass) {
+        unsigned long long next = cuts;
+        int a = 0;
+        unsigned long long rest = cuts;
+        while (rest) {
+            const int e = __builtin_ctzll(rest);      /* run [a, e] */
+            rest &= rest - 1;
+            const int nl = e - a;                      /* links a .. e-1 */
+            if (nl >= 2) {
+                const bool in = lane >= a && lane < e;
+                const bool ok = __builtin_amdgcn_ballot_w64(in && lvl >= 0) ==
+                                (((~0ull) >> (64 - nl)) << a);
+                if (ok) {
+                    const int key = nbh_wave_min_u(in ? ((lvl << 6) | lane) : 0x7fffffff);
+                    const int lmin = key >> 6, j = key & 63;
+                    /* lower median: least L with #{lvl <= L} >= ceil(nl / 2) */
+                    const int need = (nl + 1) >> 1;
+                    int lo = 0, hi = 63;
+                    while (lo < hi) {
+                        const int mid = (lo + hi) >> 1;
+                        const int c = __popcll(__builtin_amdgcn_ballot_w64(in && lvl <= mid));
+                        if (c >= need) hi = mid; else lo = mid + 1;
+                    }
+                    if (lmin < lo - NBH_GW_SPLIT_T) next |= 1ull << j;
+                }
+            }
+            a = e + 1;
+        }
+        cuts = next;
+    }
+    return cuts;
+}
+
+template <int WPB, int LC, bool QUAD, bool EPS0, bool COUNT>
+__global__ __launch_bounds__(64 * WPB) void k_walk_group(const NodeG* __restrict__ T,
+                                                         const double4* __restrict__ spos,
+                                                         const unsigned* __restrict__ perm,
+                                                         const int* __restrict__ simid_s,
+                                                         const int* __restrict__ dadj,
+                                                         SimTab* __restrict__ sims,
+                                                         double* __restrict__ oax,
+                                                         double* __restrict__ oay,
+                                                         double* __restrict__ oaz,
+                                                         double* __restrict__ vx,
+                                                         double* __restrict__ vy,
+                                                         double* __restrict__ vz,
+                                                         DevScalars* __restrict__ scal,
+                                                         WalkOut o)
+{
+    __shared__ int s_fr[WPB][NBH_GW_FCAP];
+    __shared__ float4 s_c0[WPB][LC];     /* cells: x y z (relative) m */
+    __shared__ float4 s_c1[WPB][LC];     /* cells: q xx xy xz yy */
+    __shared__ float2 s_c2[WPB][LC];     /* cells: q yz zz */
+    __shared__ float4 s_b[WPB][LC];      /* bodies: x y z (relative) m */
+
+    if (scal->status & (NBHIP_CELL_OVERFLOW_ERROR | NBHIP_ERROR)) return;
+    const int w = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    int* fr = s_fr[w];
+    float4* c0 = s_c0[w];
+    float4* c1 = s_c1[w];
+    float2* c2 = s_c2[w];
+    float4* bl = s_b[w];
+
+    const int s = blockIdx.x * (64 * WPB) + threadIdx.x;
+    const bool valid = s < o.n;
+    const unsigned long long valid_m = __builtin_amdgcn_ballot_w64(valid);
+    if (valid_m == 0ull) return;
+    const int M = scal->nmassive;
+    const int sb = scal->sim_bits;
+    const int wave_base = s - lane;
+    const double4 p = valid ? spos[s] : make_double4(0.0, 0.0, 0.0, 0.0);
+    const int sim = valid ? simid_s[s] : -1;
+    const bool massive = valid && s < M;
+    const unsigned long long massive_m = __builtin_amdgcn_ballot_w64(massive);
+
+    /* hard cuts: next lane invalid, in another simulation or of the other kind */
+    const int sim_next = __shfl_down(sim, 1, 64);
+    const bool massive_next = ((massive_m >> (lane + 1)) & 1ull) && lane < 63;
+    const bool valid_next = ((valid_m >> (lane + 1)) & 1ull) && lane < 63;
+    const bool hard = valid && (!valid_next || sim_next != sim || massive_next != massive);
+    int lvl = -1;
+    if (massive && massive_next && !hard) lvl = nbh_levels(dadj[s], sb);
+    const unsigned long long cuts =
+        nbh_group_cuts(__builtin_amdgcn_ballot_w64(hard) | (1ull << 63), lvl, lane);
+
+    double ax = 0.0, ay = 0.0, az = 0.0, kx = 0.0, ky = 0.0, kz = 0.0;
+    unsigned nbb = 0, nbc = 0, nop = 0, iters = 0;
+    unsigned self_hits = 0;
+    bool bad = false;
+    unsigned long long remaining = valid_m;
+    while (remaining) {
+        const int lead = __builtin_ctzll(remaining);
+        const int last = __builtin_ctzll(cuts & (~0ull << lead));
+        const unsigned long long mem = (last == 63 ? ~0ull : ((1ull << (last + 1)) - 1ull)) &
+                                       (~0ull << lead) & remaining;
+        remaining &= ~mem;
+        const bool me = (mem >> lane) & 1ull;
+        const int sim_u = __builtin_amdgcn_readlane(sim, lead);
+        const int root = sims[sim_u].root;
+        if (root < 0) continue;                       /* no massive body: no tree */
+        const float eps2 = (float) sims[sim_u].eps2;
+
+        /* bounding box and centre of the sub-group */
+        const double lox = nbh_wave_min_d(me ? p.x : INFINITY);
+        const double loy = nbh_wave_min_d(me ? p.y : INFINITY);
+        const double loz = nbh_wave_min_d(me ? p.z : INFINITY);
+        const double hix = nbh_wave_max_d(me ? p.x : -INFINITY);
+        const double hiy = nbh_wave_max_d(me ? p.y : -INFINITY);
+        const double hiz = nbh_wave_max_d(me ? p.z : -INFINITY);
+        const double gx = 0.5 * (lox + hix), gy = 0.5 * (loy + hiy), gz = 0.5 * (loz + hiz);
+        const float rx = me ? (float) (p.x - gx) : 0.0f;
+        const float ry = me ? (float) (p.y - gy) : 0.0f;
+        const float rz = me ? (float) (p.z - gz) : 0.0f;
+        const int expected = __popcll(mem & massive_m);
+        int own_seen = 0;
+
+        int fsize = 1;
+        if (lane == 0) fr[0] = (sim_u << 3);          /* slot sim_u, one node */
+        __builtin_amdgcn_wave_barrier();
+        int nc = 0, nb = 0;
+
+        /* drains of the two lists */
+        auto flush_cells = [&](int cnt) {
+            float tx = 0.0f, ty = 0.0f, tz = 0.0f;
+            __builtin_amdgcn_wave_barrier();
+            for (int j = 0; j < cnt; ++j) {
+                const float4 a = c0[j];
+                const float dx = a.x - rx, dy = a.y - ry, dz = a.z - rz;
+                float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, eps2)));
+                if (EPS0) d2 = d2 == 0.0f ? 1.0f : d2;
+                const float rinv = __builtin_amdgcn_rsqf(d2);
+                const float rinv2 = rinv * rinv;
+                if (QUAD) {
+                    const float4 b = c1[j];
+                    const float2 e = c2[j];
+                    /* acc += r^-5 ((m r^2 + 2.5 dQd / r^2) d - Qd) */
+                    const float qx = __builtin_fmaf(b.x, dx, __builtin_fmaf(b.y, dy, b.z * dz));
+                    const float qy = __builtin_fmaf(b.y, dx, __builtin_fmaf(b.w, dy, e.x * dz));
+                    const float qz = __builtin_fmaf(b.z, dx, __builtin_fmaf(e.x, dy, e.y * dz));
+                    const float dQd = __builtin_fmaf(qx, dx, __builtin_fmaf(qy, dy, qz * dz));
+                    const float dr5 = rinv2 * rinv2 * rinv;
+                    const float S = __builtin_fmaf(dQd, 2.5f * rinv2, a.w * d2);
+                    tx = __builtin_fmaf(dr5, __builtin_fmaf(S, dx, -qx), tx);
+                    ty = __builtin_fmaf(dr5, __builtin_fmaf(S, dy, -qy), ty);
+                    tz = __builtin_fmaf(dr5, __builtin_fmaf(S, dz, -qz), tz);
+                } else {
+                    const float mor3 = a.w * rinv * rinv2;
+                    tx = __builtin_fmaf(mor3, dx, tx);
+                    ty = __builtin_fmaf(mor3, dy, ty);
+                    tz = __builtin_fmaf(mor3, dz, tz);
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+            if (me) {
+                nbh_kahan_add(ax, kx, (double) tx);
+                nbh_kahan_add(ay, ky, (double) ty);
+                nbh_kahan_add(az, kz, (double) tz);
+            }
+            if (COUNT) {
+                nbc += me ? (unsigned) cnt : 0u;
+                iters += (unsigned) cnt;
+            }
+        };
+        auto flush_bodies = [&](int cnt) {
+            float tx = 0.0f, ty = 0.0f, tz = 0.0f;
+            __builtin_amdgcn_wave_barrier();
+            for (int j = 0; j < cnt; ++j) {
+                const float4 a = bl[j];
+                const float dx = a.x - rx, dy = a.y - ry, dz = a.z - rz;
+                float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, eps2)));
+                /* eps2 > 0: the body's own entry adds exactly 0 (d = 0);
+                 * eps2 = 0: it is skipped by making d2 finite */
+                if (EPS0) d2 = d2 == 0.0f ? 1.0f : d2;
+                const float rinv = __builtin_amdgcn_rsqf(d2);
+                const float mor3 = a.w * rinv * (rinv * rinv);
+                tx = __builtin_fmaf(mor3, dx, tx);
+                ty = __builtin_fmaf(mor3, dy, ty);
+                tz = __builtin_fmaf(mor3, dz, tz);
+            }
+            __builtin_amdgcn_wave_barrier();
+            if (me) {
+                nbh_kahan_add(ax, kx, (double) tx);
+                nbh_kahan_add(ay, ky, (double) ty);
+                nbh_kahan_add(az, kz, (double) tz);
+            }
+            if (COUNT) {
+                nbb += me ? (unsigned) cnt : 0u;
+                iters += (unsigned) cnt;
+            }
+        };
+
+        while (fsize > 0) {
+            /* pop up to 8 ranges (64 nodes); throttled near the capacity so
+             * that the pushes of this round always fit */
+            int E = min(8, fsize);
+            if (fsize + 7 * E > NBH_GW_FCAP - NBH_GW_SLACK)
+                E = max(1, min(E, (NBH_GW_FCAP - NBH_GW_SLACK - fsize) / 7));
+            const int eidx = lane >> 3, sub = lane & 7;
+            const int ent = eidx < E ? fr[fsize - 1 - eidx] : 0;
+            const int cnt = (ent & 7) + 1;
+            const bool act = eidx < E && sub < cnt;
+            fsize -= E;
+            __builtin_amdgcn_wave_barrier();
+            const NodeG r = T[act ? (ent >> 3) + sub : 0];
+            /* FP64 test against the sub-group's box */
+            const double ddx = fmax(fmax(lox - r.x, r.x - hix), 0.0);
+            const double ddy = fmax(fmax(loy - r.y, r.y - hiy), 0.0);
+            const double ddz = fmax(fmax(loz - r.z, r.z - hiz), 0.0);
+            const double d2 = ddz * ddz + (ddy * ddy + ddx * ddx);
+            const bool isbody = r.count == 0;
+            const bool acc = act && (isbody || d2 >= (double) r.rcrit2);
+            const bool opn = act && !acc;
+            const unsigned long long cm = __builtin_amdgcn_ballot_w64(acc && !isbody);
+            const unsigned long long bm = __builtin_amdgcn_ballot_w64(acc && isbody);
+            const unsigned long long om = __builtin_amdgcn_ballot_w64(opn);
+            /* a member's own leaf: its body meets itself (no incest) */
+            const unsigned rel = (unsigned) (r.link - wave_base);
+            const bool own = acc && isbody && rel < 64u && ((mem >> rel) & 1ull);
+            own_seen += __popcll(__builtin_amdgcn_ballot_w64(own));
+            const int npush = __popcll(om);
+            if (fsize + npush > NBH_GW_FCAP) {
+                bad = true;                           /* deeper than 63 levels */
+                break;
+            }
+            if (acc && !isbody) {
+                const int slot = nc + (int) nbh_mbcnt(cm);
+                c0[slot] = make_float4((float) (r.x - gx), (float) (r.y - gy),
+                                       (float) (r.z - gz), r.m);
+                if (QUAD) {
+                    c1[slot] = make_float4(r.q[0], r.q[1], r.q[2], r.q[3]);
+                    c2[slot] = make_float2(r.q[4], r.q[5]);
+                }
+            }
+            if (acc && isbody) {
+                const int slot = nb + (int) nbh_mbcnt(bm);
+                bl[slot] = make_float4((float) (r.x - gx), (float) (r.y - gy),
+                                       (float) (r.z - gz), r.m);
+            }
+            if (opn) fr[fsize + (int) nbh_mbcnt(om)] = (r.link << 3) | (r.count - 1);
+            __builtin_amdgcn_wave_barrier();
+            nc += __popcll(cm);
+            nb += __popcll(bm);
+            fsize += npush;
+            if (COUNT) nop += (unsigned) npush;
+            if (nc > LC - 64) {
+                flush_cells(nc);
+                nc = 0;
+            }
+            if (nb > LC - 64) {
+                flush_bodies(nb);
+                nb = 0;
+            }
+        }
+        if (nc > 0) flush_cells(nc);
+        if (nb > 0) flush_bodies(nb);
+        if (COUNT) self_hits += (unsigned) own_seen;
+        if (own_seen < expected && lane == 0) atomicOr(&sims[sim_u].incest, 1u);
+        if (bad) break;
+    }
+
+    if (bad && lane == 0) atomicOr(&scal->status, NBHIP_ERROR);
+    if (COUNT) {
+        /* the self interactions (d = 0) are not counted, as in the reference;
+         * nop and iters are wave-uniform */
+        const unsigned long long c0 = nbh_wave_sum_u64(nbb);
+        const unsigned long long c1 = nbh_wave_sum_u64(nbc);
+        if (lane == 0) {
+            atomicAdd(&scal->counters[0], c0 - self_hits);
+            atomicAdd(&scal->counters[1], c1);
+            atomicAdd(&scal->counters[2], (unsigned long long) nop);
+            atomicAdd(&scal->counters[3], (unsigned long long) iters);
+        }
+    }
+    if (!valid) return;
+    nbh_walk_finish(o, p.x, p.y, p.z, ax, ay, az, perm[s], oax, oay, oaz, vx, vy, vz);
+}
+
+static inline int nbh_gblocks(int n, int per) { return n > 0 ? (n + per - 1) / per : 1; }
+
+void nbh_walk_group(nbhip_state* st, const WalkOut& o, double* oax, double* oay, double* oaz,
+                    const WalkTune& t, int count)
+{
+    const int bs = (t.block == 64 || t.block == 128) ? t.block : 256;
+    int lc = (t.batch == 64 || t.batch == 128) ? t.batch : 256;
+    if (bs == 256 && lc > 128) lc = 128;       /* LDS: at most 64 KiB per workgroup */
+    dim3 grid(nbh_gblocks(st->n, bs)), block(bs);
+    const bool quad = st->ctx.use_quad != 0;
+    const bool eps0 = !o.all_eps2_pos;
+#define NBH_GWL(W, L, Q, E, C)                                                                \
+    hipLaunchKernelGGL((k_walk_group<W, L, Q, E, C>), grid, block, 0, st->stream, st->nodeg,   \
+                       st->spos, st->perm, st->simid_s, st->dadj, st->sims, oax, oay, oaz,     \
+                       st->vx, st->vy, st->vz, st->scal, o)
+#define NBH_GWC(W, L, Q, E)                                                                   \
+    do { if (count) NBH_GWL(W, L, Q, E, true); else NBH_GWL(W, L, Q, E, false); } while (0)
+#define NBH_GWE(W, L, Q)                                                                      \
+    do { if (eps0) NBH_GWC(W, L, Q, true); else NBH_GWC(W, L, Q, false); } while (0)
+#define NBH_GWQ(W, L)                                                                         \
+    do { if (quad) NBH_GWE(W, L, true); else NBH_GWE(W, L, false); } while (0)
+    if (bs == 64) {
+        if (lc == 64) NBH_GWQ(1, 64); else if (lc == 128) NBH_GWQ(1, 128); else NBH_GWQ(1, 256);
+    } else if (bs == 128) {
+        if (lc == 64) NBH_GWQ(2, 64); else if (lc == 128) NBH_GWQ(2, 128); else NBH_GWQ(2, 256);
+    } else {
+        if (lc == 64) NBH_GWQ(4, 64); else NBH_GWQ(4, 128);
+    }
+#undef NBH_GWQ
+#undef NBH_GWE
+#undef NBH_GWC
+#undef NBH_GWL
+}

@@ -34,7 +34,7 @@ NBODY_TREE_INCEST_NONFATAL = 1 << 24
 NBODY_STATUS_ALL_WARNINGS = 0xFF000000
 
 CRITERIA = {"treecode": 0, "sw93": 1, "bh86": 2, "exact": 3}
-PRECISION = {"fp64": 0, "mixed": 1, "mixed_coherent": 2}
+PRECISION = {"fp64": 0, "mixed": 1, "mixed_coherent": 2, "mixed_group": 3}
 
 
 def status_is_fatal(rc: int) -> bool:
