@@ -164,6 +164,10 @@ static unsigned nbh_create(const nbhip_ctx* ctx, int device, int nsim, const int
     st->sort_blocks = st->n > 0 ? (st->n + NBH_SORT_TILE - 1) / NBH_SORT_TILE : 1;
     st->tune.block = 256;
     st->tune.batch = 1024;
+    if (ctx->precision == NBHIP_PRECISION_MIXED_GROUP) {
+        st->tune.block = 128;
+        st->tune.batch = 128;
+    }
     st->h_body_off = (int*) calloc(nsim, sizeof(int));
     st->h_body_n = (int*) calloc(nsim, sizeof(int));
     st->h_sim_status = (unsigned*) calloc(nsim, sizeof(unsigned));
@@ -830,15 +834,20 @@ extern "C" unsigned nbhip_autotune(nbhip_state* st, nbhip_tune* best, nbhip_tune
     double* t = nullptr;
     NBH_CHECK(hipMalloc(&t, 3 * n * sizeof(double)));
     const int blocks[3] = { 64, 128, 256 };
+    const bool group = st->ctx.precision == NBHIP_PRECISION_MIXED_GROUP;
+    /* mixed walks: FP32 partial-sum node block; group walk: LDS list depth */
     const int batches_mixed[4] = { 256, 1024, 4096, 16384 };
+    const int depths_group[3] = { 64, 128, 256 };
     const bool mixed = st->ctx.precision != NBHIP_PRECISION_FP64;
-    const int nb = mixed ? 4 : 1;
+    const int nb = group ? 3 : (mixed ? 4 : 1);
     nbhip_tune bestt = { st->tune.block, st->tune.batch, 1e300 };
     int cnt = 0;
     hipError_t e = hipSuccess;
     for (int bi = 0; bi < 3 && e == hipSuccess; ++bi) {
         for (int ci = 0; ci < nb && e == hipSuccess; ++ci) {
-            WalkTune w = { blocks[bi], mixed ? batches_mixed[ci] : st->tune.batch };
+            WalkTune w = { blocks[bi], group ? depths_group[ci]
+                                           : (mixed ? batches_mixed[ci] : st->tune.batch) };
+            if (group && blocks[bi] == 256 && w.batch > 128) continue;   /* LDS limit */
             float ms[3];
             for (int r = 0; r < 3 && e == hipSuccess; ++r) {
                 e = hipEventRecord(st->ev[0], st->stream);
