@@ -95,7 +95,10 @@ typedef struct nbhip_ctx {
     int allow_incest;       /* ctx->allowIncest */
     int use_potential;      /* ctx->potentialType: 0 NONE, 1 DEFAULT */
     int precision;          /* NBHIP_PRECISION_* */
-    int reserved;
+    int lmc;                /* ctx->LMC: the LMC's moving potential and reflex shift
+                               (nbody/src/nbody_plain.c:268-315,427-433,
+                               nbody/src/nbody_grav.c:164-172) are not supported:
+                               nonzero -> NBHIP_UNSUPPORTED */
     nbhip_potential pot;    /* ctx->pot */
 } nbhip_ctx;
 
@@ -142,8 +145,67 @@ unsigned nbhip_step(nbhip_state* st);
 unsigned nbhip_run(nbhip_state* st, unsigned nsteps);
 
 /* Runs `nsteps` KDK steps without the leading force evaluation (for callers
- * that already called nbhip_gravity, e.g. after upload). */
+ * that already called nbhip_gravity, e.g. after upload).  Stops at the first
+ * step with a fatal status: with graph steps the remaining replays do
+ * nothing and the step counter counts only the steps that ran. */
 unsigned nbhip_steps(nbhip_state* st, unsigned nsteps);
+
+/* The host hooks of nbRunSystemPlain's loop (nbody/src/nbody_plain.c:366-461)
+ * for nbhip_run_system.  Either callback may be NULL. */
+typedef struct nbhip_run_hooks {
+    unsigned nstep;             /* ctx->nStep: run until the step counter reaches it */
+    int use_best_like;          /* ctx->useBestLike */
+    double best_like_start;     /* ctx->BestLikeStart: first step fraction evaluated */
+    double time_evolve;         /* ctx->timeEvolve (the best likelihood's time) */
+    int keep_best_bodies;       /* keep the best step's bodies (nbf->outFileName) */
+    unsigned chunk;             /* steps run between two checkpoint calls outside the
+                                   evaluated steps (0 or 1: every step) */
+    /* get_likelihood (nbody/src/nbody_plain.c:75-245) up to the comparison:
+     * called after every step with step / nstep >= best_like_start; fills
+     * lk[10] with nbSystemLikelihood's array (lk[0] the likelihood, the
+     * components the caller does not use set to 0, as nbody_plain.c stores
+     * them) from the state's current histograms (nbhip_histogram,
+     * nbhip_histogram_dispersion).  Returns NBHIP_SUCCESS, or a status that
+     * stops the run; a likelihood it could not form (no data file) is
+     * reported as NaN and treated as the worst case. */
+    unsigned (*likelihood)(void* user, nbhip_state* st, unsigned step, double* lk);
+    /* nbCheckpoint (nbody/src/nbody_plain.c:60-73): called after every step
+     * (every `chunk` steps outside the evaluated ones); writes a checkpoint
+     * when it is time (nbhip_download + nbhip_tree_info_get give its
+     * contents).  A nonzero return stops the run with that status. */
+    unsigned (*checkpoint)(void* user, nbhip_state* st, unsigned step);
+    void* user;
+} nbhip_run_hooks;
+
+/* st->bestLikelihood* after a run (nbody/src/nbody_plain.c:165-232). */
+typedef struct nbhip_best_like {
+    double lk[10];              /* the best step's array; lk[0] = bestLikelihood
+                                   (DEFAULT_WORST_CASE 9999999.9 before any step) */
+    double time;                /* bestLikelihood_time = step / nstep * timeEvolve */
+    unsigned step;              /* the best step (0: none) */
+    unsigned count;             /* bestLikelihood_count: improvements */
+    unsigned evaluated;         /* steps whose likelihood was evaluated */
+} nbhip_best_like;
+
+/* Replaces nbRunSystem / nbRunSystemPlain (nbody/src/nbody.c:202-212,
+ * nbody/src/nbody_plain.c:366-461) with its hooks: one force evaluation,
+ * then KDK steps until the step counter reaches hooks->nstep.  With
+ * use_best_like, every step from the first with step / nstep >=
+ * best_like_start runs alone and is followed by the likelihood hook; the
+ * likelihood is clamped as get_likelihood clamps it (beyond +-9999999.9 or
+ * NaN: 9999999.9; exactly 0: 1e-9) and kept in *best when its absolute
+ * value is smaller than the best so far (with keep_best_bodies the bodies
+ * of that step too, for nbhip_best_bodies).  Steps before that run in
+ * chunks (replayed graphs with nbhip_set_graph).  The checkpoint hook
+ * follows every step (chunk).  The run stops at the first fatal status
+ * (the likelihood of that step is still evaluated, as in the reference);
+ * the final checkpoint (nbWriteFinalCheckpoint) is the caller's. */
+unsigned nbhip_run_system(nbhip_state* st, const nbhip_run_hooks* hooks, nbhip_best_like* best);
+
+/* The bodies (n x 3 each, upload order) of the best-likelihood step of the
+ * last nbhip_run_system with keep_best_bodies (st->bestLikelihoodBodyTab);
+ * NBHIP_USER_ERROR if none was kept.  Either pointer may be NULL. */
+unsigned nbhip_best_bodies(nbhip_state* st, double* pos, double* vel);
 
 unsigned nbhip_tree_info_get(nbhip_state* st, nbhip_tree_info* info);
 

@@ -18,7 +18,7 @@ LIB_PATH = os.path.join(PKG_DIR, "libnbhip.so")
 MODELS_LIB_PATH = os.path.join(PKG_DIR, "libnbmodels.so")
 
 SOURCES = ["api.hip", "tree.hip", "walk.hip", "walk_group.hip", "adjoint.hip", "histogram.hip",
-           "exact.hip"]
+           "exact.hip", "run.hip"]
 
 # -ffp-contract=off: the plain FP64 form keeps the reference's two-rounding
 # a*b+c so that it reproduces the CPU path bit for bit.

@@ -57,7 +57,7 @@ __global__ __launch_bounds__(256) void k_adj_walk(const NodeW* __restrict__ node
                                                   const DevScalars* __restrict__ scal, AdjWalk o)
 {
     const int s = blockIdx.x * 256 + threadIdx.x;
-    if (scal->status & (NBHIP_CELL_OVERFLOW_ERROR | NBHIP_ERROR)) return;
+    if ((scal->status & (NBHIP_CELL_OVERFLOW_ERROR | NBHIP_ERROR)) || scal->halt) return;
     const int M = scal->nmassive;
     const bool valid = s < o.n;
     if (__ballot(valid) == 0ull) return;

@@ -64,6 +64,10 @@ static unsigned nbh_validate(const nbhip_ctx* c)
             return NBHIP_UNSUPPORTED;
         }
     }
+    if (c->lmc) {
+        nbh_set_error("the LMC's moving potential (NBodyCtx.LMC) is not supported on the GPU");
+        return NBHIP_UNSUPPORTED;
+    }
     if (c->precision != NBHIP_PRECISION_FP64 && c->precision != NBHIP_PRECISION_MIXED &&
         c->precision != NBHIP_PRECISION_MIXED_COHERENT &&
         c->precision != NBHIP_PRECISION_MIXED_GROUP) {
@@ -100,7 +104,7 @@ static void nbh_free_all(nbhip_state* st)
                      st->bhi,  st->bdelta, st->bparent, st->node,  st->nquad,  st->nmid,
                      st->nlevel, st->nparent, st->nsim_node, st->level_list, st->nodef,
                      st->scal, st->sims, st->ignore, st->hist_buf, st->disp_buf, st->pos4,
-                     st->exact_part, st->nodeg, st->nkid, st->kbase };
+                     st->exact_part, st->nodeg, st->nkid, st->kbase, st->best_buf };
     for (void* p : ptrs)
         if (p) (void) hipFree(p);
     if (st->graph_exec) (void) hipGraphExecDestroy(st->graph_exec);
@@ -367,6 +371,10 @@ static unsigned nbh_collect(nbhip_state* st)
     /* st->status keeps what concerns the whole state (node capacity, launch
      * errors); h_sim_status[k] what one simulation's tree or walk reported. */
     st->status |= s.status;
+    /* steps the device completed since the last read (a halted graph replay
+     * does not count the steps it skipped) */
+    st->step += s.steps_done - st->dev_steps;
+    st->dev_steps = s.steps_done;
     unsigned rc = st->status;
     for (int k = 0; k < st->nsim; ++k) {
         unsigned f = tab[k].flags;
@@ -451,7 +459,7 @@ extern "C" unsigned nbhip_step(nbhip_state* st)
     NBH_CHECK(hipSetDevice(st->device));
     unsigned rc = nbh_force(st, 1, 1);
     if (rc) return rc;
-    st->step++;
+    nbh_step_end(st);
     rc = nbh_collect(st);
     nbh_phase_read(st);
     return rc;
@@ -470,6 +478,7 @@ static unsigned nbh_capture_step(nbhip_state* st)
     nbh_bbox(st, 1);
     if (st->ctx.criterion != NBHIP_EXACT) nbh_build(st, 1);
     nbh_walk(st, 1, 0, st->ax, st->ay, st->az, 0, nullptr);
+    nbh_step_end(st);
     hipError_t e = hipStreamEndCapture(st->stream, &st->graph);
     st->profiling = prof;
     if (e != hipSuccess) {
@@ -493,7 +502,6 @@ extern "C" unsigned nbhip_steps(nbhip_state* st, unsigned nsteps)
             if (rc) return rc;
         }
         for (unsigned k = 0; k < nsteps; ++k) NBH_CHECK(hipGraphLaunch(st->graph_exec, st->stream));
-        st->step += nsteps;
         st->have_tree = st->ctx.criterion != NBHIP_EXACT;
         return nbh_collect(st);
     }

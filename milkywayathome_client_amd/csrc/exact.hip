@@ -192,10 +192,11 @@ __global__ __launch_bounds__(256) void k_exact_finish(int n, const double* __res
                                                       double* __restrict__ vx,
                                                       double* __restrict__ vy,
                                                       double* __restrict__ vz,
-                                                      double* __restrict__ g)
+                                                      double* __restrict__ g,
+                                                      const DevScalars* __restrict__ scal)
 {
     const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
+    if (i >= n || scal->halt) return;
     const size_t N = (size_t) n;
     constexpr int W = ADJ ? 4 : 3;
     const int S = nbh_exact_segments(sims[simid[i]].body_n);
@@ -251,7 +252,8 @@ void nbh_exact_force(nbhip_state* st, int kick, double* oax, double* oay, double
                        st->exact_part);
     hipLaunchKernelGGL(k_exact_finish<false>, dim3(nb), dim3(256), 0, s, st->n, st->px, st->py,
                        st->pz, st->simid, st->sims, st->exact_part, use_pot, c.pot, kick,
-                       0.5 * c.timestep, oax, oay, oaz, st->vx, st->vy, st->vz, nullptr);
+                       0.5 * c.timestep, oax, oay, oaz, st->vx, st->vy, st->vz, nullptr,
+                       st->scal);
 }
 
 void nbh_exact_adjoint(nbhip_state* st, double* g)
@@ -266,5 +268,5 @@ void nbh_exact_adjoint(nbhip_state* st, double* g)
                        st->exact_part);
     hipLaunchKernelGGL(k_exact_finish<true>, dim3(nb), dim3(256), 0, s, st->n, st->px, st->py,
                        st->pz, st->simid, st->sims, st->exact_part, c.use_potential, c.pot, 0,
-                       0.0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, g);
+                       0.0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, g, st->scal);
 }

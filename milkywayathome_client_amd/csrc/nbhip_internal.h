@@ -85,6 +85,8 @@ struct DevScalars {
     unsigned status;               /* sticky global bits (overflow, internal error) */
     int sim_bits;                  /* key bits holding the simulation id */
     unsigned long long counters[4];/* body-body, body-cell, opened, wave node steps */
+    int halt;                      /* a step ended with a fatal status: later steps do nothing */
+    unsigned steps_done;           /* KDK steps completed on the device */
     int level_count[64];           /* cells per level */
     int level_fill[64];
     int level_off[65];             /* exclusive scan of level_count */
@@ -164,6 +166,9 @@ struct nbhip_state {
     int all_eps2_pos;
 
     unsigned step;
+    unsigned dev_steps;           /* DevScalars::steps_done at the last status read */
+    double* best_buf;             /* bodies of the best-likelihood step: x y z vx vy vz */
+    int best_kept;                /* best_buf holds the last run's best step */
     unsigned status;              /* host-side sticky status (all simulations) */
     unsigned *h_sim_status;       /* host-side sticky status per simulation */
     int have_tree;
@@ -195,6 +200,7 @@ struct nbhip_state {
 /* launchers (tree.hip) */
 void nbh_bbox(nbhip_state* st, int drift);
 void nbh_build(nbhip_state* st, int graph_safe);
+void nbh_step_end(nbhip_state* st);
 /* launchers (walk.hip) */
 void nbh_walk(nbhip_state* st, int kick, int count, double* out_ax, double* out_ay,
               double* out_az, int self_only, const WalkTune* tune);

@@ -54,10 +54,12 @@ __global__ __launch_bounds__(256) void k_drift_bbox(int n, int drift, double dt,
                                                     const double* __restrict__ m,
                                                     double4* __restrict__ pos4,
                                                     const int* __restrict__ simid,
-                                                    SimTab* __restrict__ sims)
+                                                    SimTab* __restrict__ sims,
+                                                    const DevScalars* __restrict__ scal)
 {
     __shared__ int s_sim[4];
     __shared__ double s_mx[4];
+    if (scal->halt) return;               /* a fatal status stopped the run */
     const int lo = blockIdx.x * NBH_BBOX_CHUNK;
     const int hi = min(n, lo + NBH_BBOX_CHUNK);
     int cur = -1;
@@ -122,6 +124,7 @@ __global__ __launch_bounds__(256) void k_drift_bbox(int n, int drift, double dt,
 __global__ __launch_bounds__(256) void k_rsize(int nsim, SimTab* __restrict__ sims,
                                                DevScalars* __restrict__ scal)
 {
+    if (scal->halt) return;
     for (int k = threadIdx.x; k < nsim; k += 256) {
         SimTab& t = sims[k];
         const double xm = __longlong_as_double((long long) t.xyzmax_bits);
@@ -1250,6 +1253,30 @@ __global__ __launch_bounds__(NBH_TAIL_THREADS) void k_summarize_tail(
     }
 }
 
+/* End of a KDK step (nbRunSystemPlain's loop, nbody/src/nbody_plain.c:
+ * 409-448): counts the step and, when a simulation's tree or walk reported
+ * a fatal status (a structure, depth or internal error, or incest without
+ * allowIncest), halts the state so that the remaining steps of a replayed
+ * graph do nothing -- the run stops at the step that failed, as the
+ * reference's loop returns there. */
+__global__ __launch_bounds__(256) void k_step_end(int nsim, const SimTab* __restrict__ sims,
+                                                  int allow_incest, DevScalars* __restrict__ scal)
+{
+    __shared__ int fatal;
+    if (scal->halt) return;
+    if (threadIdx.x == 0) fatal = (scal->status & ~NBHIP_STATUS_ALL_WARNINGS) != 0u;
+    __syncthreads();
+    for (int k = threadIdx.x; k < nsim; k += 256) {
+        const SimTab& t = sims[k];
+        if ((t.flags & ~NBHIP_STATUS_ALL_WARNINGS) || (t.incest && !allow_incest)) fatal = 1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        scal->steps_done += 1u;
+        if (fatal) scal->halt = 1;
+    }
+}
+
 /* ---------------------------------------------------------------------- */
 /* host launchers                                                          */
 /* ---------------------------------------------------------------------- */
@@ -1261,8 +1288,14 @@ void nbh_bbox(nbhip_state* st, int drift)
     hipLaunchKernelGGL(k_drift_bbox, dim3(nbh_blocks(st->n, NBH_BBOX_CHUNK)), dim3(256), 0,
                        st->stream,
                        st->n, drift, dt, 0.5 * dt, st->px, st->py, st->pz, st->vx, st->vy, st->vz,
-                       st->ax, st->ay, st->az, st->m, st->pos4, st->simid, st->sims);
+                       st->ax, st->ay, st->az, st->m, st->pos4, st->simid, st->sims, st->scal);
     hipLaunchKernelGGL(k_rsize, dim3(1), dim3(256), 0, st->stream, st->nsim, st->sims, st->scal);
+}
+
+void nbh_step_end(nbhip_state* st)
+{
+    hipLaunchKernelGGL(k_step_end, dim3(1), dim3(256), 0, st->stream, st->nsim, st->sims,
+                       st->ctx.allow_incest, st->scal);
 }
 
 static void nbh_mark(nbhip_state* st, int k)

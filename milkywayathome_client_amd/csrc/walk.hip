@@ -36,7 +36,7 @@ __global__ __launch_bounds__(BS) void k_walk64(const NodeW* __restrict__ node,
                                                DevScalars* __restrict__ scal, WalkOut o)
 {
     const int s = blockIdx.x * BS + threadIdx.x;
-    if (scal->status & (NBHIP_CELL_OVERFLOW_ERROR | NBHIP_ERROR)) return;
+    if ((scal->status & (NBHIP_CELL_OVERFLOW_ERROR | NBHIP_ERROR)) || scal->halt) return;
     const int M = scal->nmassive;
     const bool valid = s < o.n;
     if (__ballot(valid) == 0ull) return;
@@ -163,7 +163,7 @@ __global__ __launch_bounds__(BS) void k_walk32(const NodeF* __restrict__ node,
                                                DevScalars* __restrict__ scal, WalkOut o)
 {
     const int s = blockIdx.x * BS + threadIdx.x;
-    if (scal->status & (NBHIP_CELL_OVERFLOW_ERROR | NBHIP_ERROR)) return;
+    if ((scal->status & (NBHIP_CELL_OVERFLOW_ERROR | NBHIP_ERROR)) || scal->halt) return;
     const int M = scal->nmassive;
     const bool valid = s < o.n;
     if (__ballot(valid) == 0ull) return;

@@ -135,7 +135,7 @@ __global__ __launch_bounds__(64 * WPB) void k_walk_group(const NodeG* __restrict
     __shared__ float2 s_c2[WPB][LC];     /* cells: q yz zz */
     __shared__ float4 s_b[WPB][LC];      /* bodies: x y z (relative) m */
 
-    if (scal->status & (NBHIP_CELL_OVERFLOW_ERROR | NBHIP_ERROR)) return;
+    if ((scal->status & (NBHIP_CELL_OVERFLOW_ERROR | NBHIP_ERROR)) || scal->halt) return;
     const int w = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
     int* fr = s_fr[w];

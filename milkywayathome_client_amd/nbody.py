@@ -64,7 +64,7 @@ class _Ctx(C.Structure):
         ("eps2", C.c_double), ("theta", C.c_double), ("timestep", C.c_double),
         ("tree_rsize", C.c_double), ("criterion", C.c_int), ("use_quad", C.c_int),
         ("allow_incest", C.c_int), ("use_potential", C.c_int), ("precision", C.c_int),
-        ("reserved", C.c_int), ("pot", _Potential),
+        ("lmc", C.c_int), ("pot", _Potential),
     ]
 
 
@@ -101,6 +101,22 @@ class _Tune(C.Structure):
     _fields_ = [("block", C.c_int), ("batch", C.c_int), ("ms", C.c_double)]
 
 
+_LIKE_FN = C.CFUNCTYPE(C.c_uint, C.c_void_p, C.c_void_p, C.c_uint, C.POINTER(C.c_double))
+_CKPT_FN = C.CFUNCTYPE(C.c_uint, C.c_void_p, C.c_void_p, C.c_uint)
+
+
+class _RunHooks(C.Structure):
+    _fields_ = [("nstep", C.c_uint), ("use_best_like", C.c_int),
+                ("best_like_start", C.c_double), ("time_evolve", C.c_double),
+                ("keep_best_bodies", C.c_int), ("chunk", C.c_uint),
+                ("likelihood", _LIKE_FN), ("checkpoint", _CKPT_FN), ("user", C.c_void_p)]
+
+
+class _BestLike(C.Structure):
+    _fields_ = [("lk", C.c_double * 10), ("time", C.c_double), ("step", C.c_uint),
+                ("count", C.c_uint), ("evaluated", C.c_uint)]
+
+
 EXPORTED = [
     "nbhip_create", "nbhip_upload", "nbhip_download", "nbhip_gravity", "nbhip_step",
     "nbhip_run", "nbhip_steps", "nbhip_tree_info_get", "nbhip_tree_download",
@@ -111,6 +127,7 @@ EXPORTED = [
     "nbhip_adjoint_forward", "nbhip_adjoint_backward", "nbhip_gravity_vjp", "nbhip_set_ignore",
     "nbhip_histogram", "nbhip_orbit_tangent", "nbhip_restore", "nbhip_histogram_dispersion",
     "nbhip_set_open_scale", "nbhip_batch_tree_download", "nbhip_histogram_noalloc",
+    "nbhip_run_system", "nbhip_best_bodies",
 ]
 
 _LIB: Optional[C.CDLL] = None
@@ -163,6 +180,8 @@ def load_library() -> C.CDLL:
         "nbhip_set_ignore": (u, [st, C.POINTER(C.c_ubyte)]),
         "nbhip_histogram": (u, [st, C.POINTER(_HistParams), C.POINTER(C.c_uint),
                                 C.POINTER(C.c_uint)]),
+        "nbhip_run_system": (u, [st, C.POINTER(_RunHooks), C.POINTER(_BestLike)]),
+        "nbhip_best_bodies": (u, [st, _dp, _dp]),
         "nbhip_histogram_noalloc": (u, [st, C.POINTER(_HistParams), C.POINTER(C.c_uint),
                                         C.POINTER(C.c_uint)]),
         "nbhip_adjoint_forward": (u, [st, C.c_uint, C.c_uint]),
@@ -341,7 +360,7 @@ class NBodyCtx:
         c.allow_incest = int(bool(self.allowIncest))
         c.use_potential = 0 if potential is None else 1
         c.precision = PRECISION[precision]
-        c.reserved = 0
+        c.lmc = 0
         c.pot = (potential or Potential()).to_c()
         return c
 
@@ -410,6 +429,53 @@ class NBodyState:
         k = self.ctx.nStep if nsteps is None else nsteps
         self.last_status = self._check(self.lib.nbhip_run(self._h, k))
         return self.last_status
+
+    def run_system(self, likelihood=None, checkpoint=None, nstep: Optional[int] = None,
+                   use_best_like: bool = True, best_like_start: float = 0.95,
+                   keep_best_bodies: bool = False, chunk: int = 64):
+        """nbRunSystem with its hooks (nbhip_run_system): the force, then KDK
+        steps to nstep (ctx.nStep by default).  likelihood(step) returns the
+        10-entry nbSystemLikelihood array of the current state (NaN where
+        unknown); checkpoint(step) may write a checkpoint and returns a status.
+        Returns (status, best) with best = dict(lk, time, step, count,
+        evaluated)."""
+        failure = []
+
+        def _like(user, h, step, out):
+            try:
+                arr = likelihood(int(step))
+                for i in range(10):
+                    out[i] = float(arr[i]) if i < len(arr) else float("nan")
+                return 0
+            except Exception as e:          # surfaced after the run
+                failure.append(e)
+                return NBODY_ERROR
+
+        def _ckpt(user, h, step):
+            try:
+                return int(checkpoint(int(step)) or 0)
+            except Exception as e:
+                failure.append(e)
+                return NBODY_ERROR
+
+        hooks = _RunHooks(int(self.ctx.nStep if nstep is None else nstep), int(use_best_like),
+                          float(best_like_start), float(self.ctx.timeEvolve),
+                          int(keep_best_bodies), int(chunk),
+                          _LIKE_FN(_like) if likelihood else _LIKE_FN(),
+                          _CKPT_FN(_ckpt) if checkpoint else _CKPT_FN(), None)
+        best = _BestLike()
+        rc = self.lib.nbhip_run_system(self._h, C.byref(hooks), C.byref(best))
+        if failure:
+            raise failure[0]
+        self.last_status = self._check(rc)
+        return rc, {"lk": list(best.lk), "time": best.time, "step": best.step,
+                    "count": best.count, "evaluated": best.evaluated}
+
+    def best_bodies(self):
+        """(pos, vel) of the best-likelihood step kept by run_system."""
+        pos, vel = np.empty((self.n, 3)), np.empty((self.n, 3))
+        self._check(self.lib.nbhip_best_bodies(self._h, _ptr(pos), _ptr(vel)))
+        return pos, vel
 
     def bodies(self):
         pos = np.empty((self.n, 3))

@@ -56,3 +56,13 @@ def test_status_codes_match_nbodystatus():
     assert nb.NBODY_TREE_INCEST_NONFATAL == 1 << 24
     assert not nb.status_is_fatal(nb.NBODY_TREE_INCEST_NONFATAL)
     assert nb.status_is_fatal(nb.NBODY_TREE_INCEST_FATAL)
+
+
+def test_lmc_context_is_rejected_before_device_access():
+    lib = nb.load_library()
+    ctx = nb.NBodyCtx.create(timestep=1e-3, timeEvolve=1.0, eps2=1e-4, theta=0.5)
+    h = ctypes.c_void_p()
+    c = ctx.to_c(nb.milky_way_potential())
+    c.lmc = 1                               # NBodyCtx.LMC: not supported on the GPU
+    assert lib.nbhip_create(ctypes.byref(c), 0, 10, ctypes.byref(h)) == nb.NBODY_UNSUPPORTED
+    assert b"LMC" in lib.nbhip_last_error()
