@@ -124,7 +124,9 @@ __global__ __launch_bounds__(256) void k_drift_bbox(int n, int drift, double dt,
 __global__ __launch_bounds__(256) void k_rsize(int nsim, SimTab* __restrict__ sims,
                                                DevScalars* __restrict__ scal)
 {
-    if (scal->halt) return;
+    /* Runs on a halted state too: the per-build counters below must start
+     * every build from zero (a halted step skips the drift, so the maximum
+     * read here is 0 and the root size does not change). */
     for (int k = threadIdx.x; k < nsim; k += 256) {
         SimTab& t = sims[k];
         const double xm = __longlong_as_double((long long) t.xyzmax_bits);
@@ -1011,6 +1013,7 @@ __global__ __launch_bounds__(256) void k_level_scatter(const int* __restrict__ n
     __shared__ int base[64];
     __shared__ int off[65];
     const int tid = threadIdx.x;
+    __shared__ int consistent;
     if (tid == 0) {
         int acc = 0;
         for (int l = 0; l < 64; ++l) {
@@ -1018,10 +1021,15 @@ __global__ __launch_bounds__(256) void k_level_scatter(const int* __restrict__ n
             acc += scal->level_count[l];
         }
         off[64] = acc;
+        /* the levels must account for exactly the cells built (else the
+         * level lists would not fit the table: stop, flag, write nothing) */
+        consistent = acc == scal->ncell;
+        if (!consistent) atomicOr(&scal->status, NBHIP_ERROR);
     }
     __syncthreads();
     if (blockIdx.x == 0 && tid <= 64) scal->level_off[tid] = off[tid];
-    const int nn = (scal->status & (NBHIP_CELL_OVERFLOW_ERROR | NBHIP_ERROR)) ? 0 : scal->nnode;
+    const int nn = (!consistent || (scal->status & (NBHIP_CELL_OVERFLOW_ERROR | NBHIP_ERROR)))
+                       ? 0 : scal->nnode;
     for (int c0 = blockIdx.x * NBH_LS_CHUNK; c0 < nn; c0 += gridDim.x * NBH_LS_CHUNK) {
         const int cend = min(c0 + NBH_LS_CHUNK, nn);
         if (tid < 64) lh[tid] = 0;

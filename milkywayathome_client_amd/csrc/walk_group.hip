@@ -32,11 +32,7 @@
  */
 #include "nbhip_walk.h"
 
-#define NBH_GW_FCAP 1024   /* range-stack entries per wavefront */
-/* A depth-first descent pushes at most 7 entries more than it pops per
- * level; the stack keeps this much room below its capacity for the deepest
- * tree (63 levels), so throttled pops can always make progress. */
-#define NBH_GW_SLACK 448
+#define NBH_GW_FCAP 512    /* range-stack entries per wavefront */
 #define NBH_GW_SPLIT_T 3   /* split when min link level < median link level - 3 */
 
 __device__ __forceinline__ double nbh_uniform_d(double v)
@@ -150,6 +146,11 @@ __global__ __launch_bounds__(64 * WPB) void k_walk_group(const NodeG* __restrict
     if (valid_m == 0ull) return;
     const int M = scal->nmassive;
     const int sb = scal->sim_bits;
+    /* A depth-first descent pushes at most 7 entries more than it pops per
+     * level below the entry it starts from; the stack keeps that much room
+     * for the deepest tree of the state, so a throttled pop (one entry) can
+     * always proceed without overflowing. */
+    const int slack = 7 * (min(scal->maxdepth, 63) + 2);
     const int wave_base = s - lane;
     const double4 p = valid ? spos[s] : make_double4(0.0, 0.0, 0.0, 0.0);
     const int sim = valid ? simid_s[s] : -1;
@@ -206,6 +207,7 @@ __global__ __launch_bounds__(64 * WPB) void k_walk_group(const NodeG* __restrict
         auto flush_cells = [&](int cnt) {
             float tx = 0.0f, ty = 0.0f, tz = 0.0f;
             __builtin_amdgcn_wave_barrier();
+            #pragma unroll 2
             for (int j = 0; j < cnt; ++j) {
                 const float4 a = c0[j];
                 const float dx = a.x - rx, dy = a.y - ry, dz = a.z - rz;
@@ -247,6 +249,7 @@ __global__ __launch_bounds__(64 * WPB) void k_walk_group(const NodeG* __restrict
         auto flush_bodies = [&](int cnt) {
             float tx = 0.0f, ty = 0.0f, tz = 0.0f;
             __builtin_amdgcn_wave_barrier();
+            #pragma unroll 2
             for (int j = 0; j < cnt; ++j) {
                 const float4 a = bl[j];
                 const float dx = a.x - rx, dy = a.y - ry, dz = a.z - rz;
@@ -276,8 +279,8 @@ __global__ __launch_bounds__(64 * WPB) void k_walk_group(const NodeG* __restrict
             /* pop up to 8 ranges (64 nodes); throttled near the capacity so
              * that the pushes of this round always fit */
             int E = min(8, fsize);
-            if (fsize + 7 * E > NBH_GW_FCAP - NBH_GW_SLACK)
-                E = max(1, min(E, (NBH_GW_FCAP - NBH_GW_SLACK - fsize) / 7));
+            if (fsize + 7 * E > NBH_GW_FCAP - slack)
+                E = max(1, min(E, (NBH_GW_FCAP - slack - fsize) / 7));
             const int eidx = lane >> 3, sub = lane & 7;
             const int ent = eidx < E ? fr[fsize - 1 - eidx] : 0;
             const int cnt = (ent & 7) + 1;
