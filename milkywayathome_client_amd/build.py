@@ -7,6 +7,8 @@ recompiles (it is the "does it build" check); loaders never build.
 """
 from __future__ import annotations
 
+import glob
+import hashlib
 import os
 import shutil
 import subprocess
@@ -16,6 +18,10 @@ PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(PKG_DIR, "csrc")
 LIB_PATH = os.path.join(PKG_DIR, "libnbhip.so")
 MODELS_LIB_PATH = os.path.join(PKG_DIR, "libnbmodels.so")
+# Digest of the sources and flags the two libraries were built from, written
+# beside them by build() and checked by the loader (a library built from
+# other sources is refused, not used).
+STAMP_PATH = os.path.join(PKG_DIR, "libnbhip.srchash")
 
 SOURCES = ["api.hip", "tree.hip", "walk.hip", "walk_group.hip", "adjoint.hip", "histogram.hip",
            "exact.hip", "run.hip"]
@@ -33,6 +39,20 @@ HIPCC_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
 ]
+
+
+def source_hash() -> str:
+    """SHA-256 over every source file, header and compiler flag of the build."""
+    h = hashlib.sha256()
+    files = sorted(glob.glob(os.path.join(CSRC, "*")) +
+                   glob.glob(os.path.join(PKG_DIR, "..", "include", "*.h")))
+    for f in files:
+        if os.path.isfile(f):
+            h.update(os.path.basename(f).encode())
+            with open(f, "rb") as fh:
+                h.update(fh.read())
+    h.update(" ".join(HIPCC_FLAGS + SOURCES).encode())
+    return h.hexdigest()
 
 
 def hipcc_path() -> str:
@@ -69,6 +89,8 @@ def build(verbose: bool = False) -> str:
     if res.returncode != 0:
         raise RuntimeError("hipcc failed:\n" + res.stdout + res.stderr)
     os.replace(tmp, LIB_PATH)
+    with open(STAMP_PATH, "w") as f:
+        f.write(source_hash() + "\n")
     return LIB_PATH
 
 

@@ -362,6 +362,7 @@ void nbh_adj_force(nbhip_state* st)
                            st->perm, st->simid_s, st->sims, ga, ga + N, ga + 2 * N, A->g,
                            A->g + N, A->g + 2 * N, A->gnode, st->scal, o);
     /* roots first: a cell's record is complete before it is handed down */
+    if (!st->h_levels_valid && nbh_level_lists(st) != NBHIP_SUCCESS) return;
     for (int l = 0; l < 64; ++l) {
         int cnt = st->h_levels_valid ? st->h_level_count[l] : st->node_cap;
         if (cnt <= 0) continue;

@@ -104,7 +104,8 @@ static void nbh_free_all(nbhip_state* st)
                      st->bhi,  st->bdelta, st->bparent, st->node,  st->nquad,  st->nmid,
                      st->nlevel, st->nparent, st->nsim_node, st->level_list, st->nodef,
                      st->scal, st->sims, st->ignore, st->hist_buf, st->disp_buf, st->pos4,
-                     st->exact_part, st->nodeg, st->nkid, st->kbase, st->best_buf };
+                     st->exact_part, st->nodeg, st->nkid, st->kbase, st->best_buf,
+                     st->nrank, st->kid, st->blist, st->span, st->span_sorted };
     for (void* p : ptrs)
         if (p) (void) hipFree(p);
     if (st->graph_exec) (void) hipGraphExecDestroy(st->graph_exec);
@@ -211,11 +212,9 @@ static unsigned nbh_create(const nbhip_ctx* ctx, int device, int nsim, const int
     A(nsim_node, nc); A(level_list, nc);
     if (ctx->precision == NBHIP_PRECISION_MIXED || ctx->precision == NBHIP_PRECISION_MIXED_COHERENT)
         A(nodef, nc + 1);   /* + 1: walk prefetch */
-    if (ctx->precision == NBHIP_PRECISION_MIXED_GROUP) {
-        A(nodeg, nc + (size_t) nsim);
-        A(nkid, nc);
-        A(kbase, nc);
-    }
+    if (ctx->precision == NBHIP_PRECISION_MIXED_GROUP) A(nodeg, nc + (size_t) nsim);
+    A(nkid, nc); A(kbase, nc); A(nrank, nc); A(kid, nc); A(blist, nc); A(span, nc);
+    A(span_sorted, nc);
     if (ctx->criterion == NBHIP_EXACT) A(exact_part, 4 * (size_t) st->exact_seg * n);
     A(scal, 1);
     A(sims, (size_t) nsim);

@@ -87,6 +87,7 @@ struct DevScalars {
     unsigned long long counters[4];/* body-body, body-cell, opened, wave node steps */
     int halt;                      /* a step ended with a fatal status: later steps do nothing */
     unsigned steps_done;           /* KDK steps completed on the device */
+    int span_count;                /* cells summarised by k_sum_span */
     int level_count[64];           /* cells per level */
     int level_fill[64];
     int level_off[65];             /* exclusive scan of level_count */
@@ -155,8 +156,12 @@ struct nbhip_state {
     int *level_list;              /* cell pre-order indices grouped by level */
     NodeF *nodef;                 /* mixed-precision walk records (precision MIXED) */
     NodeG *nodeg;                 /* group walk table (precision MIXED_GROUP): nsim + node_cap */
-    unsigned *nkid;               /* children per node (group walk table build) */
+    unsigned *nkid;               /* children per node */
     unsigned *kbase;              /* exclusive scan of nkid: each cell's child block */
+    int *nrank;                   /* a node's rank among its parent's children (arrival order) */
+    int *kid;                     /* child lists: kid[kbase[p] .. + nkid[p]) */
+    int *blist;                   /* summaries: inside cells of each body block, by level */
+    int *span, *span_sorted;      /* summaries: cells spanning several body blocks */
     double *exact_part;           /* Exact criterion: per-segment partial sums */
     int exact_seg;                /* Exact criterion: most segments of any simulation */
 
@@ -200,6 +205,7 @@ struct nbhip_state {
 /* launchers (tree.hip) */
 void nbh_bbox(nbhip_state* st, int drift);
 void nbh_build(nbhip_state* st, int graph_safe);
+unsigned nbh_level_lists(nbhip_state* st);
 void nbh_step_end(nbhip_state* st);
 /* launchers (walk.hip) */
 void nbh_walk(nbhip_state* st, int kick, int count, double* out_ax, double* out_ay,

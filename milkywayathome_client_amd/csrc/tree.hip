@@ -149,6 +149,7 @@ __global__ __launch_bounds__(256) void k_rsize(int nsim, SimTab* __restrict__ si
         scal->ncell = 0;
         scal->nnode = 0;
         scal->maxdepth = 0;
+        scal->span_count = 0;
         scal->counters[0] = scal->counters[1] = scal->counters[2] = scal->counters[3] = 0ull;
     }
     if (threadIdx.x < 64) {
@@ -585,17 +586,24 @@ __global__ __launch_bounds__(1024) void k_scan_add(unsigned* __restrict__ data,
 }
 
 /* Exclusive scan of the children per node over the node table (length
- * scal->nnode, up to the node capacity): each cell's child block in the
- * group walk table.  Tiles of NBH_SCAN_TILE entries, then the tile totals
- * (one workgroup, 4096 at a time with a running carry), then the tile
- * offsets added back. */
+ * scal->nnode): each cell's child block in the group walk table.  Tiles of
+ * NBH_SCAN_TILE entries, then the tile totals (one workgroup, 4096 at a time
+ * with a running carry), then the tile offsets added back.  A node table
+ * that overflowed its capacity has no child counts: nothing is scanned. */
+__device__ __forceinline__ int nbh_nscan_len(const DevScalars* scal, int node_cap)
+{
+    if (scal->status & NBHIP_CELL_OVERFLOW_ERROR) return 0;
+    return min(scal->nnode, node_cap);
+}
+
 __global__ __launch_bounds__(1024) void k_nscan_tiles(const unsigned* __restrict__ in,
                                                       unsigned* __restrict__ out,
                                                       unsigned* __restrict__ part,
+                                                      int node_cap,
                                                       const DevScalars* __restrict__ scal)
 {
     __shared__ unsigned wsum[16];
-    const int nn = scal->nnode;
+    const int nn = nbh_nscan_len(scal, node_cap);
     if (blockIdx.x * NBH_SCAN_TILE >= nn) return;
     const int base = blockIdx.x * NBH_SCAN_TILE + threadIdx.x * 4;
     unsigned v[4];
@@ -612,11 +620,11 @@ __global__ __launch_bounds__(1024) void k_nscan_tiles(const unsigned* __restrict
     if (threadIdx.x == 0) part[blockIdx.x] = tot;
 }
 
-__global__ __launch_bounds__(1024) void k_nscan_part(unsigned* __restrict__ part,
+__global__ __launch_bounds__(1024) void k_nscan_part(unsigned* __restrict__ part, int node_cap,
                                                      const DevScalars* __restrict__ scal)
 {
     __shared__ unsigned wsum[16];
-    const int nb = (scal->nnode + NBH_SCAN_TILE - 1) / NBH_SCAN_TILE;
+    const int nb = (nbh_nscan_len(scal, node_cap) + NBH_SCAN_TILE - 1) / NBH_SCAN_TILE;
     unsigned carry = 0;
     for (int c0 = 0; c0 < nb; c0 += NBH_SCAN_TILE) {
         const int base = c0 + threadIdx.x * 4;
@@ -638,9 +646,10 @@ __global__ __launch_bounds__(1024) void k_nscan_part(unsigned* __restrict__ part
 
 __global__ __launch_bounds__(1024) void k_nscan_add(unsigned* __restrict__ out,
                                                     const unsigned* __restrict__ part,
+                                                    int node_cap,
                                                     const DevScalars* __restrict__ scal)
 {
-    const int nn = scal->nnode;
+    const int nn = nbh_nscan_len(scal, node_cap);
     if (blockIdx.x == 0 || blockIdx.x * NBH_SCAN_TILE >= nn) return;
     const int base = blockIdx.x * NBH_SCAN_TILE + threadIdx.x * 4;
     const unsigned add = part[blockIdx.x];
@@ -672,9 +681,9 @@ __global__ __launch_bounds__(256) void k_sim_roots(const int* __restrict__ simid
         if (nn > node_cap) scal->status |= NBHIP_CELL_OVERFLOW_ERROR;
     }
     if (a >= scal->nmassive) return;
-    if (nkid) {
-        /* the group walk's child counters of body a's new cells and its leaf
-         * (pre-order a + cincl[a-1] .. a + cincl[a]), counted by the emission */
+    {
+        /* child counters of body a's new cells and its leaf (pre-order
+         * a + cincl[a-1] .. a + cincl[a]), counted by the emission */
         const int lo = a + (a > 0 ? (int) cincl[a - 1] : 0);
         const int hi = a + (int) cincl[a];
         for (int i = lo; i <= hi && i < node_cap; ++i) nkid[i] = 0u;
@@ -861,7 +870,7 @@ __device__ __forceinline__ void nbh_emit_cells_body(
     const int* __restrict__ simid_s, int n, NodeW* __restrict__ node,
     double* __restrict__ nmid, int* __restrict__ nlevel, int* __restrict__ nparent,
     int* __restrict__ nsim, SimTab* __restrict__ sims, unsigned* __restrict__ nkid,
-    DevScalars* __restrict__ scal, int* lh)
+    int* __restrict__ nrank, DevScalars* __restrict__ scal, int* lh)
 {
     const int k = blockIdx.x * 256 + threadIdx.x;
     const int M = scal->nmassive;
@@ -896,7 +905,9 @@ __device__ __forceinline__ void nbh_emit_cells_body(
         const int par = (l == Lp + 1) ? top_parent : pre - 1;
         nbh_emit_cell(pre, l, next, par, sim, R, ux, uy, uz, node, nmid, nlevel, nparent, nsim,
                       sims);
-        if (nkid && par >= 0) atomicAdd(&nkid[par], 1u);
+        /* the cell's rank among its parent's children (arrival order; the
+         * summaries sort each child list into octant order) */
+        if (par >= 0) nrank[pre] = (int) atomicAdd(&nkid[par], 1u);
         atomicAdd(&lh[l], 1);
     }
 }
@@ -916,13 +927,14 @@ __global__ __launch_bounds__(256) void k_emit_cells(const int* __restrict__ blo,
                                                     int* __restrict__ nsim,
                                                     SimTab* __restrict__ sims,
                                                     unsigned* __restrict__ nkid,
+                                                    int* __restrict__ nrank,
                                                     DevScalars* __restrict__ scal)
 {
     __shared__ int lh[64];
     if (threadIdx.x < 64) lh[threadIdx.x] = 0;
     __syncthreads();
     nbh_emit_cells_body(blo, bhi, bdelta, bparent, dadj, cincl, su, simid_s, n, node, nmid,
-                        nlevel, nparent, nsim, sims, nkid, scal, lh);
+                        nlevel, nparent, nsim, sims, nkid, nrank, scal, lh);
     __syncthreads();
     if (threadIdx.x < 64 && lh[threadIdx.x])
         atomicAdd(&scal->level_count[threadIdx.x], lh[threadIdx.x]);
@@ -945,6 +957,7 @@ __global__ __launch_bounds__(256) void k_emit_leaves(const int* __restrict__ blo
                                                      SimTab* __restrict__ sims,
                                                      NodeF* __restrict__ nodef,
                                                      unsigned* __restrict__ nkid,
+                                                     int* __restrict__ nrank,
                                                      DevScalars* __restrict__ scal)
 {
     const int s = blockIdx.x * 256 + threadIdx.x;
@@ -969,7 +982,7 @@ __global__ __launch_bounds__(256) void k_emit_leaves(const int* __restrict__ blo
         atomicOr(&scal->status, NBHIP_ERROR);
         return;
     }
-    if (nkid) atomicAdd(&nkid[parent], 1u);
+    nrank[pre] = (int) atomicAdd(&nkid[parent], 1u);
     const double4 p = spos[s];
     NodeW b;
     b.x = p.x; b.y = p.y; b.z = p.z; b.m = p.w;
@@ -1056,13 +1069,26 @@ __global__ __launch_bounds__(256) void k_level_scatter(const int* __restrict__ n
 }
 
 /* ---------------------------------------------------------------------- */
-/* bottom-up summarisation, one launch per level, deepest first            */
+/* bottom-up summarisation: block-local subtrees, then the spanning cells  */
 /* ---------------------------------------------------------------------- */
 /* hackCofM + findRCrit + nbCheckTreeStructure (nbody/src/nbody_tree.c:
- * 345-424) and hackQuad (:70-121), children in octant (pre-)order.  All
- * children of a level-l cell are bodies or level-(l+1) cells, finished by an
- * earlier launch.  Grid-stride over the level's cells, so a fixed launch
- * shape serves any count (graph capture). */
+ * 345-424) and hackQuad (:70-121), children in octant (pre-)order.
+ *
+ * Two launches, no host synchronisation and no cross-workgroup waits:
+ *  - k_sum_blocks: workgroup b takes the massive bodies [b*NBH_SUM_BODIES,
+ *    (b+1)*NBH_SUM_BODIES) and the pre-order range [P0, P1) of their leaves
+ *    and of the cells whose first body they are.  A cell of that range whose
+ *    subtree ends inside it (Next <= P1) is "inside": all its descendants
+ *    are in the range, so the workgroup summarises its inside cells level by
+ *    level, deepest first, with a barrier between levels.  Cells whose
+ *    subtree runs past P1 ("spanning") are listed for the second launch.
+ *  - k_sum_span: one workgroup summarises the spanning cells level by level,
+ *    deepest first (their children are inside cells, bodies or deeper
+ *    spanning cells).
+ * A cell's children come from its child list (k_fill_kids): the pre-order
+ * indices of its children, sorted into octant order and checked against the
+ * Next links, then loaded together instead of one after another. */
+
 /* The group walk's table (precision MIXED_GROUP; T is null otherwise):
  * every cell writes its children's records into its child block
  * (slot nsim + kbase[cell] + rank, rank in octant order), and a root also
@@ -1098,47 +1124,104 @@ __device__ __forceinline__ NodeG nbh_group_record(const NodeW& w, int c,
     return r;
 }
 
-/* One cell's summary (see below); false on a broken Next link. */
-__device__ __forceinline__ bool nbh_summarize_cell(const int* __restrict__ level_list, int level,
-                                                   int i, NodeW* __restrict__ node,
-                                                   double* __restrict__ nquad,
-                                                   const double* __restrict__ nmid,
-                                                   const int* __restrict__ nsim,
-                                                   SimTab* __restrict__ sims,
-                                                   NodeF* __restrict__ nodef,
-                                                   const GroupTab& g,
-                                                   const nbhip_ctx& ctx, double rc_scale,
-                                                   DevScalars* scal, int nn)
+/* Child lists: kid[kbase[p] .. kbase[p] + nkid[p]) holds the children of
+ * cell p in arrival order (nrank, from the emission). */
+__global__ __launch_bounds__(256) void k_fill_kids(const int* __restrict__ nparent,
+                                                   const int* __restrict__ nrank,
+                                                   const unsigned* __restrict__ nkid,
+                                                   const unsigned* __restrict__ kbase,
+                                                   int* __restrict__ kid, int node_cap,
+                                                   DevScalars* __restrict__ scal)
 {
-    const int p = level_list[scal->level_off[level] + i];
-    const int sim = nsim[p];
-    const double rsize = sims[sim].rsize;
+    if (scal->status & (NBHIP_CELL_OVERFLOW_ERROR | NBHIP_ERROR)) return;
+    const int nn = min(scal->nnode, node_cap);
+    for (int q = blockIdx.x * 256 + threadIdx.x; q < nn; q += gridDim.x * 256) {
+        const int par = nparent[q];
+        if (par < 0) continue;
+        const int r = nrank[q];
+        if (par >= q || r < 0 || r >= (int) nkid[par] || r > 7) {
+            atomicOr(&scal->status, NBHIP_ERROR);
+            continue;
+        }
+        kid[kbase[par] + r] = q;
+    }
+}
+
+struct SumView {
+    NodeW* node;
+    double* nquad;
+    const double* nmid;
+    const int* nsim;
+    const int* nlevel;
+    const int* kid;
+    SimTab* sims;
+    NodeF* nodef;
+    GroupTab g;
+    nbhip_ctx ctx;
+    double rc_scale;
+    int nn;
+};
+
+/* One cell's summary; false on a child list that disagrees with the Next
+ * links (a corrupt table). */
+__device__ __forceinline__ bool nbh_summarize_cell(const SumView& v, int p)
+{
+    NodeW* __restrict__ node = v.node;
+    const int level = v.nlevel[p];
+    const int sim = v.nsim[p];
+    const double rsize = v.sims[sim].rsize;
     const int end = node[p].next;
-    if (p < 0 || end <= p || end > nn) {
-        return false;
-    }
-    const int nk = g.T ? (int) g.nkid[p] : 0;
-    const int kb = g.T ? g.nsim + (int) g.kbase[p] : 0;
-    int rank = 0;
+    const int nk = (int) v.g.nkid[p];
+    if (level < 0 || end <= p + 1 || end > v.nn || nk < 1 || nk > 8) return false;
+    const int kb = (int) v.g.kbase[p];
+    /* the children, sorted into pre-order (= octant order) */
+    int ch[8];
+    #pragma unroll
+    for (int j = 0; j < 8; ++j) ch[j] = j < nk ? v.kid[kb + j] : 0x7fffffff;
+    /* Batcher's odd-even merge network for 8 keys (19 compare-exchanges,
+     * static indices: the list stays in registers) */
+#define NBH_CE(a, b)                                                    \
+    do {                                                                \
+        const int lo_ = min(ch[a], ch[b]), hi_ = max(ch[a], ch[b]);     \
+        ch[a] = lo_;                                                    \
+        ch[b] = hi_;                                                    \
+    } while (0)
+    NBH_CE(0, 1); NBH_CE(2, 3); NBH_CE(4, 5); NBH_CE(6, 7);
+    NBH_CE(0, 2); NBH_CE(1, 3); NBH_CE(4, 6); NBH_CE(5, 7);
+    NBH_CE(1, 2); NBH_CE(5, 6);
+    NBH_CE(0, 4); NBH_CE(1, 5); NBH_CE(2, 6); NBH_CE(3, 7);
+    NBH_CE(2, 4); NBH_CE(3, 5);
+    NBH_CE(1, 2); NBH_CE(3, 4); NBH_CE(5, 6);
+#undef NBH_CE
+    NodeW w[8];
+    #pragma unroll
+    for (int j = 0; j < 8; ++j)
+        if (j < nk) w[j] = node[ch[j]];
+    /* the list must be the Next chain p+1 -> ... -> end */
+    if (ch[0] != p + 1) return false;
+    #pragma unroll
+    for (int j = 0; j < 8; ++j)
+        if (j < nk && w[j].next != (j + 1 < nk ? ch[j + 1] : end)) return false;
+
     double mass = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
-    for (int c = p + 1; c < end;) {
-        const NodeW ch = node[c];
-        mass += ch.m;
-        cx += ch.m * ch.x;
-        cy += ch.m * ch.y;
-        cz += ch.m * ch.z;
-        if (ch.next <= c || ch.next > end) {
-            return false;
+    #pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        if (j < nk) {
+            mass += w[j].m;
+            cx += w[j].m * w[j].x;
+            cy += w[j].m * w[j].y;
+            cz += w[j].m * w[j].z;
         }
-        if (g.T) {
-            if (rank >= nk) return false;
-            g.T[kb + rank] = nbh_group_record(ch, c, nquad, g, rc_scale, ctx.use_quad);
-        }
-        ++rank;
-        c = ch.next;
     }
-    if (g.T && rank != nk) return false;
-    const double mx = nmid[3 * p + 0], my = nmid[3 * p + 1], mz = nmid[3 * p + 2];
+    if (v.g.T) {
+        const int tb = v.g.nsim + kb;
+        #pragma unroll
+        for (int j = 0; j < 8; ++j)
+            if (j < nk)
+                v.g.T[tb + j] = nbh_group_record(w[j], ch[j], v.nquad, v.g, v.rc_scale,
+                                                 v.ctx.use_quad);
+    }
+    const double mx = v.nmid[3 * p + 0], my = v.nmid[3 * p + 1], mz = v.nmid[3 * p + 2];
     if (mass > 0.0) {
         cx /= mass;
         cy /= mass;
@@ -1150,115 +1233,184 @@ __device__ __forceinline__ bool nbh_summarize_cell(const int* __restrict__ level
     const double half = 0.5 * psize;
     if (cx < mx - half || cx > mx + half || cy < my - half || cy > my + half ||
         cz < mz - half || cz > mz + half)
-        atomicOr(&sims[sim].flags, NBHIP_TREE_STRUCTURE_ERROR);
-    const double rc2 = nbh_rcrit2(ctx, mx, my, mz, rsize, cx, cy, cz, psize);
+        atomicOr(&v.sims[sim].flags, NBHIP_TREE_STRUCTURE_ERROR);
+    const double rc2 = nbh_rcrit2(v.ctx, mx, my, mz, rsize, cx, cy, cz, psize);
 
-    if (ctx.use_quad) {
+    double* pq = &v.nquad[6 * (size_t) p];
+    if (v.ctx.use_quad) {
         double q0 = 0.0, q1 = 0.0, q2 = 0.0, q3 = 0.0, q4 = 0.0, q5 = 0.0;
-        for (int c = p + 1; c < end;) {
-            const NodeW ch = node[c];
-            const double dx = ch.x - cx, dy = ch.y - cy, dz = ch.z - cz;
+        #pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            if (j >= nk) continue;
+            const double dx = w[j].x - cx, dy = w[j].y - cy, dz = w[j].z - cz;
             const double drsq = dz * dz + (dy * dy + dx * dx);
-            const double m = ch.m;
+            const double m = w[j].m;
             double e0 = m * (3.0 * (dx * dx) - drsq);
             double e1 = m * (3.0 * (dx * dy));
             double e2 = m * (3.0 * (dx * dz));
             double e3 = m * (3.0 * (dy * dy) - drsq);
             double e4 = m * (3.0 * (dy * dz));
             double e5 = m * (3.0 * (dz * dz) - drsq);
-            if (ch.body < 0) {
-                const double* cq = &nquad[6 * c];
+            if (w[j].body < 0) {
+                const double* cq = &v.nquad[6 * (size_t) ch[j]];
                 e0 += cq[0]; e1 += cq[1]; e2 += cq[2];
                 e3 += cq[3]; e4 += cq[4]; e5 += cq[5];
             }
             q0 += e0; q1 += e1; q2 += e2; q3 += e3; q4 += e4; q5 += e5;
-            c = ch.next;
         }
-        double* pq = &nquad[6 * p];
         pq[0] = q0; pq[1] = q1; pq[2] = q2; pq[3] = q3; pq[4] = q4; pq[5] = q5;
-        if (nodef) {
-            NodeF& f = nodef[p];
+        if (v.nodef) {
+            NodeF& f = v.nodef[p];
             f.q[0] = (float) q0; f.q[1] = (float) q1; f.q[2] = (float) q2;
             f.q[3] = (float) q3; f.q[4] = (float) q4; f.q[5] = (float) q5;
         }
     } else {
-        double* pq = &nquad[6 * p];
         pq[0] = pq[1] = pq[2] = pq[3] = pq[4] = pq[5] = 0.0;
-        if (nodef)
-            for (int k = 0; k < 6; ++k) nodef[p].q[k] = 0.0f;
+        if (v.nodef)
+            for (int k = 0; k < 6; ++k) v.nodef[p].q[k] = 0.0f;
     }
-    NodeW w;
-    w.x = cx;
-    w.y = cy;
-    w.z = cz;
-    w.m = mass;
-    w.rcrit2 = rc2;
-    w.next = end;
-    w.body = -1;
-    node[p] = w;
-    if (g.T && level == 0) g.T[sim] = nbh_group_record(w, p, nquad, g, rc_scale, ctx.use_quad);
-    if (nodef) {
+    NodeW r;
+    r.x = cx;
+    r.y = cy;
+    r.z = cz;
+    r.m = mass;
+    r.rcrit2 = rc2;
+    r.next = end;
+    r.body = -1;
+    node[p] = r;
+    if (v.g.T && level == 0)
+        v.g.T[sim] = nbh_group_record(r, p, v.nquad, v.g, v.rc_scale, v.ctx.use_quad);
+    if (v.nodef) {
         /* the mixed walk's record: FP64 position, the rest in FP32 */
-        NodeF& f = nodef[p];
+        NodeF& f = v.nodef[p];
         f.x = cx;
         f.y = cy;
         f.z = cz;
         f.m = (float) mass;
         /* the coherent walk may open more (rc_scale >= 1; 1 otherwise) */
-        f.rcrit2 = (float) (rc2 * rc_scale);
+        f.rcrit2 = (float) (rc2 * v.rc_scale);
         f.next = end;
         f.body = -1;
     }
     return true;
 }
 
-__global__ __launch_bounds__(256) void k_summarize_level(const int* __restrict__ level_list,
-                                                         int level, NodeW* __restrict__ node,
-                                                         double* __restrict__ nquad,
-                                                         const double* __restrict__ nmid,
-                                                         const int* __restrict__ nsim,
-                                                         SimTab* __restrict__ sims,
-                                                         NodeF* __restrict__ nodef,
-                                                         GroupTab g,
-                                                         nbhip_ctx ctx, double rc_scale,
-                                                         DevScalars* scal)
+#define NBH_SUM_BODIES 1024    /* massive bodies per k_sum_blocks workgroup */
+#define NBH_SUM_THREADS 256
+#define NBH_SPAN_THREADS 512
+
+/* Levels holding cells in [lo, hi): counts into cnt[64] (LDS), then an
+ * exclusive scan into off[65]; callers add their own cells. */
+__global__ __launch_bounds__(NBH_SUM_THREADS) void k_sum_blocks(SumView v,
+                                                                const unsigned* __restrict__ cincl,
+                                                                int* __restrict__ blist,
+                                                                int* __restrict__ span,
+                                                                DevScalars* __restrict__ scal)
 {
+    __shared__ int cnt[64];
+    __shared__ int off[65];
+    __shared__ int fill[64];
+    const int tid = threadIdx.x;
     if (scal->status & (NBHIP_CELL_OVERFLOW_ERROR | NBHIP_ERROR)) return;
-    const int cnt = scal->level_count[level];
-    const int nn = scal->nnode;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
-        if (!nbh_summarize_cell(level_list, level, i, node, nquad, nmid, nsim, sims, nodef, g,
-                                ctx, rc_scale, scal, nn)) {
+    const int M = scal->nmassive;
+    const int b0 = blockIdx.x * NBH_SUM_BODIES;
+    if (b0 >= M) return;
+    const int b1 = min(b0 + NBH_SUM_BODIES, M);
+    const int P0 = b0 + (b0 > 0 ? (int) cincl[b0 - 1] : 0);
+    const int P1 = min((b1 - 1) + (int) cincl[b1 - 1] + 1, v.nn);
+    if (tid < 64) {
+        cnt[tid] = 0;
+        fill[tid] = 0;
+    }
+    __syncthreads();
+    /* inside cells per level; spanning cells to the second launch */
+    for (int q = P0 + tid; q < P1; q += NBH_SUM_THREADS) {
+        const int l = v.nlevel[q];
+        if (l < 0) continue;                      /* a body's leaf */
+        if (l > 63) {
             atomicOr(&scal->status, NBHIP_ERROR);
-            return;
+            continue;
+        }
+        if (v.node[q].next <= P1) {
+            atomicAdd(&cnt[l], 1);
+        } else {
+            const int k = atomicAdd(&scal->span_count, 1);
+            span[k] = q;
         }
     }
-}
-
-/* Levels [lo, 63], deepest first, in one workgroup with a barrier between
- * levels: the deep end of a tree has few cells per level, so one launch
- * replaces up to 64 - lo launches that mostly find their level empty (the
- * graph-captured step, which cannot read the level counts back). */
-#define NBH_TAIL_THREADS 1024
-#define NBH_TAIL_LEVEL 24
-__global__ __launch_bounds__(NBH_TAIL_THREADS) void k_summarize_tail(
-    const int* __restrict__ level_list, int lo, NodeW* __restrict__ node,
-    double* __restrict__ nquad, const double* __restrict__ nmid, const int* __restrict__ nsim,
-    SimTab* __restrict__ sims, NodeF* __restrict__ nodef, GroupTab g, nbhip_ctx ctx,
-    double rc_scale, DevScalars* scal)
-{
-    const int nn = scal->nnode;
-    for (int level = 63; level >= lo; --level) {
-        /* uniform: read by every thread after the previous level's barrier */
-        if (scal->status & (NBHIP_CELL_OVERFLOW_ERROR | NBHIP_ERROR)) return;
-        const int cnt = scal->level_count[level];
-        if (cnt == 0) continue;
-        for (int i = threadIdx.x; i < cnt; i += NBH_TAIL_THREADS)
-            if (!nbh_summarize_cell(level_list, level, i, node, nquad, nmid, nsim, sims, nodef,
-                                    g, ctx, rc_scale, scal, nn))
-                atomicOr(&scal->status, NBHIP_ERROR);
+    __syncthreads();
+    if (tid == 0) {
+        int acc = 0;
+        for (int l = 0; l < 64; ++l) {
+            off[l] = acc;
+            acc += cnt[l];
+        }
+        off[64] = acc;
+    }
+    __syncthreads();
+    /* the block's inside cells grouped by level, stored in its own slice
+     * [P0, P0 + count) of blist (slices of different blocks are disjoint) */
+    for (int q = P0 + tid; q < P1; q += NBH_SUM_THREADS) {
+        const int l = v.nlevel[q];
+        if (l < 0 || l > 63 || v.node[q].next > P1) continue;
+        blist[P0 + off[l] + atomicAdd(&fill[l], 1)] = q;
+    }
+    __syncthreads();
+    bool ok = true;
+    for (int l = 63; l >= 0; --l) {
+        const int c = off[l + 1] - off[l];
+        if (c == 0) continue;                     /* uniform */
+        for (int i = tid; i < c; i += NBH_SUM_THREADS)
+            ok = nbh_summarize_cell(v, blist[P0 + off[l] + i]) && ok;
         __syncthreads();
     }
+    if (!ok) atomicOr(&scal->status, NBHIP_ERROR);
+}
+
+/* The spanning cells, one workgroup: grouped by level (counting sort into
+ * span_sorted), then summarised deepest level first. */
+__global__ __launch_bounds__(NBH_SPAN_THREADS) void k_sum_span(SumView v,
+                                                               const int* __restrict__ span,
+                                                               int* __restrict__ span_sorted,
+                                                               DevScalars* __restrict__ scal)
+{
+    __shared__ int cnt[64];
+    __shared__ int off[65];
+    __shared__ int fill[64];
+    const int tid = threadIdx.x;
+    if (scal->status & (NBHIP_CELL_OVERFLOW_ERROR | NBHIP_ERROR)) return;
+    const int S = min(scal->span_count, v.nn);
+    if (tid < 64) {
+        cnt[tid] = 0;
+        fill[tid] = 0;
+    }
+    __syncthreads();
+    for (int i = tid; i < S; i += NBH_SPAN_THREADS) atomicAdd(&cnt[v.nlevel[span[i]]], 1);
+    __syncthreads();
+    if (tid == 0) {
+        int acc = 0;
+        for (int l = 0; l < 64; ++l) {
+            off[l] = acc;
+            acc += cnt[l];
+        }
+        off[64] = acc;
+    }
+    __syncthreads();
+    for (int i = tid; i < S; i += NBH_SPAN_THREADS) {
+        const int q = span[i];
+        const int l = v.nlevel[q];
+        span_sorted[off[l] + atomicAdd(&fill[l], 1)] = q;
+    }
+    __syncthreads();
+    bool ok = true;
+    for (int l = 63; l >= 0; --l) {
+        const int c = off[l + 1] - off[l];
+        if (c == 0) continue;                     /* uniform */
+        for (int i = tid; i < c; i += NBH_SPAN_THREADS)
+            ok = nbh_summarize_cell(v, span_sorted[off[l] + i]) && ok;
+        __syncthreads();
+    }
+    if (!ok) atomicOr(&scal->status, NBHIP_ERROR);
 }
 
 /* End of a KDK step (nbRunSystemPlain's loop, nbody/src/nbody_plain.c:
@@ -1312,8 +1464,35 @@ static void nbh_mark(nbhip_state* st, int k)
 }
 
 /* graph_safe: no host synchronisation (all 64 level launches, grid-stride) */
+/* Level lists of the current tree (cells grouped by level, in level_list
+ * with scal->level_off) and their counts on the host: the adjoint's
+ * top-down pass over the cells needs them; the summaries do not. */
+__global__ void k_level_fill_reset(DevScalars* __restrict__ scal)
+{
+    if (threadIdx.x < 64) scal->level_fill[threadIdx.x] = 0;
+}
+
+unsigned nbh_level_lists(nbhip_state* st)
+{
+    hipStream_t s = st->stream;
+    hipLaunchKernelGGL(k_level_fill_reset, dim3(1), dim3(64), 0, s, st->scal);
+    hipLaunchKernelGGL(k_level_scatter, dim3(nbh_blocks(2 * st->n + 1024, NBH_LS_CHUNK)),
+                       dim3(256), 0, s, st->nlevel, st->level_list, st->scal);
+    hipError_t e = hipMemcpyAsync(st->h_counts_pinned, st->scal->level_count, 64 * sizeof(int),
+                                  hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+        nbh_set_error("level lists: %s", hipGetErrorString(e));
+        return NBHIP_GPU_ERROR;
+    }
+    memcpy(st->h_level_count, st->h_counts_pinned, sizeof(st->h_level_count));
+    st->h_levels_valid = 1;
+    return NBHIP_SUCCESS;
+}
+
 void nbh_build(nbhip_state* st, int graph_safe)
 {
+    (void) graph_safe;             /* no step of the build reads anything back */
     const int n = st->n;
     const int g256 = nbh_blocks(n, 256);
     hipStream_t s = st->stream;
@@ -1359,62 +1538,42 @@ void nbh_build(nbhip_state* st, int graph_safe)
     hipLaunchKernelGGL(k_emit_cells, dim3(g256), dim3(256), 0, s, st->blo, st->bhi, st->bdelta,
                        st->bparent, st->dadj, st->cincl, st->su, st->simid_s, n, st->node,
                        st->nmid, st->nlevel, st->nparent, st->nsim_node, st->sims, st->nkid,
-                       st->scal);
+                       st->nrank, st->scal);
     hipLaunchKernelGGL(k_emit_leaves, dim3(g256), dim3(256), 0, s, st->blo, st->bdelta,
                        st->bparent, st->dadj, st->cincl, st->spos, st->simid_s, st->node,
                        st->nmid, st->nlevel, st->nparent, st->nsim_node, st->sims, st->nodef,
-                       st->nkid, st->scal);
-    if (st->nodeg) {
-        /* each cell's child block in the group walk table */
-        const int nt = nbh_blocks(st->node_cap, NBH_SCAN_TILE);
-        hipLaunchKernelGGL(k_nscan_tiles, dim3(nt), dim3(1024), 0, s, st->nkid, st->kbase,
-                           st->scan_part, st->scal);
-        hipLaunchKernelGGL(k_nscan_part, dim3(1), dim3(1024), 0, s, st->scan_part, st->scal);
-        hipLaunchKernelGGL(k_nscan_add, dim3(nt), dim3(1024), 0, s, st->kbase, st->scan_part,
-                           st->scal);
-    }
-    /* the per-level cell counts are final once the cells are emitted: read
-     * them back (eager builds) while the level lists are being built */
-    const bool readback = !graph_safe &&
-        hipMemcpyAsync(st->h_counts_pinned, st->scal->level_count, 64 * sizeof(int),
-                       hipMemcpyDeviceToHost, s) == hipSuccess &&
-        hipEventRecord(st->ev_counts, s) == hipSuccess;
-    hipLaunchKernelGGL(k_level_scatter, dim3(nbh_blocks(2 * n + 1024, NBH_LS_CHUNK)), dim3(256),
-                       0, s, st->nlevel, st->level_list, st->scal);
+                       st->nkid, st->nrank, st->scal);
+    /* each cell's child block (kbase) and child list (kid) */
+    const int nt = nbh_blocks(st->node_cap, NBH_SCAN_TILE);
+    hipLaunchKernelGGL(k_nscan_tiles, dim3(nt), dim3(1024), 0, s, st->nkid, st->kbase,
+                       st->scan_part, st->node_cap, st->scal);
+    hipLaunchKernelGGL(k_nscan_part, dim3(1), dim3(1024), 0, s, st->scan_part, st->node_cap,
+                       st->scal);
+    hipLaunchKernelGGL(k_nscan_add, dim3(nt), dim3(1024), 0, s, st->kbase, st->scan_part,
+                       st->node_cap, st->scal);
+    hipLaunchKernelGGL(k_fill_kids, dim3(nbh_blocks(st->node_cap, 256 * 8)), dim3(256), 0, s,
+                       st->nparent, st->nrank, st->nkid, st->kbase, st->kid, st->node_cap,
+                       st->scal);
     nbh_mark(st, 3);
-    int counts[64];
-    int top = 63;
-    const bool have = readback && hipEventSynchronize(st->ev_counts) == hipSuccess;
-    if (have) {
-        memcpy(counts, st->h_counts_pinned, sizeof(counts));
-        top = 0;
-        for (int l = 0; l < 64; ++l)
-            if (counts[l] > 0) top = l;
-    }
-    st->h_levels_valid = have ? 1 : 0;
-    if (have) memcpy(st->h_level_count, counts, sizeof(counts));
-    const GroupTab gt = { st->nodeg, st->nkid, st->kbase, st->nsim };
-    /* 64-lane workgroups: a level's cells spread over more CUs */
-    const int gmax = nbh_blocks(n, 64 * 8) + 1;
-    if (!have) {
-        /* without the counts: the deep levels in one single-workgroup launch */
-        hipLaunchKernelGGL(k_summarize_tail, dim3(1), dim3(NBH_TAIL_THREADS), 0, s,
-                           st->level_list, NBH_TAIL_LEVEL, st->node, st->nquad, st->nmid,
-                           st->nsim_node, st->sims, st->nodef, gt, st->ctx, st->open_scale,
-                           st->scal);
-        top = NBH_TAIL_LEVEL - 1;
-    }
-    for (int l = top; l >= 0; --l) {
-        if (have && counts[l] == 0) continue;
-        int g = have ? nbh_blocks(counts[l], 64) : gmax;
-        if (!have && 3 * l < 40) {
-            /* a level holds at most 8^l cells per simulation: no larger grid */
-            const long long cap = (long long) st->nsim << (3 * l);
-            if (cap < (long long) g * 64) g = nbh_blocks((int) cap, 64);
-        }
-        hipLaunchKernelGGL(k_summarize_level, dim3(g), dim3(64), 0, s, st->level_list, l,
-                           st->node, st->nquad, st->nmid, st->nsim_node, st->sims, st->nodef,
-                           gt, st->ctx, st->open_scale, st->scal);
-    }
+    /* level lists are built only for the adjoint's top-down pass (on its
+     * request: nbh_level_lists) */
+    st->h_levels_valid = 0;
+    SumView v;
+    v.node = st->node;
+    v.nquad = st->nquad;
+    v.nmid = st->nmid;
+    v.nsim = st->nsim_node;
+    v.nlevel = st->nlevel;
+    v.kid = st->kid;
+    v.sims = st->sims;
+    v.nodef = st->nodef;
+    v.g = GroupTab{ st->nodeg, st->nkid, st->kbase, st->nsim };
+    v.ctx = st->ctx;
+    v.rc_scale = st->open_scale;
+    v.nn = st->node_cap;
+    hipLaunchKernelGGL(k_sum_blocks, dim3(nbh_blocks(n, NBH_SUM_BODIES)), dim3(NBH_SUM_THREADS),
+                       0, s, v, st->cincl, st->blist, st->span, st->scal);
+    hipLaunchKernelGGL(k_sum_span, dim3(1), dim3(NBH_SPAN_THREADS), 0, s, v, st->span,
+                       st->span_sorted, st->scal);
     nbh_mark(st, 4);
 }

@@ -208,6 +208,11 @@ def from_counts(counts, hp: HistogramParams, total_simulated: int,
     return h
 
 
+# Largest transport problem emd_calc solves as a linear program (sparse
+# constraints: memory grows with the number of flows, (lambda x beta bins)^2).
+EMD_LP_MAX_FLOWS = 4_000_000
+
+
 def emd_calc(w1, lam1, beta1, w2, lam2, beta2) -> float:
     """Minimum total transport cost over the total flow, min(sum w1, sum w2),
     with ground distance sqrt(dlambda^2 + dbeta^2) (emdCalc).  NaN when a
@@ -231,16 +236,22 @@ def emd_calc(w1, lam1, beta1, w2, lam2, beta2) -> float:
         d = np.concatenate([w1 / s1, -w2 / s2])[order]
         cdf = np.cumsum(d)[:-1]
         return float(np.sum(np.abs(cdf) * np.diff(xs)))
+    from scipy import sparse
     from scipy.optimize import linprog
     n1, n2 = w1.size, w2.size
+    if n1 * n2 > EMD_LP_MAX_FLOWS:
+        raise ValueError(f"emd_calc: {n1} x {n2} bins is {n1 * n2} transport variables, "
+                         f"above the {EMD_LP_MAX_FLOWS} this solver takes")
     cost = np.sqrt((lam1[:, None] - lam2[None, :]) ** 2 + (beta1[:, None] - beta2[None, :]) ** 2)
-    a_ub = np.zeros((n1 + n2, n1 * n2))
-    for i in range(n1):
-        a_ub[i, i * n2:(i + 1) * n2] = 1.0
-    for j in range(n2):
-        a_ub[n1 + j, j::n2] = 1.0
+    # flow f[i, j] at column i * n2 + j: row i sums the flows out of source bin
+    # i, row n1 + j the flows into target bin j (two nonzeros per column)
+    cols = np.arange(n1 * n2)
+    rows = np.concatenate([cols // n2, n1 + cols % n2])
+    a_ub = sparse.csr_matrix((np.ones(2 * cols.size), (rows, np.concatenate([cols, cols]))),
+                             shape=(n1 + n2, n1 * n2))
+    a_eq = sparse.csr_matrix(np.ones((1, n1 * n2)))
     res = linprog(cost.ravel(), A_ub=a_ub, b_ub=np.concatenate([w1, w2]),
-                  A_eq=np.ones((1, n1 * n2)), b_eq=[flow], bounds=(0, None), method="highs")
+                  A_eq=a_eq, b_eq=[flow], bounds=(0, None), method="highs")
     if not res.success:
         return math.nan
     return float(res.fun / flow)

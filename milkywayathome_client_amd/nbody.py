@@ -144,6 +144,12 @@ def load_library() -> C.CDLL:
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(
             f"{LIB_PATH} is missing: build it with `python -m milkywayathome_client_amd.build`")
+    from . import build as _build
+    stamp = open(_build.STAMP_PATH).read().strip() if os.path.exists(_build.STAMP_PATH) else ""
+    if stamp != _build.source_hash():
+        raise RuntimeError(
+            f"{LIB_PATH} was not built from the current sources (digest "
+            f"{stamp[:12] or 'missing'}): rebuild with `python -m milkywayathome_client_amd.build`")
     lib = C.CDLL(LIB_PATH)
     u = C.c_uint
     st = C.c_void_p

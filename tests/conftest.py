@@ -12,6 +12,9 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP library)")
 
 
+C_CLIENT = os.path.join(ROOT, "tests", "c", "run_system_client")
+
+
 def _ensure_built():
     from milkywayathome_client_amd import build as b
     if not (os.path.exists(b.LIB_PATH) and os.path.exists(b.MODELS_LIB_PATH)):
@@ -19,6 +22,9 @@ def _ensure_built():
     from oracle import oracle
     if not os.path.exists(oracle.LIB_PATH):
         oracle.build()
+    if not os.path.exists(C_CLIENT):
+        import subprocess
+        subprocess.run(["make", "-s", "-C", os.path.dirname(C_CLIENT)], check=True)
 
 
 _ensure_built()

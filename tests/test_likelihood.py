@@ -82,6 +82,25 @@ def test_one_dimensional_emd_equals_linear_program():
     assert L.emd_calc(w, lam, beta, ws, lam, beta) == pytest.approx(lam[1] - lam[0])
 
 
+def test_two_dimensional_emd_on_a_50_by_20_grid():
+    """A 50 x 20 lambda-beta grid (10^6 flows, sparse constraints): moving
+    all weight one beta row costs one row spacing; a histogram against
+    itself costs 0.  Larger problems are refused, not attempted."""
+    lam_c = np.linspace(-150, 150, 50)
+    beta_c = np.linspace(-15, 15, 20)
+    lam, beta = (a.ravel() for a in np.meshgrid(lam_c, beta_c, indexing="ij"))
+    rng = np.random.default_rng(5)
+    w = np.zeros((50, 20))
+    w[:, 7] = rng.uniform(0.5, 1.0, size=50)
+    w /= w.sum()
+    assert L.emd_calc(w.ravel(), lam, beta, np.roll(w, 1, axis=1).ravel(), lam, beta) == \
+        pytest.approx(beta_c[1] - beta_c[0], rel=1e-9)
+    big = np.ones(2001)
+    with pytest.raises(ValueError, match="transport variables"):
+        L.emd_calc(big, np.arange(2001.0), np.arange(2001.0), big, np.arange(2001.0),
+                   np.arange(2001.0))
+
+
 def _hist(var, err, n, tot, mass, use=None):
     nb = len(var)
     return L.Histogram(lambda_bins=nb, beta_bins=1, total_num=n, total_simulated=tot,
