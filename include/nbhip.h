@@ -28,6 +28,7 @@ extern "C" {
 #define NBHIP_ERROR                  (1u << 1)
 #define NBHIP_TREE_STRUCTURE_ERROR   (1u << 3)
 #define NBHIP_TREE_INCEST_FATAL      (1u << 4)
+#define NBHIP_CHECKPOINT_ERROR       (1u << 6)
 #define NBHIP_GPU_ERROR              (1u << 7)   /* NBODY_CL_ERROR */
 #define NBHIP_CAPABILITY_ERROR       (1u << 8)
 #define NBHIP_UNSUPPORTED            (1u << 11)
@@ -415,6 +416,74 @@ unsigned nbhip_orbit_tangent(const nbhip_potential* pot, const double* pos, cons
                              double* dpos, double* dvel);
 unsigned nbhip_adjoint_backward(nbhip_state* st, const double* gpos, const double* gvel,
                                 double* gpos0, double* gvel0, double* gmass, double* gdt);
+
+/* ---- checkpoint files (nbody/src/nbody_checkpoint.c) ----------------------
+ * The reference's checkpoint layout (nbody_checkpoint.c:76-104 header,
+ * nbFreezeState :488-536, nbThawState :396-486) for x86-64 hosts with real =
+ * double: the "mwnbody" header (versions, nbody, step, sizeof(real),
+ * sizeof(void*), nOrbitTrace, nShiftLMC, treeIncest, tree rsize, the raw
+ * NBodyCtx), a size_t and the raw NBodyState, bodytab and
+ * bestLikelihoodBodyTab as 96-byte Body records, the orbit trace and LMC
+ * shifts as 32-byte mwvectors, "end"; the file is sized as the reference
+ * sizes it (:213-214).  The NBodyCtx and NBodyState images are the host
+ * program's and are written and returned byte for byte. */
+typedef struct nbhip_checkpoint_header {
+    unsigned major_version, minor_version;   /* NBODY_VERSION_MAJOR / _MINOR */
+    unsigned nbody, step;                    /* st->nbody, st->step */
+    unsigned real_size, ptr_size;            /* on read: sizeof(real), sizeof(void*) of the writer */
+    unsigned n_orbit_trace, n_shift_lmc;     /* vectors that follow the bodies */
+    unsigned tree_incest;                    /* st->treeIncest */
+    double rsize;                            /* st->tree.rsize */
+} nbhip_checkpoint_header;
+
+typedef struct nbhip_checkpoint_io {
+    unsigned major_version, minor_version;   /* written; on read the expected versions */
+    void* ctx;                               /* NBodyCtx image (ctx_size bytes, required) */
+    size_t ctx_size;                         /* sizeof(NBodyCtx): 592 on x86-64 */
+    void* state;                             /* NBodyState image (state_size bytes) */
+    size_t state_size;                       /* sizeof(NBodyState); on read 0 accepts any */
+    short* type;                             /* Body.type per body (write: NULL gives 1, or -1 for
+                                                bodies flagged by nbhip_set_ignore; read: filled
+                                                when not NULL) */
+    unsigned* id;                            /* Body.id per body (write: NULL gives the index) */
+    double* best_pos;                        /* bestLikelihoodBodyTab positions, n x 3 (write:
+                                                NULL gives the kept best-likelihood bodies of
+                                                the state, else the current ones) */
+    double* best_vel;                        /* ... and velocities */
+    double* orbit_trace;                     /* mwvector x y z w per entry */
+    unsigned n_orbit_trace;                  /* entries (read: capacity in, count out) */
+    double* shift_lmc;                       /* mwvector per entry (LMC reflex accelerations) */
+    unsigned n_shift_lmc;                    /* entries (read: capacity in, count out) */
+    double lmc_pos[4], lmc_vel[4];           /* st->LMCpos, st->LMCvel (with shifts only) */
+} nbhip_checkpoint_io;
+
+/* Host-only halves (no device): nbFreezeState's file from host arrays
+ * (pos, vel n x 3, mass n; hdr gives nbody, step, rsize, tree_incest; the
+ * file is written to path.tmp<pid> and renamed over path, as
+ * nbWriteCheckpointWithTmpFile does), and nbThawState's checks
+ * (nbVerifyCheckpointHeader: "mwnbody", file size, real and pointer size,
+ * version; the end marker) and contents; pos, vel, mass may be NULL to read
+ * the header alone.  A file that fails a check -> NBHIP_CHECKPOINT_ERROR. */
+unsigned nbhip_checkpoint_pack(const char* path, const nbhip_checkpoint_header* hdr,
+                               const nbhip_checkpoint_io* io, const double* pos,
+                               const double* vel, const double* mass);
+unsigned nbhip_checkpoint_unpack(const char* path, nbhip_checkpoint_header* hdr,
+                                 nbhip_checkpoint_io* io, double* pos, double* vel,
+                                 double* mass);
+
+/* nbWriteCheckpoint (nbody_checkpoint.c:629-672) for a GPU state: marshals
+ * the bodies out (nbMarshalBodies, as nbCheckpointCL does,
+ * nbody/src/nbody_cl.c:1549-1570) and writes them with the state's step,
+ * root size and incest flag.  One simulation per file. */
+unsigned nbhip_checkpoint_write(nbhip_state* st, const char* path, const nbhip_checkpoint_io* io);
+
+/* nbReadCheckpoint (nbody_checkpoint.c:584-610) into a GPU state created for
+ * the file's body count: uploads the bodies, restores step, root size and
+ * incest flag (nbhip_restore) and the best-likelihood bodies
+ * (nbhip_best_bodies), fills io's ctx / state images and optional arrays.
+ * The next nbhip_run evaluates the force at the restored positions, as
+ * nbRunSystemPlain does on resume (nbody/src/nbody_plain.c:388). */
+unsigned nbhip_checkpoint_read(nbhip_state* st, const char* path, nbhip_checkpoint_io* io);
 
 /* Waits for all work queued by the library. */
 unsigned nbhip_sync(nbhip_state* st);

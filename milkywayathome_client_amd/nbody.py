@@ -117,6 +117,76 @@ class _BestLike(C.Structure):
                 ("count", C.c_uint), ("evaluated", C.c_uint)]
 
 
+class _CkptHeader(C.Structure):
+    _fields_ = [("major_version", C.c_uint), ("minor_version", C.c_uint), ("nbody", C.c_uint),
+                ("step", C.c_uint), ("real_size", C.c_uint), ("ptr_size", C.c_uint),
+                ("n_orbit_trace", C.c_uint), ("n_shift_lmc", C.c_uint),
+                ("tree_incest", C.c_uint), ("rsize", C.c_double)]
+
+
+class _CkptIO(C.Structure):
+    _fields_ = [("major_version", C.c_uint), ("minor_version", C.c_uint),
+                ("ctx", C.c_void_p), ("ctx_size", C.c_size_t),
+                ("state", C.c_void_p), ("state_size", C.c_size_t),
+                ("type", C.POINTER(C.c_short)), ("id", C.POINTER(C.c_uint)),
+                ("best_pos", C.POINTER(C.c_double)), ("best_vel", C.POINTER(C.c_double)),
+                ("orbit_trace", C.POINTER(C.c_double)), ("n_orbit_trace", C.c_uint),
+                ("shift_lmc", C.POINTER(C.c_double)), ("n_shift_lmc", C.c_uint),
+                ("lmc_pos", C.c_double * 4), ("lmc_vel", C.c_double * 4)]
+
+
+# The reference's NBodyCtx as an x86-64 build lays it out
+# (nbody/include/nbody_types.h:450-521 with Potential,
+# nbody/include/nbody_potential_types.h:42-175; mwbool is a char, enums are
+# ints, and the MW_ALIGN_TYPE structs are 16-byte aligned, which the explicit
+# pads reproduce).  A checkpoint file carries it raw in its header.
+class _RefSpherical(C.Structure):
+    _fields_ = [("type", C.c_int), ("mass", C.c_double), ("scale", C.c_double),
+                ("_pad", C.c_char * 8)]
+
+
+class _RefDisk(C.Structure):
+    _fields_ = [("type", C.c_int), ("mass", C.c_double), ("scaleLength", C.c_double),
+                ("scaleHeight", C.c_double), ("patternSpeed", C.c_double),
+                ("startAngle", C.c_double)]
+
+
+class _RefHalo(C.Structure):
+    _fields_ = [("type", C.c_int)] + [(k, C.c_double) for k in (
+        "vhalo", "scaleLength", "flattenZ", "flattenY", "flattenX", "triaxAngle", "c1", "c2",
+        "c3", "mass", "gamma", "lambda_", "rho0")]
+
+
+class _RefPotential(C.Structure):
+    _fields_ = [("sphere", _RefSpherical), ("disk", _RefDisk), ("disk2", _RefDisk),
+                ("halo", _RefHalo), ("rings", C.c_void_p), ("_pad", C.c_char * 8)]
+
+
+class _RefNBodyCtx(C.Structure):
+    _fields_ = [(k, C.c_double) for k in (
+        "eps2", "theta", "timestep", "timeEvolve", "timeBack", "treeRSize", "sunGCDist",
+        "sunVelx", "sunVely", "sunVelz", "NGPdec", "NGPra", "lNCP", "b", "r", "vx", "vy",
+        "vz")] + [("criterion", C.c_int), ("potentialType", C.c_int)] + [(k, C.c_char) for k in (
+        "Nstep_control", "useBestLike", "useVelDisp", "useBetaDisp", "useBetaComp", "useVlos",
+        "useDist", "usePropMot", "MultiOutput", "InitialOutput", "SimpleOutput", "useQuad",
+        "allowIncest", "quietErrors")] + [(k, C.c_double) for k in (
+        "BestLikeStart", "OutputFreq", "BetaSigma", "VelSigma", "DistSigma", "PMSigma",
+        "IterMax", "BetaCorrect", "VelCorrect", "DistCorrect", "PMCorrect")] + [
+        ("LMC", C.c_char), ("LMCmass", C.c_double), ("LMCscale", C.c_double),
+        ("LMCDynaFric", C.c_char), ("coulomb_log", C.c_double), ("calibrationRuns", C.c_uint),
+        ("Ntsteps", C.c_double), ("checkpointT", C.c_int64), ("nStep", C.c_uint),
+        ("_pad", C.c_char * 12), ("pot", _RefPotential)]
+
+
+assert C.sizeof(_RefSpherical) == 32 and C.sizeof(_RefPotential) == 256
+assert _RefNBodyCtx.pot.offset == 336 and C.sizeof(_RefNBodyCtx) == 592
+
+# NBODY_VERSION_MAJOR / _MINOR of the reference build (nbody/CMakeLists.txt:24-27)
+NBODY_VERSION = (1, 93)
+NBODY_CHECKPOINT_ERROR = 1 << 6
+# sizeof(NBodyState) of an x86-64 build without OpenCL
+NBODY_STATE_SIZE = 384
+
 EXPORTED = [
     "nbhip_create", "nbhip_upload", "nbhip_download", "nbhip_gravity", "nbhip_step",
     "nbhip_run", "nbhip_steps", "nbhip_tree_info_get", "nbhip_tree_download",
@@ -127,7 +197,8 @@ EXPORTED = [
     "nbhip_adjoint_forward", "nbhip_adjoint_backward", "nbhip_gravity_vjp", "nbhip_set_ignore",
     "nbhip_histogram", "nbhip_orbit_tangent", "nbhip_restore", "nbhip_histogram_dispersion",
     "nbhip_set_open_scale", "nbhip_batch_tree_download", "nbhip_histogram_noalloc",
-    "nbhip_run_system", "nbhip_best_bodies",
+    "nbhip_run_system", "nbhip_best_bodies", "nbhip_checkpoint_pack", "nbhip_checkpoint_unpack",
+    "nbhip_checkpoint_write", "nbhip_checkpoint_read",
 ]
 
 _LIB: Optional[C.CDLL] = None
@@ -198,6 +269,12 @@ def load_library() -> C.CDLL:
         "nbhip_restore": (u, [st, C.c_int, C.c_uint, C.c_double, C.c_uint]),
         "nbhip_histogram_dispersion": (u, [st, C.c_int, C.POINTER(_HistParams),
                                            C.POINTER(_DispParams), _dp]),
+        "nbhip_checkpoint_pack": (u, [C.c_char_p, C.POINTER(_CkptHeader), C.POINTER(_CkptIO),
+                                      _dp, _dp, _dp]),
+        "nbhip_checkpoint_unpack": (u, [C.c_char_p, C.POINTER(_CkptHeader), C.POINTER(_CkptIO),
+                                        _dp, _dp, _dp]),
+        "nbhip_checkpoint_write": (u, [st, C.c_char_p, C.POINTER(_CkptIO)]),
+        "nbhip_checkpoint_read": (u, [st, C.c_char_p, C.POINTER(_CkptIO)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -357,6 +434,31 @@ class NBodyCtx:
                         treeRSize=treeRSize, criterion=crit, useQuad=useQuad,
                         allowIncest=allowIncest, nStep=int(n_step))
 
+    def reference_image(self, potential: Optional[Potential] = None) -> bytes:
+        """This context as the reference's NBodyCtx bytes (x86-64 layout, 592
+        bytes), as a checkpoint header carries it: the fields this mirror
+        holds and the potential; the rest (output switches, likelihood
+        settings, LMC) are zero."""
+        r = _RefNBodyCtx()
+        r.eps2, r.theta, r.timestep = self.eps2, self.theta, self.timestep
+        r.timeEvolve, r.treeRSize = self.timeEvolve, self.treeRSize
+        r.criterion = CRITERIA[self.criterion]
+        r.potentialType = 1 if potential is None else 0     # EXTERNAL_POTENTIAL_NONE / _DEFAULT
+        r.useQuad = bytes([int(bool(self.useQuad))])
+        r.allowIncest = bytes([int(bool(self.allowIncest))])
+        r.nStep = self.nStep
+        r.checkpointT = -1
+        if potential is not None:
+            sp, dk, d2, ha = potential.spherical, potential.disk, potential.disk2, potential.halo
+            r.pot.sphere.type, r.pot.sphere.mass, r.pot.sphere.scale = sp.type, sp.mass, sp.scale
+            r.pot.disk.type, r.pot.disk.mass = dk.type, dk.mass
+            r.pot.disk.scaleLength, r.pot.disk.scaleHeight = dk.scaleLength, dk.scaleHeight
+            r.pot.disk2.type, r.pot.disk2.mass = d2.type, d2.mass
+            r.pot.disk2.scaleLength, r.pot.disk2.scaleHeight = d2.scaleLength, d2.scaleHeight
+            r.pot.halo.type, r.pot.halo.vhalo = ha.type, ha.vhalo
+            r.pot.halo.scaleLength, r.pot.halo.flattenZ = ha.scaleLength, ha.flattenZ
+        return bytes(r)
+
     def to_c(self, potential: Optional[Potential], precision: str = "fp64") -> _Ctx:
         c = _Ctx()
         c.eps2, c.theta, c.timestep, c.tree_rsize = (self.eps2, self.theta, self.timestep,
@@ -369,6 +471,98 @@ class NBodyCtx:
         c.lmc = 0
         c.pot = (potential or Potential()).to_c()
         return c
+
+
+class _CkptBuffers:
+    """Keeps the arrays an nbhip_checkpoint_io points to alive."""
+
+    def __init__(self, ctx_image, state_image, version, type_=None, id_=None, best=None,
+                 orbit_trace=None, shift_lmc=None, lmc_posvel=None):
+        self.ctx = C.create_string_buffer(bytes(ctx_image), len(ctx_image))
+        self.state = C.create_string_buffer(bytes(state_image), max(len(state_image), 1))
+        io = _CkptIO()
+        io.major_version, io.minor_version = version
+        io.ctx, io.ctx_size = C.cast(self.ctx, C.c_void_p), len(ctx_image)
+        io.state, io.state_size = C.cast(self.state, C.c_void_p), len(state_image)
+        self.keep = []
+
+        def arr(a, dt, shape=None):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dtype=dt)
+            if shape is not None:
+                a = a.reshape(shape)
+            self.keep.append(a)
+            return a
+
+        t = arr(type_, np.int16)
+        i = arr(id_, np.uint32)
+        io.type = _ptr(t, C.c_short)
+        io.id = _ptr(i, C.c_uint)
+        if best is not None:
+            io.best_pos = _ptr(arr(best[0], np.float64))
+            io.best_vel = _ptr(arr(best[1], np.float64))
+        tr = arr(orbit_trace, np.float64, (-1, 4))
+        if tr is not None:
+            io.orbit_trace, io.n_orbit_trace = _ptr(tr), tr.shape[0]
+        sh = arr(shift_lmc, np.float64, (-1, 4))
+        if sh is not None:
+            io.shift_lmc, io.n_shift_lmc = _ptr(sh), sh.shape[0]
+            lp, lv = lmc_posvel
+            io.lmc_pos[:] = [float(x) for x in lp]
+            io.lmc_vel[:] = [float(x) for x in lv]
+        self.io = io
+
+
+def checkpoint_pack(path: str, nbody: int, step: int, rsize: float, tree_incest: int,
+                    pos, vel, mass, ctx_image: bytes, state_image: bytes = b"",
+                    version=NBODY_VERSION, **kw) -> None:
+    """Writes a checkpoint file from host arrays (nbhip_checkpoint_pack)."""
+    lib = load_library()
+    h = _CkptHeader()
+    h.nbody, h.step, h.rsize, h.tree_incest = nbody, step, rsize, tree_incest
+    b = _CkptBuffers(ctx_image, state_image, version, **kw)
+    x = np.ascontiguousarray(pos, dtype=np.float64)
+    v = np.ascontiguousarray(vel, dtype=np.float64)
+    m = np.ascontiguousarray(mass, dtype=np.float64)
+    rc = lib.nbhip_checkpoint_pack(os.fsencode(path), C.byref(h), C.byref(b.io), _ptr(x),
+                                   _ptr(v), _ptr(m))
+    if rc:
+        raise NBodyError(rc, lib.nbhip_last_error().decode())
+
+
+def checkpoint_unpack(path: str, ctx_size: int = 592, state_size: int = 0,
+                      version=NBODY_VERSION, max_vectors: int = 1 << 20) -> dict:
+    """Reads a checkpoint file on the host (nbhip_checkpoint_unpack): header
+    fields, the NBodyCtx / NBodyState images, bodies and vectors."""
+    lib = load_library()
+    h = _CkptHeader()
+    b = _CkptBuffers(bytes(ctx_size), bytes(state_size), version)
+    rc = lib.nbhip_checkpoint_unpack(os.fsencode(path), C.byref(h), C.byref(b.io), None, None,
+                                     None)
+    if rc:
+        raise NBodyError(rc, lib.nbhip_last_error().decode())
+    n = h.nbody
+    out = {k: getattr(h, k) for k, _ in _CkptHeader._fields_}
+    pos, vel, mass = np.empty((n, 3)), np.empty((n, 3)), np.empty(n)
+    bpos, bvel = np.empty((n, 3)), np.empty((n, 3))
+    typ, ids = np.empty(n, np.int16), np.empty(n, np.uint32)
+    tr = np.empty((max(h.n_orbit_trace, 1), 4))
+    sh = np.empty((max(h.n_shift_lmc, 1), 4))
+    b = _CkptBuffers(bytes(ctx_size), bytes(state_size), version)
+    b.io.type, b.io.id = _ptr(typ, C.c_short), _ptr(ids, C.c_uint)
+    b.io.best_pos, b.io.best_vel = _ptr(bpos), _ptr(bvel)
+    b.io.orbit_trace, b.io.n_orbit_trace = _ptr(tr), h.n_orbit_trace
+    b.io.shift_lmc, b.io.n_shift_lmc = _ptr(sh), h.n_shift_lmc
+    rc = lib.nbhip_checkpoint_unpack(os.fsencode(path), C.byref(h), C.byref(b.io), _ptr(pos),
+                                     _ptr(vel), _ptr(mass))
+    if rc:
+        raise NBodyError(rc, lib.nbhip_last_error().decode())
+    out.update(pos=pos, vel=vel, mass=mass, best_pos=bpos, best_vel=bvel, type=typ, id=ids,
+               ctx=b.ctx.raw[:ctx_size], state=b.state.raw[:b.io.state_size],
+               orbit_trace=tr[:h.n_orbit_trace], shift_lmc=sh[:h.n_shift_lmc],
+               lmc_pos=np.array(b.io.lmc_pos[:]), lmc_vel=np.array(b.io.lmc_vel[:]))
+    return out
 
 
 @dataclass
@@ -417,6 +611,33 @@ class NBodyState:
         if rc and (status_is_fatal(rc) or not allow_warning):
             raise NBodyError(rc, self.lib.nbhip_last_error().decode())
         return rc
+
+    def write_checkpoint(self, path: str, state_image: bytes = bytes(NBODY_STATE_SIZE),
+                         version=NBODY_VERSION, **kw) -> None:
+        """nbWriteCheckpoint of this GPU state (nbhip_checkpoint_write): the
+        context's NBodyCtx image, `state_image` as the raw NBodyState."""
+        b = _CkptBuffers(self.ctx.reference_image(self.potential), state_image, version, **kw)
+        self._check(self.lib.nbhip_checkpoint_write(self._h, os.fsencode(path), C.byref(b.io)),
+                    allow_warning=False)
+
+    @staticmethod
+    def from_checkpoint(path: str, ctx: NBodyCtx, potential: Optional[Potential],
+                        device: int = 0, precision: str = "fp64", version=NBODY_VERSION,
+                        state_size: int = NBODY_STATE_SIZE) -> "NBodyState":
+        """nbReadCheckpoint into a new GPU state (nbhip_checkpoint_read); no
+        force evaluation: the next run() starts with one, as on resume.  The
+        file's NBodyCtx and NBodyState images are kept as .checkpoint_ctx and
+        .checkpoint_state."""
+        head = checkpoint_unpack(path, state_size=state_size, version=version)
+        n = int(head["nbody"])
+        st = NBodyState(ctx, potential, np.zeros((n, 3)), np.zeros((n, 3)), np.zeros(n),
+                        device=device, precision=precision, prestep=False)
+        b = _CkptBuffers(bytes(592), bytes(state_size), version)
+        st._check(st.lib.nbhip_checkpoint_read(st._h, os.fsencode(path), C.byref(b.io)),
+                  allow_warning=False)
+        st.checkpoint_ctx = b.ctx.raw[:592]
+        st.checkpoint_state = b.state.raw[:state_size]
+        return st
 
     def gravity(self) -> int:
         self.last_status = self._check(self.lib.nbhip_gravity(self._h))

@@ -341,3 +341,81 @@ def frozen_vjp(X, m, t, children, lists, eps2, G, quad=True):
     gx[ref[leaves]] += gcm[leaves]
     gm[ref[leaves]] += gM[leaves]
     return gx, gm
+
+
+# ---------------------------------------------------------------------------
+# KDK over frozen trees: the map the tree adjoint differentiates
+# ---------------------------------------------------------------------------
+# Each state n of a tree run has its own tree and interaction lists (built
+# at x_n by the run itself).  Holding them fixed while the inputs move
+# gives a smooth map of (x0, v0, m, dt); its derivative is what
+# nbhip_adjoint_backward computes for the tree criteria.  frozen[n] =
+# (tree, children, lists) of state n, from the oracle's run.
+
+def frozen_trees(oracle_state, nsteps):
+    """Trees and lists of states 0..nsteps of an oracle run (force already
+    evaluated at state 0); the oracle is left at state nsteps."""
+    out = []
+    for k in range(nsteps + 1):
+        t = oracle_state.tree()
+        out.append((t, tree_children(t), walk_lists(t)))
+        if k < nsteps:
+            st = oracle_state.step()
+            if st != 0:
+                raise RuntimeError(f"oracle step {k} returned status {st}")
+    return out
+
+
+def kdk_forward_frozen(x0, v0, m, eps2, dt, frozen, pot=None, quad=True):
+    """States [(x, v, a)] of the KDK leapfrog whose force at state n is the
+    frozen-list force of frozen[n] plus the external acceleration."""
+    p = pot_params(pot)
+
+    def force(x, n):
+        t, ch, lists = frozen[n]
+        return frozen_accel(x, m, t, ch, lists, eps2, quad) + ext_accel(p, x)
+
+    x, v = np.array(x0, dtype=np.float64), np.array(v0, dtype=np.float64)
+    a = force(x, 0)
+    out = [(x.copy(), v.copy(), a.copy())]
+    c = 0.5 * dt
+    for n in range(1, len(frozen)):
+        vh = v + c * a
+        x = x + dt * vh
+        a = force(x, n)
+        v = vh + c * a
+        out.append((x.copy(), v.copy(), a.copy()))
+    return out
+
+
+def kdk_adjoint_frozen(states, m, eps2, dt, frozen, gxN, gvN, pot=None, quad=True):
+    """Reverse sweep of kdk_forward_frozen: dL/d(x0, v0, m, dt) from
+    dL/d(xN, vN); the same recurrences as kdk_adjoint with frozen_vjp in
+    place of the direct-sum Jacobian."""
+    p = pot_params(pot)
+
+    def vjp(x, n, g):
+        t, ch, lists = frozen[n]
+        jx, jm = frozen_vjp(x, m, t, ch, lists, eps2, g, quad)
+        return jx + ext_jac(p, x, g), jm
+
+    c = 0.5 * dt
+    gx, gv = np.array(gxN, dtype=np.float64), np.array(gvN, dtype=np.float64)
+    ga = np.zeros_like(gx)
+    gm = np.zeros(m.shape[0])
+    gdt = 0.0
+    for n in range(len(states) - 2, -1, -1):
+        x1, _, a1 = states[n + 1]
+        _, v0, a0 = states[n]
+        ga = ga + c * gv
+        gdt += 0.5 * (a1 * gv).sum()
+        jx, jm = vjp(x1, n + 1, ga)
+        gx = gx + jx
+        gm = gm + jm
+        vh = v0 + c * a0
+        gvh = gv + dt * gx
+        gdt += (vh * gx).sum() + 0.5 * (a0 * gvh).sum()
+        gv = gvh
+        ga = c * gvh
+    jx, jm = vjp(states[0][0], 0, ga)
+    return gx + jx, gv, gm + jm, gdt

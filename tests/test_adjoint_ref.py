@@ -134,3 +134,49 @@ def test_frozen_tree_vjp_equals_differences(criterion, quad):
     fd = ((ar.frozen_accel(pos, m + h * Dm, t, ch, lists, eps2, quad)
            - ar.frozen_accel(pos, m - h * Dm, t, ch, lists, eps2, quad)) * G).sum() / (2 * h)
     assert abs((gm * Dm).sum() - fd) <= 1e-7 * abs(fd)
+
+
+@pytest.mark.parametrize("use_pot", [False, True])
+def test_frozen_tree_reverse_sweep_equals_differences(use_pot):
+    """Several KDK steps of a theta = 0.6 tree run with each state's tree and
+    lists held fixed: the frozen forward map reproduces the oracle's run, and
+    its reverse sweep (frozen_vjp in each step) equals central differences of
+    that map along random directions in x0, v0, m and for dt.  The map is
+    smooth, so the differences converge (the gap falls ~100x per 10x step)."""
+    n, steps, eps2, dt = 200, 5, 1e-4, 2e-4
+    pos, vel, m = plummer_bodies(n, seed=31)
+    m = m * np.random.default_rng(4).uniform(0.5, 1.5, n)
+    pos = pos + ((8.0, 3.0, -2.0) if use_pot else 0.0)
+    pot = orc.milky_way() if use_pot else None
+    o = orc.Oracle(orc.make_ctx(eps2=eps2, theta=0.6, timestep=dt, potential=pot), pos, vel, m)
+    assert o.gravity() == 0
+    frozen = ar.frozen_trees(o, steps)
+    states = ar.kdk_forward_frozen(pos, vel, m, eps2, dt, frozen, pot)
+    x, v, a = states[-1]
+    np.testing.assert_allclose(x, o.pos, rtol=0, atol=1e-12 * np.abs(x).max())
+    np.testing.assert_allclose(v, o.vel, rtol=0, atol=1e-11 * np.abs(v).max())
+    np.testing.assert_allclose(a, o.acc, rtol=0, atol=1e-10 * np.abs(a).max())
+
+    rng = np.random.default_rng(5)
+    c0 = pos.mean(0)
+    wx, wv = rng.normal(size=(n, 3)), rng.normal(size=(n, 3))
+
+    def loss(p, vv, mm, t):
+        xN, vN, _ = ar.kdk_forward_frozen(p, vv, mm, eps2, t, frozen, pot)[-1]
+        return (wx * (xN - c0)).sum() + (wv * vN).sum() + 0.5 * ((xN - c0) ** 2).sum()
+
+    gx, gv, gm, gdt = ar.kdk_adjoint_frozen(states, m, eps2, dt, frozen, wx + (x - c0), wv, pot)
+    D = rng.normal(size=(n, 3))
+    h = 1e-7
+    fd = (loss(pos + h * D, vel, m, dt) - loss(pos - h * D, vel, m, dt)) / (2 * h)
+    assert abs((gx * D).sum() - fd) <= 1e-7 * abs(fd), ("x", (gx * D).sum(), fd)
+    fd = (loss(pos, vel + h * D, m, dt) - loss(pos, vel - h * D, m, dt)) / (2 * h)
+    assert abs((gv * D).sum() - fd) <= 1e-7 * abs(fd), ("v", (gv * D).sum(), fd)
+    # masses: below h ~ 1e-6 the differences are round-off-limited here
+    Dm = rng.normal(size=n) * m
+    hm = 1e-5
+    fd = (loss(pos, vel, m + hm * Dm, dt) - loss(pos, vel, m - hm * Dm, dt)) / (2 * hm)
+    assert abs((gm * Dm).sum() - fd) <= 1e-7 * abs(fd), ("m", (gm * Dm).sum(), fd)
+    hd = 1e-5 * dt
+    fd = (loss(pos, vel, m, dt + hd) - loss(pos, vel, m, dt - hd)) / (2 * hd)
+    assert abs(gdt - fd) <= 1e-7 * abs(fd), ("dt", gdt, fd)

@@ -171,3 +171,41 @@ def test_batched_adjoint_equals_individual_runs():
         assert abs(bdt[i] - gdt[0]) <= 1e-12 * abs(gdt[0]), i
         off += k
         st.close()
+
+
+@pytest.mark.parametrize("use_pot", [False, True])
+def test_tree_adjoint_over_steps_matches_frozen_reverse_sweep(use_pot):
+    """The tree adjoint over several steps (theta = 0.6, quadrupoles) against
+    the numpy reverse sweep over the oracle's trees of the same run
+    (oracle/adjoint_ref.kdk_adjoint_frozen, pinned by central differences of
+    the frozen-list forward map in tests/test_adjoint_ref.py): the GPU's
+    trees are the oracle's node for node, so the two differentiate the same
+    map and agree to round-off."""
+    n, steps, eps2, dt = 1200, 8, 1e-4, 2e-4
+    pos, vel, m = plummer_bodies(n, seed=33)
+    m = m * np.random.default_rng(4).uniform(0.5, 1.5, n)
+    if use_pot:
+        pos = pos + np.array([8.0, 3.0, -2.0])
+    pot = nb.milky_way_potential() if use_pot else None
+    opot = orc.milky_way() if use_pot else None
+    ctx = nb.NBodyCtx.create(timestep=dt, timeEvolve=dt * steps, eps2=eps2, theta=0.6)
+    st = nb.NBodyState(ctx, pot, pos, vel, m, prestep=False)
+    st.adjoint_forward(steps, 3)
+    x, v, _ = st.bodies()
+    rng = np.random.default_rng(9)
+    c0 = pos.mean(0)
+    wx, wv = rng.normal(size=(n, 3)), rng.normal(size=(n, 3))
+    got = st.adjoint_backward(wx + (x - c0), wv)
+    st.close()
+
+    o = orc.Oracle(orc.make_ctx(eps2=eps2, theta=0.6, timestep=dt, potential=opot), pos, vel, m)
+    assert o.gravity() == 0
+    frozen = ar.frozen_trees(o, steps)
+    assert np.array_equal(x, o.pos)
+    states = ar.kdk_forward_frozen(pos, vel, m, eps2, dt, frozen, opot)
+    want = ar.kdk_adjoint_frozen(states, m, eps2, dt, frozen, wx + (x - c0), wv, opot)
+    for name, a, b in zip(("x", "v", "m", "dt"), got, want):
+        a, b = np.asarray(a, dtype=np.float64).ravel(), np.asarray(b, dtype=np.float64).ravel()
+        err = np.abs(a - b).max() / np.abs(b).max()
+        print(name, "GPU tree adjoint vs frozen reverse sweep, max rel", err, flush=True)
+        assert err <= 1e-10, name

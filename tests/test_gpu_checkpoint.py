@@ -93,3 +93,74 @@ def test_batch_resume_per_simulation():
         np.testing.assert_array_equal(vc, va)
     for s in (a, b, c):
         s.close()
+
+
+def test_checkpoint_file_resume_matches_oracle(tmp_path):
+    """nbhip_checkpoint_write after k steps, a fresh state from the file
+    (nbhip_checkpoint_read), k more steps: the file holds the oracle's state
+    at step k (bodies, step, root size) and the resumed run ends bit for bit
+    where the oracle's uninterrupted run ends (FP64, Milky Way potential)."""
+    from oracle import oracle as orc
+    pos, vel, m = plummer_bodies(3000, seed=14, mass=10.0, radius=0.25,
+                                 position=(6.0, -2.0, 3.0), velocity=(-30.0, 80.0, 20.0))
+    ctx = _ctx()
+    pot = nb.milky_way_potential()
+    k = 12
+    o = orc.Oracle(orc.make_ctx(eps2=1e-4, theta=0.5, timestep=ctx.timestep, tree_rsize=0.125,
+                                potential=orc.milky_way()), pos, vel, m)
+    assert o.gravity() == 0
+    a = nb.NBodyState(ctx, pot, pos, vel, m, prestep=False)
+    a.run(k)
+    path = str(tmp_path / "nbody_checkpoint")
+    ignore = np.zeros(m.size, np.uint8)
+    ignore[::3] = 1
+    a.set_ignore(ignore)
+    a.write_checkpoint(path)
+    a.close()
+    for _ in range(k):
+        assert o.step() == 0
+    f = nb.checkpoint_unpack(path, state_size=nb.NBODY_STATE_SIZE)
+    assert (f["nbody"], f["step"], f["rsize"]) == (m.size, k, o.rsize)
+    np.testing.assert_array_equal(f["pos"], o.pos)
+    np.testing.assert_array_equal(f["vel"], o.vel)
+    np.testing.assert_array_equal(f["mass"], m)
+    assert np.array_equal(f["type"], np.where(ignore == 1, -1, 1))
+    assert np.array_equal(f["id"], np.arange(m.size))
+    b = nb.NBodyState.from_checkpoint(path, ctx, pot)
+    assert b.tree_info().step == k
+    b.run(k)
+    for _ in range(k):
+        assert o.step() == 0
+    pb, vb, ab = b.bodies()
+    assert b.tree_info().step == 2 * k and b.tree_info().rsize == o.rsize
+    np.testing.assert_array_equal(pb, o.pos)
+    np.testing.assert_array_equal(vb, o.vel)
+    np.testing.assert_array_equal(ab, o.acc)
+    b.close()
+
+
+def test_checkpoint_file_keeps_the_best_likelihood_bodies(tmp_path):
+    """bestLikelihoodBodyTab: the bodies kept by nbhip_run_system's best step
+    go into the file and come back through nbhip_best_bodies."""
+    pos, vel, m = plummer_bodies(2000, seed=15)
+    ctx = nb.NBodyCtx.create(timestep=1e-3, timeEvolve=0.02, eps2=1e-4, theta=0.5)
+    a = nb.NBodyState(ctx, None, pos, vel, m, prestep=False)
+    target = 14
+
+    def like(step):
+        return [abs(step - target) + 0.5] + [0.0] * 9
+
+    status, best = a.run_system(likelihood=like, best_like_start=0.5, keep_best_bodies=True)
+    assert status == 0 and best["step"] == target
+    kept = a.best_bodies()
+    path = str(tmp_path / "cp")
+    a.write_checkpoint(path)
+    f = nb.checkpoint_unpack(path, state_size=nb.NBODY_STATE_SIZE)
+    np.testing.assert_array_equal(f["best_pos"], kept[0])
+    np.testing.assert_array_equal(f["best_vel"], kept[1])
+    b = nb.NBodyState.from_checkpoint(path, ctx, None)
+    back = b.best_bodies()
+    np.testing.assert_array_equal(back[0], kept[0])
+    np.testing.assert_array_equal(back[1], kept[1])
+    a.close()
+    b.close()
