@@ -485,6 +485,52 @@ unsigned nbhip_checkpoint_write(nbhip_state* st, const char* path, const nbhip_c
  * nbRunSystemPlain does on resume (nbody/src/nbody_plain.c:388). */
 unsigned nbhip_checkpoint_read(nbhip_state* st, const char* path, nbhip_checkpoint_io* io);
 
+/* ---- two-component dwarf initial conditions ---------------------------------
+ * predefinedModels.mixeddwarf (nbGenerateMixedDwarfCore,
+ * nbody/src/nbody_mixeddwarf.c:765-1026) on the GPU: the reference's
+ * component densities and potentials (nbody/src/nbody_dwarf_potential.c),
+ * set_vars / get_extra_nfw_mass (:696-762: an NFW mass is M200, reset to the
+ * mass inside 5 r200), radii by r_mag's rejection, speeds by rejection
+ * against each component's Eddington distribution function in the total
+ * potential below 0.99 v_esc, isotropic directions (velocity first), the
+ * per-component centre-of-mass and momentum correction to pos / vel
+ * (cm_correction_by_comp), baryons first.  The distribution functions are
+ * tabulated once (Eddington's integral with analytic derivatives) and every
+ * body draws from its own Philox4x32-10 stream (key = seed, counter = body
+ * index and draw number): results do not depend on the launch shape and
+ * repeat bit for bit, but they are not the reference's dSFMT stream. */
+#define NBHIP_DWARF_PLUMMER       0   /* dwarf_t Plummer */
+#define NBHIP_DWARF_NFW           1   /* dwarf_t NFW */
+#define NBHIP_DWARF_GEN_HERNQUIST 2   /* dwarf_t General_Hernquist */
+
+typedef struct nbhip_dwarf_params {
+    int light_type, dark_type;            /* NBHIP_DWARF_* (comp1, comp2) */
+    double light_mass, light_scale;       /* mass (an NFW mass is M200), scale radius */
+    double dark_mass, dark_scale;
+    unsigned nbody, nbody_baryon;         /* bodies; the first nbody_baryon are light */
+    unsigned long long seed;              /* stream key */
+    double pos[3], vel[3];                /* rShift, vShift: the orbit point */
+} nbhip_dwarf_params;
+
+/* Bodies into host arrays (pos, vel n x 3, mass n); comp_mass (2, may be
+ * NULL) receives the components' masses after the NFW reset.  Light bodies
+ * have type 1 and dark bodies -1 (BODY(isdark): left out of histograms).
+ * NBHIP_ERROR if a body's rejection loop did not finish (4M trials) or a
+ * speed exceeded its rejection envelope. */
+unsigned nbhip_mixed_dwarf(const nbhip_dwarf_params* p, int device, double* pos, double* vel,
+                           double* mass, double* comp_mass);
+/* The same bodies written straight into simulation `sim` of a state (its
+ * body count must be p->nbody), with the dark bodies flagged as ignored
+ * (nbhip_set_ignore); nothing crosses PCIe.  Call nbhip_gravity next. */
+unsigned nbhip_mixed_dwarf_state(nbhip_state* st, int sim, const nbhip_dwarf_params* p,
+                                 double* comp_mass);
+/* The tabulated distribution function of component comp (0 light, 1 dark)
+ * at `count` energies (psi - v^2/2); NaN outside the table. */
+unsigned nbhip_dwarf_df(const nbhip_dwarf_params* p, int device, int comp, int count,
+                        const double* energy, double* f);
+/* One Philox4x32-10 block (host code): out = philox(ctr[4], key[2]). */
+unsigned nbhip_philox4x32_10(const unsigned* ctr, const unsigned* key, unsigned* out);
+
 /* Waits for all work queued by the library. */
 unsigned nbhip_sync(nbhip_state* st);
 

@@ -24,7 +24,7 @@ MODELS_LIB_PATH = os.path.join(PKG_DIR, "libnbmodels.so")
 STAMP_PATH = os.path.join(PKG_DIR, "libnbhip.srchash")
 
 SOURCES = ["api.hip", "tree.hip", "walk.hip", "walk_group.hip", "adjoint.hip", "histogram.hip",
-           "exact.hip", "run.hip", "checkpoint.hip"]
+           "exact.hip", "run.hip", "checkpoint.hip", "dwarf.hip"]
 
 # -ffp-contract=off: the plain FP64 form keeps the reference's two-rounding
 # a*b+c so that it reproduces the CPU path bit for bit.

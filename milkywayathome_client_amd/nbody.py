@@ -181,6 +181,18 @@ class _RefNBodyCtx(C.Structure):
 assert C.sizeof(_RefSpherical) == 32 and C.sizeof(_RefPotential) == 256
 assert _RefNBodyCtx.pot.offset == 336 and C.sizeof(_RefNBodyCtx) == 592
 
+class _DwarfParams(C.Structure):
+    _fields_ = [("light_type", C.c_int), ("dark_type", C.c_int),
+                ("light_mass", C.c_double), ("light_scale", C.c_double),
+                ("dark_mass", C.c_double), ("dark_scale", C.c_double),
+                ("nbody", C.c_uint), ("nbody_baryon", C.c_uint), ("seed", C.c_ulonglong),
+                ("pos", C.c_double * 3), ("vel", C.c_double * 3)]
+
+
+# dwarf_t (nbody/include/nbody_potential_types.h:139-152) of the supported components
+DWARF_TYPES = {"plummer": 0, "nfw": 1, "general_hernquist": 2}
+
+
 # NBODY_VERSION_MAJOR / _MINOR of the reference build (nbody/CMakeLists.txt:24-27)
 NBODY_VERSION = (1, 93)
 NBODY_CHECKPOINT_ERROR = 1 << 6
@@ -198,7 +210,8 @@ EXPORTED = [
     "nbhip_histogram", "nbhip_orbit_tangent", "nbhip_restore", "nbhip_histogram_dispersion",
     "nbhip_set_open_scale", "nbhip_batch_tree_download", "nbhip_histogram_noalloc",
     "nbhip_run_system", "nbhip_best_bodies", "nbhip_checkpoint_pack", "nbhip_checkpoint_unpack",
-    "nbhip_checkpoint_write", "nbhip_checkpoint_read",
+    "nbhip_checkpoint_write", "nbhip_checkpoint_read", "nbhip_mixed_dwarf",
+    "nbhip_mixed_dwarf_state", "nbhip_dwarf_df", "nbhip_philox4x32_10",
 ]
 
 _LIB: Optional[C.CDLL] = None
@@ -275,6 +288,11 @@ def load_library() -> C.CDLL:
                                         _dp, _dp, _dp]),
         "nbhip_checkpoint_write": (u, [st, C.c_char_p, C.POINTER(_CkptIO)]),
         "nbhip_checkpoint_read": (u, [st, C.c_char_p, C.POINTER(_CkptIO)]),
+        "nbhip_mixed_dwarf": (u, [C.POINTER(_DwarfParams), C.c_int, _dp, _dp, _dp, _dp]),
+        "nbhip_mixed_dwarf_state": (u, [st, C.c_int, C.POINTER(_DwarfParams), _dp]),
+        "nbhip_dwarf_df": (u, [C.POINTER(_DwarfParams), C.c_int, C.c_int, C.c_int, _dp, _dp]),
+        "nbhip_philox4x32_10": (u, [C.POINTER(C.c_uint), C.POINTER(C.c_uint),
+                                    C.POINTER(C.c_uint)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -565,6 +583,57 @@ def checkpoint_unpack(path: str, ctx_size: int = 592, state_size: int = 0,
     return out
 
 
+def dwarf_params(nbody: int, nbody_baryon: int, light=("plummer", 12.0, 0.2),
+                 dark=("nfw", 60.0, 0.8), seed: int = 34086709, position=(0.0, 0.0, 0.0),
+                 velocity=(0.0, 0.0, 0.0)) -> _DwarfParams:
+    """predefinedModels.mixeddwarf's arguments: each component as (type,
+    mass, scale radius), an NFW mass being M200."""
+    p = _DwarfParams()
+    p.light_type, p.light_mass, p.light_scale = DWARF_TYPES[light[0]], light[1], light[2]
+    p.dark_type, p.dark_mass, p.dark_scale = DWARF_TYPES[dark[0]], dark[1], dark[2]
+    p.nbody, p.nbody_baryon, p.seed = int(nbody), int(nbody_baryon), int(seed)
+    p.pos[:] = [float(x) for x in position]
+    p.vel[:] = [float(x) for x in velocity]
+    return p
+
+
+def mixed_dwarf(nbody: int, nbody_baryon: int, device: int = 0, **kw):
+    """Two-component dwarf sampled on the GPU (nbhip_mixed_dwarf): returns
+    (pos (n,3), vel (n,3), mass (n,), type (n,), component masses (2,));
+    baryons first (type 1), dark matter after (type -1)."""
+    lib = load_library()
+    p = dwarf_params(nbody, nbody_baryon, **kw)
+    pos, vel, m, cm = np.empty((nbody, 3)), np.empty((nbody, 3)), np.empty(nbody), np.empty(2)
+    rc = lib.nbhip_mixed_dwarf(C.byref(p), device, _ptr(pos), _ptr(vel), _ptr(m), _ptr(cm))
+    if rc:
+        raise NBodyError(rc, lib.nbhip_last_error().decode())
+    typ = np.where(np.arange(nbody) < nbody_baryon, 1, -1).astype(np.int32)
+    return pos, vel, m, typ, cm
+
+
+def dwarf_df(energy, comp: int, device: int = 0, **kw) -> np.ndarray:
+    """The sampler's tabulated distribution function of component comp
+    (0 light, 1 dark) at the given energies (nbhip_dwarf_df)."""
+    lib = load_library()
+    p = dwarf_params(kw.pop("nbody", 0), kw.pop("nbody_baryon", 0), **kw)
+    e = np.ascontiguousarray(energy, dtype=np.float64).ravel()
+    f = np.empty_like(e)
+    rc = lib.nbhip_dwarf_df(C.byref(p), device, comp, e.size, _ptr(e), _ptr(f))
+    if rc:
+        raise NBodyError(rc, lib.nbhip_last_error().decode())
+    return f
+
+
+def philox4x32_10(ctr, key) -> tuple:
+    """One Philox4x32-10 block from the library (host code)."""
+    lib = load_library()
+    c = (C.c_uint * 4)(*[int(x) & 0xFFFFFFFF for x in ctr])
+    k = (C.c_uint * 2)(*[int(x) & 0xFFFFFFFF for x in key])
+    o = (C.c_uint * 4)()
+    lib.nbhip_philox4x32_10(c, k, o)
+    return tuple(o)
+
+
 @dataclass
 class TreeInfo:
     rsize: float
@@ -611,6 +680,18 @@ class NBodyState:
         if rc and (status_is_fatal(rc) or not allow_warning):
             raise NBodyError(rc, self.lib.nbhip_last_error().decode())
         return rc
+
+    def sample_mixed_dwarf(self, sim: int = 0, **kw):
+        """Samples a two-component dwarf straight into simulation `sim`
+        (nbhip_mixed_dwarf_state; dark bodies flagged as ignored).  Takes
+        dwarf_params' keywords; returns the component masses."""
+        nsim_n = self.n if getattr(self, "nsim", 1) == 1 else self.counts[sim]
+        nb = kw.pop("nbody_baryon", nsim_n // 2)
+        p = dwarf_params(nsim_n, nb, **kw)
+        cm = np.empty(2)
+        self._check(self.lib.nbhip_mixed_dwarf_state(self._h, int(sim), C.byref(p), _ptr(cm)),
+                    allow_warning=False)
+        return cm
 
     def write_checkpoint(self, path: str, state_image: bytes = bytes(NBODY_STATE_SIZE),
                          version=NBODY_VERSION, **kw) -> None:
