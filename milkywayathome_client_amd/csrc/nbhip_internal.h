@@ -87,7 +87,6 @@ struct DevScalars {
     unsigned long long counters[4];/* body-body, body-cell, opened, wave node steps */
     int halt;                      /* a step ended with a fatal status: later steps do nothing */
     unsigned steps_done;           /* KDK steps completed on the device */
-    int span_count;                /* cells summarised by k_sum_span */
     int level_count[64];           /* cells per level */
     int level_fill[64];
     int level_off[65];             /* exclusive scan of level_count */
@@ -161,7 +160,7 @@ struct nbhip_state {
     int *nrank;                   /* a node's rank among its parent's children (arrival order) */
     int *kid;                     /* child lists: kid[kbase[p] .. + nkid[p]) */
     int *blist;                   /* summaries: inside cells of each body block, by level */
-    int *span, *span_sorted;      /* summaries: cells spanning several body blocks */
+    int *span, *span_sorted;      /* summaries: cells spanning body blocks / super-blocks */
     double *exact_part;           /* Exact criterion: per-segment partial sums */
     int exact_seg;                /* Exact criterion: most segments of any simulation */
 

@@ -149,7 +149,6 @@ __global__ __launch_bounds__(256) void k_rsize(int nsim, SimTab* __restrict__ si
         scal->ncell = 0;
         scal->nnode = 0;
         scal->maxdepth = 0;
-        scal->span_count = 0;
         scal->counters[0] = scal->counters[1] = scal->counters[2] = scal->counters[3] = 0ull;
     }
     if (threadIdx.x < 64) {
@@ -1074,17 +1073,22 @@ __global__ __launch_bounds__(256) void k_level_scatter(const int* __restrict__ n
 /* hackCofM + findRCrit + nbCheckTreeStructure (nbody/src/nbody_tree.c:
  * 345-424) and hackQuad (:70-121), children in octant (pre-)order.
  *
- * Two launches, no host synchronisation and no cross-workgroup waits:
+ * Three launches, no host synchronisation and no cross-workgroup waits:
  *  - k_sum_blocks: workgroup b takes the massive bodies [b*NBH_SUM_BODIES,
  *    (b+1)*NBH_SUM_BODIES) and the pre-order range [P0, P1) of their leaves
  *    and of the cells whose first body they are.  A cell of that range whose
  *    subtree ends inside it (Next <= P1) is "inside": all its descendants
  *    are in the range, so the workgroup summarises its inside cells level by
  *    level, deepest first, with a barrier between levels.  Cells whose
- *    subtree runs past P1 ("spanning") are listed for the second launch.
- *  - k_sum_span: one workgroup summarises the spanning cells level by level,
- *    deepest first (their children are inside cells, bodies or deeper
- *    spanning cells).
+ *    subtree runs past P1 ("spanning") all contain the block's last body, so
+ *    there is at most one per level: they go to the block's own 64 slots.
+ *  - k_sum_super: workgroup s takes NBH_SUM_GROUP blocks (a super-block) and
+ *    their spanning cells; those whose subtree ends inside the super-block
+ *    are summarised the same way (their children are inside cells, bodies
+ *    or deeper cells of this list), the rest (again at most one per level)
+ *    go to the super-block's 64 slots.
+ *  - k_sum_top: one workgroup summarises the super-blocks' spanning cells
+ *    (a few per level and super-block), deepest first.
  * A cell's children come from its child list (k_fill_kids): the pre-order
  * indices of its children, sorted into octant order and checked against the
  * Next links, then loaded together instead of one after another. */
@@ -1297,6 +1301,8 @@ __device__ __forceinline__ bool nbh_summarize_cell(const SumView& v, int p)
 
 #define NBH_SUM_BODIES 1024    /* massive bodies per k_sum_blocks workgroup */
 #define NBH_SUM_THREADS 256
+#define NBH_SUM_GROUP 64       /* blocks per k_sum_super workgroup */
+#define NBH_SPAN_SLOTS 64      /* spanning cells one block can hand up (one per level) */
 #define NBH_SPAN_THREADS 512
 
 /* Levels holding cells in [lo, hi): counts into cnt[64] (LDS), then an
@@ -1305,11 +1311,13 @@ __global__ __launch_bounds__(NBH_SUM_THREADS) void k_sum_blocks(SumView v,
                                                                 const unsigned* __restrict__ cincl,
                                                                 int* __restrict__ blist,
                                                                 int* __restrict__ span,
+                                                                int* __restrict__ span_n,
                                                                 DevScalars* __restrict__ scal)
 {
     __shared__ int cnt[64];
     __shared__ int off[65];
     __shared__ int fill[64];
+    __shared__ int nspan;
     const int tid = threadIdx.x;
     if (scal->status & (NBHIP_CELL_OVERFLOW_ERROR | NBHIP_ERROR)) return;
     const int M = scal->nmassive;
@@ -1322,6 +1330,7 @@ __global__ __launch_bounds__(NBH_SUM_THREADS) void k_sum_blocks(SumView v,
         cnt[tid] = 0;
         fill[tid] = 0;
     }
+    if (tid == 0) nspan = 0;
     __syncthreads();
     /* inside cells per level; spanning cells to the second launch */
     for (int q = P0 + tid; q < P1; q += NBH_SUM_THREADS) {
@@ -1334,11 +1343,15 @@ __global__ __launch_bounds__(NBH_SUM_THREADS) void k_sum_blocks(SumView v,
         if (v.node[q].next <= P1) {
             atomicAdd(&cnt[l], 1);
         } else {
-            const int k = atomicAdd(&scal->span_count, 1);
-            span[k] = q;
+            const int k = atomicAdd(&nspan, 1);
+            if (k < NBH_SPAN_SLOTS)
+                span[(size_t) blockIdx.x * NBH_SPAN_SLOTS + k] = q;
+            else
+                atomicOr(&scal->status, NBHIP_ERROR);   /* not nested: a corrupt table */
         }
     }
     __syncthreads();
+    if (tid == 0) span_n[blockIdx.x] = min(nspan, NBH_SPAN_SLOTS);
     if (tid == 0) {
         int acc = 0;
         for (int l = 0; l < 64; ++l) {
@@ -1367,25 +1380,37 @@ __global__ __launch_bounds__(NBH_SUM_THREADS) void k_sum_blocks(SumView v,
     if (!ok) atomicOr(&scal->status, NBHIP_ERROR);
 }
 
-/* The spanning cells, one workgroup: grouped by level (counting sort into
- * span_sorted), then summarised deepest level first. */
-__global__ __launch_bounds__(NBH_SPAN_THREADS) void k_sum_span(SumView v,
-                                                               const int* __restrict__ span,
-                                                               int* __restrict__ span_sorted,
-                                                               DevScalars* __restrict__ scal)
+/* Cells handed up by several lower workgroups (cand[0..nc) in LDS or
+ * global memory), grouped by level in LDS and summarised deepest level
+ * first; those whose subtree ends past P1 go to out[0..NBH_SPAN_SLOTS)
+ * instead (out may be null when P1 covers the whole table). */
+template <int NT>
+__device__ void nbh_sum_handed(const SumView& v, const int* cand, int nc, int P1, int* out,
+                               int* out_n, int* list, int* cnt, int* off, int* fill, int* nspan,
+                               DevScalars* scal)
 {
-    __shared__ int cnt[64];
-    __shared__ int off[65];
-    __shared__ int fill[64];
     const int tid = threadIdx.x;
-    if (scal->status & (NBHIP_CELL_OVERFLOW_ERROR | NBHIP_ERROR)) return;
-    const int S = min(scal->span_count, v.nn);
     if (tid < 64) {
         cnt[tid] = 0;
         fill[tid] = 0;
     }
+    if (tid == 0) *nspan = 0;
     __syncthreads();
-    for (int i = tid; i < S; i += NBH_SPAN_THREADS) atomicAdd(&cnt[v.nlevel[span[i]]], 1);
+    for (int i = tid; i < nc; i += NT) {
+        const int q = cand[i];
+        const int l = v.nlevel[q];
+        if (l < 0 || l > 63) {
+            atomicOr(&scal->status, NBHIP_ERROR);
+            continue;
+        }
+        if (v.node[q].next <= P1) {
+            atomicAdd(&cnt[l], 1);
+        } else {
+            const int k = atomicAdd(nspan, 1);
+            if (out && k < NBH_SPAN_SLOTS) out[k] = q;
+            else atomicOr(&scal->status, NBHIP_ERROR);
+        }
+    }
     __syncthreads();
     if (tid == 0) {
         int acc = 0;
@@ -1394,23 +1419,91 @@ __global__ __launch_bounds__(NBH_SPAN_THREADS) void k_sum_span(SumView v,
             acc += cnt[l];
         }
         off[64] = acc;
+        if (out_n) *out_n = min(*nspan, NBH_SPAN_SLOTS);
     }
     __syncthreads();
-    for (int i = tid; i < S; i += NBH_SPAN_THREADS) {
-        const int q = span[i];
+    for (int i = tid; i < nc; i += NT) {
+        const int q = cand[i];
         const int l = v.nlevel[q];
-        span_sorted[off[l] + atomicAdd(&fill[l], 1)] = q;
+        if (l < 0 || l > 63 || v.node[q].next > P1) continue;
+        list[off[l] + atomicAdd(&fill[l], 1)] = q;
     }
     __syncthreads();
     bool ok = true;
     for (int l = 63; l >= 0; --l) {
         const int c = off[l + 1] - off[l];
         if (c == 0) continue;                     /* uniform */
-        for (int i = tid; i < c; i += NBH_SPAN_THREADS)
-            ok = nbh_summarize_cell(v, span_sorted[off[l] + i]) && ok;
+        for (int i = tid; i < c; i += NT) ok = nbh_summarize_cell(v, list[off[l] + i]) && ok;
         __syncthreads();
     }
     if (!ok) atomicOr(&scal->status, NBHIP_ERROR);
+}
+
+/* Workgroup s: the spanning cells of blocks [s*NBH_SUM_GROUP, ...). */
+__global__ __launch_bounds__(NBH_SUM_THREADS) void k_sum_super(SumView v,
+                                                               const unsigned* __restrict__ cincl,
+                                                               const int* __restrict__ span,
+                                                               const int* __restrict__ span_n,
+                                                               int* __restrict__ span2,
+                                                               int* __restrict__ span2_n,
+                                                               DevScalars* __restrict__ scal)
+{
+    __shared__ int cand[NBH_SUM_GROUP * NBH_SPAN_SLOTS];
+    __shared__ int list[NBH_SUM_GROUP * NBH_SPAN_SLOTS];
+    __shared__ int cnt[64];
+    __shared__ int off[65];
+    __shared__ int fill[64];
+    __shared__ int nspan;
+    __shared__ int ncand;
+    const int tid = threadIdx.x;
+    if (scal->status & (NBHIP_CELL_OVERFLOW_ERROR | NBHIP_ERROR)) return;
+    const int M = scal->nmassive;
+    const int nblk = (M + NBH_SUM_BODIES - 1) / NBH_SUM_BODIES;
+    const int k0 = blockIdx.x * NBH_SUM_GROUP;
+    if (k0 >= nblk) return;
+    const int k1 = min(k0 + NBH_SUM_GROUP, nblk);
+    const int b1 = min(k1 * NBH_SUM_BODIES, M);
+    const int P1 = min((b1 - 1) + (int) cincl[b1 - 1] + 1, v.nn);
+    if (tid == 0) ncand = 0;
+    __syncthreads();
+    for (int k = k0 + tid; k < k1; k += NBH_SUM_THREADS) {
+        const int c = span_n[k];
+        const int base = atomicAdd(&ncand, c);
+        for (int j = 0; j < c; ++j) cand[base + j] = span[(size_t) k * NBH_SPAN_SLOTS + j];
+    }
+    __syncthreads();
+    nbh_sum_handed<NBH_SUM_THREADS>(v, cand, ncand, P1, span2 + (size_t) blockIdx.x * NBH_SPAN_SLOTS,
+                                    span2_n + blockIdx.x, list, cnt, off, fill, &nspan, scal);
+}
+
+/* One workgroup: every super-block's spanning cells. */
+__global__ __launch_bounds__(NBH_SPAN_THREADS) void k_sum_top(SumView v,
+                                                              const int* __restrict__ span2,
+                                                              const int* __restrict__ span2_n,
+                                                              int nsuper, int* __restrict__ cand,
+                                                              int* __restrict__ list,
+                                                              DevScalars* __restrict__ scal)
+{
+    __shared__ int cnt[64];
+    __shared__ int off[65];
+    __shared__ int fill[64];
+    __shared__ int nspan;
+    __shared__ int ncand;
+    const int tid = threadIdx.x;
+    if (scal->status & (NBHIP_CELL_OVERFLOW_ERROR | NBHIP_ERROR)) return;
+    const int M = scal->nmassive;
+    const int nblk = (M + NBH_SUM_BODIES - 1) / NBH_SUM_BODIES;
+    const int ns = min(nsuper, (nblk + NBH_SUM_GROUP - 1) / NBH_SUM_GROUP);
+    if (tid == 0) ncand = 0;
+    __syncthreads();
+    for (int k = tid; k < ns; k += NBH_SPAN_THREADS) {
+        const int c = span2_n[k];
+        const int base = atomicAdd(&ncand, c);
+        for (int j = 0; j < c; ++j) cand[base + j] = span2[(size_t) k * NBH_SPAN_SLOTS + j];
+    }
+    __syncthreads();
+    nbh_sum_handed<NBH_SPAN_THREADS>(v, cand, ncand, v.nn, nullptr, nullptr, list, cnt, off,
+                                     fill, &nspan, scal);
 }
 
 /* End of a KDK step (nbRunSystemPlain's loop, nbody/src/nbody_plain.c:
@@ -1571,9 +1664,21 @@ void nbh_build(nbhip_state* st, int graph_safe)
     v.ctx = st->ctx;
     v.rc_scale = st->open_scale;
     v.nn = st->node_cap;
-    hipLaunchKernelGGL(k_sum_blocks, dim3(nbh_blocks(n, NBH_SUM_BODIES)), dim3(NBH_SUM_THREADS),
-                       0, s, v, st->cincl, st->blist, st->span, st->scal);
-    hipLaunchKernelGGL(k_sum_span, dim3(1), dim3(NBH_SPAN_THREADS), 0, s, v, st->span,
-                       st->span_sorted, st->scal);
+    /* span: each block's spanning cells (NBH_SPAN_SLOTS each); span_sorted:
+     * each super-block's, then the per-block and per-super-block counts,
+     * then the top launch's candidates and level-sorted list */
+    const int nblk = nbh_blocks(n, NBH_SUM_BODIES);
+    const int nsup = nbh_blocks(nblk, NBH_SUM_GROUP);
+    int* span2 = st->span_sorted;
+    int* span_n = span2 + (size_t) nsup * NBH_SPAN_SLOTS;
+    int* span2_n = span_n + nblk;
+    int* top_cand = span2_n + nsup;
+    int* top_list = top_cand + (size_t) nsup * NBH_SPAN_SLOTS;
+    hipLaunchKernelGGL(k_sum_blocks, dim3(nblk), dim3(NBH_SUM_THREADS), 0, s, v, st->cincl,
+                       st->blist, st->span, span_n, st->scal);
+    hipLaunchKernelGGL(k_sum_super, dim3(nsup), dim3(NBH_SUM_THREADS), 0, s, v, st->cincl,
+                       st->span, span_n, span2, span2_n, st->scal);
+    hipLaunchKernelGGL(k_sum_top, dim3(1), dim3(NBH_SPAN_THREADS), 0, s, v, span2, span2_n, nsup,
+                       top_cand, top_list, st->scal);
     nbh_mark(st, 4);
 }
