@@ -27,6 +27,8 @@ sys.path.insert(0, ROOT)
 METRIC = "body-steps/s at N=1M θ=0.5: mixed vs FP64 GPU at ≤1e-5 rel accel err"
 PEAK = {"fp64": 78.6, "fp32": 157.3}     # TFLOP/s, MI355X published
 ERR_BOUND = 1e-5                         # median relative acceleration error of the metric
+# walk forms timed by default: 'precision[:open scale[:far-field ratio]]'
+ALL_FORMS = ["fp64", "mixed", "mixed_coherent", "mixed_group"]
 HBM_PEAK = 8000.0                        # GB/s
 
 
@@ -45,12 +47,17 @@ def make_workload(n: int, seed: int):
     return pot, pos, vel, m, eps2, dt
 
 
-def walk_flops(counts, use_quad=True):
+def walk_flops(counts, use_quad=True, nbody=0):
     """Algorithmic flops of one walk from the reference expressions
     (nbody/src/nbody_grav.c:53-99): 18 per interaction, +38 per accepted
-    cell with quadrupoles, 9 per opened-cell test."""
+    cell with quadrupoles, 9 per opened-cell test; for the group walk's
+    far-field expansions (counts[4]) 170 per (cell, group) pair (the
+    field, Jacobian and third derivatives at the group's centre, FMA = 2)
+    and 40 per body evaluating its group's expansion."""
     nbb, nbc, nop = counts[:3]
-    return 18.0 * nbb + (56.0 if use_quad else 18.0) * nbc + 9.0 * nop
+    far = counts[4] if len(counts) > 4 else 0
+    return (18.0 * nbb + (56.0 if use_quad else 18.0) * nbc + 9.0 * nop + 170.0 * far
+            + (40.0 * nbody if far else 0.0))
 
 
 def run_gpu(precision, pot, pos, vel, m, eps2, dt, steps, warmup, theta, autotune=True,
@@ -66,7 +73,13 @@ def run_gpu(precision, pot, pos, vel, m, eps2, dt, steps, warmup, theta, autotun
     from milkywayathome_client_amd import nbody as nb
     ctx = nb.NBodyCtx.create(timestep=dt, timeEvolve=dt * (steps + warmup), eps2=eps2,
                              theta=theta, criterion="BH86", useQuad=True, treeRSize=4.0)
-    st = nb.NBodyState(ctx, pot, pos, vel, m, precision=precision)
+    name, scale, ratio = parse_form(precision)
+    st = nb.NBodyState(ctx, pot, pos, vel, m, precision=name, prestep=False)
+    if scale != 1.0:
+        st.set_open_scale(scale)
+    if ratio > 0.0:
+        st.set_group_expansion(ratio)
+    st.gravity()
     st.sync()
     # the error gate: the pre-step force of the timed workload itself
     errs0 = accel_errors(st, pot, pos.shape[0])
@@ -89,7 +102,7 @@ def run_gpu(precision, pot, pos, vel, m, eps2, dt, steps, warmup, theta, autotun
     st.sync()
     wall = time.perf_counter() - t0
     st.set_profiling(False)
-    counts = st.walk_counters()
+    counts = st.walk_counters5() if name == "mixed_group" else st.walk_counters()
     graph_ms = None
     if graph:
         st.set_graph(True)
@@ -101,6 +114,14 @@ def run_gpu(precision, pot, pos, vel, m, eps2, dt, steps, warmup, theta, autotun
         graph_ms = 1e3 * (time.perf_counter() - t0) / steps
         st.set_graph(False)
     return st, wall, phases / steps, counts, tune, graph_ms, errs0, acc0
+
+
+def parse_form(form):
+    """'precision[:open scale[:far-field ratio]]' -> (precision, scale, ratio)."""
+    parts = form.split(":")
+    scale = float(parts[1]) if len(parts) > 1 else 1.0
+    ratio = float(parts[2]) if len(parts) > 2 else 0.0
+    return parts[0], scale, ratio
 
 
 def accel_errors(st, pot, n, nsample=4096, seed=1):
@@ -431,11 +452,10 @@ def main():
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--seed", type=int, default=34086709)
     ap.add_argument("--theta", type=float, default=0.5)
-    ap.add_argument("--precision", choices=["all", "both", "fp64", "mixed", "mixed_coherent",
-                                            "mixed_group"],
-                    default="all",
-                    help="all: fp64, mixed, mixed_coherent and mixed_group; both: fp64 and "
-                         "mixed")
+    ap.add_argument("--precision", default="all",
+                    help="all: the forms of ALL_FORMS; both: fp64 and mixed; or a comma list "
+                         "of forms 'precision[:open scale[:far-field ratio]]' with precision "
+                         "fp64, mixed, mixed_coherent or mixed_group")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default walk settings instead of timing the candidates")
@@ -487,8 +507,8 @@ def main():
 
     from milkywayathome_client_amd import nbody as nb
     results, acc0 = {}, {}
-    precs = {"all": ["fp64", "mixed", "mixed_coherent", "mixed_group"],
-             "both": ["fp64", "mixed"]}.get(args.precision, [args.precision])
+    precs = {"all": ALL_FORMS,
+             "both": ["fp64", "mixed"]}.get(args.precision, args.precision.split(","))
     for prec in precs:
         log(f"[bench] {prec}: warmup {args.warmup}, timed {args.steps}")
         try:
@@ -518,7 +538,7 @@ def main():
     if not results:
         print(json.dumps({"metric": METRIC, "error": "no GPU result (see stderr)"}))
         sys.exit(1)
-    mixed_modes = [k for k in ("mixed", "mixed_coherent", "mixed_group") if k in results]
+    mixed_modes = [k for k in results if k != "fp64"]
     for k in mixed_modes:
         if "fp64" in acc0:
             rel = (np.linalg.norm(acc0[k] - acc0["fp64"], axis=1)
@@ -534,7 +554,7 @@ def main():
     head = max(pool, key=lambda k: results[k]["body_steps_per_s"]) if pool else "fp64"
     r = results[head]
     walk_ms = float(r["phases_ms"][4])
-    flops = walk_flops(r["counts"])
+    flops = walk_flops(r["counts"], nbody=args.n)
     achieved = flops / (walk_ms * 1e-3) / 1e12
     peak = PEAK["fp64" if head == "fp64" else "fp32"]
     traffic = load_traffic()

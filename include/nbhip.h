@@ -242,6 +242,9 @@ unsigned nbhip_walk_accel(nbhip_state* st, double* acc);
  * (flop accounting), [3] wavefront node steps (64 x [3] lane slots were
  * issued; ([0]+[1]+[2]) / (64 x [3]) is the walk's lane utilisation). */
 unsigned nbhip_walk_counters(nbhip_state* st, unsigned long long* counts4);
+/* The same four counters and [4]: the group walk's far-field (cell, group)
+ * expansions ([1] then counts only the cells evaluated per body). */
+unsigned nbhip_walk_counters5(nbhip_state* st, unsigned long long* counts5);
 
 /* Per-phase device time of the last step in milliseconds, measured with HIP
  * events on the library's stream: [0] drift+bbox, [1] keys+sort, [2] build,
@@ -288,6 +291,18 @@ unsigned nbhip_set_graph(nbhip_state* st, int enable);
  * at the next tree build; 1 is the default.  NBHIP_USER_ERROR for another
  * precision or a scale outside [1, 1e6]. */
 unsigned nbhip_set_open_scale(nbhip_state* st, double scale);
+
+/* No counterpart in the reference.  Group walk only
+ * (NBHIP_PRECISION_MIXED_GROUP): an accepted cell whose distance |D| from a
+ * group's centre satisfies u_max <= ratio * |D| (u_max: half the diagonal
+ * of the group's bounding box) is evaluated once per group as a Taylor
+ * expansion of its softened field about the centre -- field, Jacobian
+ * (monopole + quadrupole) and third derivatives (monopole) -- instead of
+ * once per body; the remainder is O(ratio^3) of the cell's field.  The
+ * cells accepted do not change.  0 (the default) evaluates every accepted
+ * cell per body.  Takes effect at the next walk; NBHIP_USER_ERROR for
+ * another precision or a ratio outside [0, 0.5]. */
+unsigned nbhip_set_group_expansion(nbhip_state* st, double ratio);
 
 /* Walk launch parameters: workgroup size (64, 128, 256 threads) and
  * `batch`: for the mixed walks the node block (a power of two, in pre-order

@@ -166,6 +166,7 @@ static unsigned nbh_create(const nbhip_ctx* ctx, int device, int nsim, const int
     st->sim_shift = nsim > 1 ? 3 * st->top_levels : 64;
     st->node_cap = 5 * st->n + 1024 * nsim;
     st->open_scale = 1.0;
+    st->group_far = 0.0;
     st->sort_blocks = st->n > 0 ? (st->n + NBH_SORT_TILE - 1) / NBH_SORT_TILE : 1;
     st->tune.block = 256;
     st->tune.batch = 1024;
@@ -750,9 +751,9 @@ extern "C" unsigned nbhip_walk_accel(nbhip_state* st, double* acc)
     return NBHIP_SUCCESS;
 }
 
-extern "C" unsigned nbhip_walk_counters(nbhip_state* st, unsigned long long* counts4)
+static unsigned nbh_walk_counters(nbhip_state* st, unsigned long long* counts, int ncount)
 {
-    if (!st || !counts4) return NBHIP_USER_ERROR;
+    if (!st || !counts) return NBHIP_USER_ERROR;
     if (!st->have_tree) {
         nbh_set_error("no tree: call nbhip_gravity first");
         return NBHIP_USER_ERROR;
@@ -761,7 +762,7 @@ extern "C" unsigned nbhip_walk_counters(nbhip_state* st, unsigned long long* cou
     const size_t n = (size_t) st->n;
     double* t = nullptr;
     NBH_CHECK(hipMalloc(&t, 3 * (n ? n : 1) * sizeof(double)));
-    unsigned long long zero[4] = { 0, 0, 0, 0 };
+    unsigned long long zero[5] = { 0, 0, 0, 0, 0 };
     hipError_t e = hipMemcpyAsync(&st->scal->counters[0], zero, sizeof(zero),
                                   hipMemcpyHostToDevice, st->stream);
     if (e == hipSuccess) {
@@ -769,7 +770,7 @@ extern "C" unsigned nbhip_walk_counters(nbhip_state* st, unsigned long long* cou
         e = hipGetLastError();
     }
     if (e == hipSuccess)
-        e = hipMemcpyAsync(counts4, &st->scal->counters[0], 4 * sizeof(unsigned long long),
+        e = hipMemcpyAsync(counts, &st->scal->counters[0], ncount * sizeof(unsigned long long),
                            hipMemcpyDeviceToHost, st->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(st->stream);
     (void) hipFree(t);
@@ -778,6 +779,16 @@ extern "C" unsigned nbhip_walk_counters(nbhip_state* st, unsigned long long* cou
         return NBHIP_GPU_ERROR;
     }
     return NBHIP_SUCCESS;
+}
+
+extern "C" unsigned nbhip_walk_counters(nbhip_state* st, unsigned long long* counts4)
+{
+    return nbh_walk_counters(st, counts4, 4);
+}
+
+extern "C" unsigned nbhip_walk_counters5(nbhip_state* st, unsigned long long* counts5)
+{
+    return nbh_walk_counters(st, counts5, 5);
 }
 
 extern "C" unsigned nbhip_set_profiling(nbhip_state* st, int enable)
@@ -822,6 +833,22 @@ extern "C" unsigned nbhip_set_open_scale(nbhip_state* st, double scale)
     }
     st->open_scale = scale;
     st->graph_ready = 0;          /* the captured step carries the old factor */
+    return NBHIP_SUCCESS;
+}
+
+extern "C" unsigned nbhip_set_group_expansion(nbhip_state* st, double ratio)
+{
+    if (!st) return NBHIP_USER_ERROR;
+    if (st->ctx.precision != NBHIP_PRECISION_MIXED_GROUP) {
+        nbh_set_error("nbhip_set_group_expansion: only the group walk expands far cells");
+        return NBHIP_USER_ERROR;
+    }
+    if (!(ratio >= 0.0) || !(ratio <= 0.5)) {
+        nbh_set_error("nbhip_set_group_expansion: ratio %.17g is outside [0, 0.5]", ratio);
+        return NBHIP_USER_ERROR;
+    }
+    st->group_far = ratio;
+    st->graph_ready = 0;          /* the captured step carries the old ratio */
     return NBHIP_SUCCESS;
 }
 

@@ -84,7 +84,8 @@ struct DevScalars {
     int maxdepth;
     unsigned status;               /* sticky global bits (overflow, internal error) */
     int sim_bits;                  /* key bits holding the simulation id */
-    unsigned long long counters[4];/* body-body, body-cell, opened, wave node steps */
+    unsigned long long counters[5];/* body-body, body-cell, opened, wave node steps,
+                                      group far-field (cell, group) pairs */
     int halt;                      /* a step ended with a fatal status: later steps do nothing */
     unsigned steps_done;           /* KDK steps completed on the device */
     int level_count[64];           /* cells per level */
@@ -191,6 +192,7 @@ struct nbhip_state {
     unsigned* hist_buf;           /* device counts + totals of nbhip_histogram */
     size_t hist_cap;              /* unsigned entries in hist_buf */
     double open_scale;            /* coherent walk: critical radius^2 factor (>= 1) */
+    double group_far;             /* group walk: far-field expansion ratio (0: off) */
     char* disp_buf;               /* scratch of nbhip_histogram_dispersion (grow-only) */
     size_t disp_cap;              /* bytes in disp_buf */
 

@@ -16,6 +16,7 @@ struct WalkOut {
     int multi;          /* several simulations: lanes park at their own end */
     int all_eps2_pos;   /* every simulation has eps2 > 0 */
     int batch_shift;    /* mixed walk: log2 of the node block per FP32 partial sum */
+    double far2;        /* group walk: squared far-field ratio (0: every cell per body) */
     double dtHalf;
     nbhip_potential pot;
 };

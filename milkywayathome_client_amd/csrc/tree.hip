@@ -150,6 +150,7 @@ __global__ __launch_bounds__(256) void k_rsize(int nsim, SimTab* __restrict__ si
         scal->nnode = 0;
         scal->maxdepth = 0;
         scal->counters[0] = scal->counters[1] = scal->counters[2] = scal->counters[3] = 0ull;
+        scal->counters[4] = 0ull;
     }
     if (threadIdx.x < 64) {
         scal->level_count[threadIdx.x] = 0;

@@ -363,6 +363,7 @@ void nbh_walk(nbhip_state* st, int kick, int count, double* oax, double* oay, do
     int bsh = 0;
     while (bsh < 30 && (1 << (bsh + 1)) <= (t.batch > 0 ? t.batch : 1024)) ++bsh;
     o.batch_shift = bsh;
+    o.far2 = st->group_far * st->group_far;
     o.dtHalf = 0.5 * c.timestep;
     o.pot = c.pot;
     if (c.precision == NBHIP_PRECISION_MIXED_GROUP) {

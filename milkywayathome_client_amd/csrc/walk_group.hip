@@ -25,7 +25,17 @@
  *     interacts with every entry (monopole + quadrupole with the reference's
  *     softened expressions, nbody_grav.c:53-99, in FP32 with v_rsq_f32); one
  *     drained list is one FP32 partial sum, added into FP64 with Kahan
- *     compensation.
+ *     compensation;
+ *  6. with a far-field ratio delta > 0 (nbhip_set_group_expansion), an
+ *     accepted cell whose distance D from the group's centre satisfies
+ *     u_max <= delta |D| (u_max: half the diagonal of the group's box) is not
+ *     evaluated per body: each lane takes one such cell and adds its softened
+ *     field g = grad Phi, Jacobian J = grad grad Phi (monopole + quadrupole)
+ *     and third-derivative tensor K (monopole) at the centre into 19 sums,
+ *     which are added across the wave once per group; every member body then
+ *     adds g - J u + K[u,u] / 2 at its offset u (a Taylor expansion whose
+ *     remainder is O((u/|D|)^3) of the cell's field).  Far cells are kept at
+ *     the back of the cell list (near ones at the front).
  *
  * The tests and the table are FP64 and deterministic, so a body's list does
  * not depend on timing, and repeated runs are bit-identical.
@@ -109,7 +119,7 @@ __device__ __forceinline__ unsigned long long nbh_group_cuts(unsigned long long 
     return cuts;
 }
 
-template <int WPB, int LC, bool QUAD, bool EPS0, bool COUNT>
+template <int WPB, int LC, bool QUAD, bool EPS0, bool COUNT, bool FAR>
 __global__ __launch_bounds__(64 * WPB) void k_walk_group(const NodeG* __restrict__ T,
                                                          const double4* __restrict__ spos,
                                                          const unsigned* __restrict__ perm,
@@ -168,7 +178,7 @@ __global__ __launch_bounds__(64 * WPB) void k_walk_group(const NodeG* __restrict
         nbh_group_cuts(__builtin_amdgcn_ballot_w64(hard) | (1ull << 63), lvl, lane);
 
     double ax = 0.0, ay = 0.0, az = 0.0, kx = 0.0, ky = 0.0, kz = 0.0;
-    unsigned nbb = 0, nbc = 0, nop = 0, iters = 0;
+    unsigned nbb = 0, nbc = 0, nop = 0, iters = 0, nfar = 0;
     unsigned self_hits = 0;
     bool bad = false;
     unsigned long long remaining = valid_m;
@@ -197,11 +207,20 @@ __global__ __launch_bounds__(64 * WPB) void k_walk_group(const NodeG* __restrict
         const float rz = me ? (float) (p.z - gz) : 0.0f;
         const int expected = __popcll(mem & massive_m);
         int own_seen = 0;
+        /* far field: u_max^2 (half the box diagonal, squared) and the
+         * lane's partial sums of g (3), J (xx xy xz yy yz zz) and K (xxx
+         * xxy xxz xyy xyz xzz yyy yyz yzz zzz) */
+        const double hbx = 0.5 * (hix - lox), hby = 0.5 * (hiy - loy), hbz = 0.5 * (hiz - loz);
+        const double umax2 = hbz * hbz + (hby * hby + hbx * hbx);
+        const bool use_far = FAR;
+        float fg[3] = { 0.0f, 0.0f, 0.0f };
+        float fj[6] = { 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f };
+        float fk[10] = { 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f };
 
         int fsize = 1;
         if (lane == 0) fr[0] = (sim_u << 3);          /* slot sim_u, one node */
         __builtin_amdgcn_wave_barrier();
-        int nc = 0, nb = 0;
+        int nc = 0, nb = 0, nf = 0;
 
         /* drains of the two lists */
         auto flush_cells = [&](int cnt) {
@@ -245,6 +264,81 @@ __global__ __launch_bounds__(64 * WPB) void k_walk_group(const NodeG* __restrict
                 nbc += me ? (unsigned) cnt : 0u;
                 iters += (unsigned) cnt;
             }
+        };
+        /* far cells: slots LC-1, LC-2, ...; one cell per lane */
+        auto flush_far = [&](int cnt) {
+            __builtin_amdgcn_wave_barrier();
+            for (int j0 = 0; j0 < cnt; j0 += 64) {
+                const int e = j0 + lane;
+                if (e < cnt) {
+                    const int slot = LC - 1 - e;
+                    const float4 a = c0[slot];
+                    const float dx = a.x, dy = a.y, dz = a.z, m = a.w;
+                    const float s2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy,
+                                                    __builtin_fmaf(dz, dz, eps2)));
+                    const float ri = __builtin_amdgcn_rsqf(s2);
+                    const float ri2 = ri * ri;
+                    const float ri3 = ri * ri2, ri5 = ri3 * ri2, ri7 = ri5 * ri2;
+                    /* monopole: g = m d / s^3, J = m (I / s^3 - 3 d d^T / s^5),
+                     * K = B (delta d + ...) + A d d d, B = -3 m / s^5, A = 15 m / s^7 */
+                    const float mr3 = m * ri3;
+                    const float B = -3.0f * m * ri5;
+                    const float A = 15.0f * m * ri7;
+                    const float xx = dx * dx, xy = dx * dy, xz = dx * dz;
+                    const float yy = dy * dy, yz = dy * dz, zz = dz * dz;
+                    fg[0] = __builtin_fmaf(mr3, dx, fg[0]);
+                    fg[1] = __builtin_fmaf(mr3, dy, fg[1]);
+                    fg[2] = __builtin_fmaf(mr3, dz, fg[2]);
+                    fj[0] += __builtin_fmaf(B, xx, mr3);
+                    fj[1] = __builtin_fmaf(B, xy, fj[1]);
+                    fj[2] = __builtin_fmaf(B, xz, fj[2]);
+                    fj[3] += __builtin_fmaf(B, yy, mr3);
+                    fj[4] = __builtin_fmaf(B, yz, fj[4]);
+                    fj[5] += __builtin_fmaf(B, zz, mr3);
+                    const float Ax = A * dx, Ay = A * dy, Az = A * dz;
+                    fk[0] += __builtin_fmaf(Ax, xx, 3.0f * B * dx);   /* xxx */
+                    fk[1] += __builtin_fmaf(Ax, xy, B * dy);          /* xxy */
+                    fk[2] += __builtin_fmaf(Ax, xz, B * dz);          /* xxz */
+                    fk[3] += __builtin_fmaf(Ax, yy, B * dx);          /* xyy */
+                    fk[4] = __builtin_fmaf(Ax, yz, fk[4]);            /* xyz */
+                    fk[5] += __builtin_fmaf(Ax, zz, B * dx);          /* xzz */
+                    fk[6] += __builtin_fmaf(Ay, yy, 3.0f * B * dy);   /* yyy */
+                    fk[7] += __builtin_fmaf(Ay, yz, B * dz);          /* yyz */
+                    fk[8] += __builtin_fmaf(Ay, zz, B * dy);          /* yzz */
+                    fk[9] += __builtin_fmaf(Az, zz, 3.0f * B * dz);   /* zzz */
+                    if (QUAD) {
+                        /* g_Q = 2.5 (dQd) d / s^7 - Q d / s^5, J_Q = -Q / s^5
+                         * + 5 (q d^T + d q^T) / s^7 + 2.5 dQd I / s^7
+                         * - 17.5 dQd d d^T / s^9 (q = Q d) */
+                        const float4 b = c1[slot];
+                        const float2 e2 = c2[slot];
+                        const float qx = __builtin_fmaf(b.x, dx, __builtin_fmaf(b.y, dy, b.z * dz));
+                        const float qy = __builtin_fmaf(b.y, dx, __builtin_fmaf(b.w, dy, e2.x * dz));
+                        const float qz = __builtin_fmaf(b.z, dx, __builtin_fmaf(e2.x, dy, e2.y * dz));
+                        const float dqd = __builtin_fmaf(qx, dx, __builtin_fmaf(qy, dy, qz * dz));
+                        const float h = 2.5f * dqd * ri7;
+                        fg[0] += __builtin_fmaf(h, dx, -ri5 * qx);
+                        fg[1] += __builtin_fmaf(h, dy, -ri5 * qy);
+                        fg[2] += __builtin_fmaf(h, dz, -ri5 * qz);
+                        const float f5 = 5.0f * ri7;
+                        const float g9 = -17.5f * dqd * ri7 * ri2;
+                        fj[0] += __builtin_fmaf(g9, xx, __builtin_fmaf(2.0f * f5, qx * dx,
+                                                                       h - ri5 * b.x));
+                        fj[1] += __builtin_fmaf(g9, xy, __builtin_fmaf(f5, __builtin_fmaf(qx, dy, qy * dx),
+                                                                       -ri5 * b.y));
+                        fj[2] += __builtin_fmaf(g9, xz, __builtin_fmaf(f5, __builtin_fmaf(qx, dz, qz * dx),
+                                                                       -ri5 * b.z));
+                        fj[3] += __builtin_fmaf(g9, yy, __builtin_fmaf(2.0f * f5, qy * dy,
+                                                                       h - ri5 * b.w));
+                        fj[4] += __builtin_fmaf(g9, yz, __builtin_fmaf(f5, __builtin_fmaf(qy, dz, qz * dy),
+                                                                       -ri5 * e2.x));
+                        fj[5] += __builtin_fmaf(g9, zz, __builtin_fmaf(2.0f * f5, qz * dz,
+                                                                       h - ri5 * e2.y));
+                    }
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+            if (COUNT) nfar += (unsigned) cnt;
         };
         auto flush_bodies = [&](int cnt) {
             float tx = 0.0f, ty = 0.0f, tz = 0.0f;
@@ -296,7 +390,13 @@ __global__ __launch_bounds__(64 * WPB) void k_walk_group(const NodeG* __restrict
             const bool isbody = r.count == 0;
             const bool acc = act && (isbody || d2 >= (double) r.rcrit2);
             const bool opn = act && !acc;
-            const unsigned long long cm = __builtin_amdgcn_ballot_w64(acc && !isbody);
+            bool far = false;
+            if (FAR) {
+                const double Dx = r.x - gx, Dy = r.y - gy, Dz = r.z - gz;
+                far = acc && !isbody && umax2 <= o.far2 * (Dz * Dz + (Dy * Dy + Dx * Dx));
+            }
+            const unsigned long long cm = __builtin_amdgcn_ballot_w64(acc && !isbody && !far);
+            const unsigned long long fm = FAR ? __builtin_amdgcn_ballot_w64(far) : 0ull;
             const unsigned long long bm = __builtin_amdgcn_ballot_w64(acc && isbody);
             const unsigned long long om = __builtin_amdgcn_ballot_w64(opn);
             /* a member's own leaf: its body meets itself (no incest) */
@@ -309,7 +409,8 @@ __global__ __launch_bounds__(64 * WPB) void k_walk_group(const NodeG* __restrict
                 break;
             }
             if (acc && !isbody) {
-                const int slot = nc + (int) nbh_mbcnt(cm);
+                const int slot = (FAR && far) ? LC - 1 - (nf + (int) nbh_mbcnt(fm))
+                                     : nc + (int) nbh_mbcnt(cm);
                 c0[slot] = make_float4((float) (r.x - gx), (float) (r.y - gy),
                                        (float) (r.z - gz), r.m);
                 if (QUAD) {
@@ -325,12 +426,15 @@ __global__ __launch_bounds__(64 * WPB) void k_walk_group(const NodeG* __restrict
             if (opn) fr[fsize + (int) nbh_mbcnt(om)] = (r.link << 3) | (r.count - 1);
             __builtin_amdgcn_wave_barrier();
             nc += __popcll(cm);
+            nf += __popcll(fm);
             nb += __popcll(bm);
             fsize += npush;
             if (COUNT) nop += (unsigned) npush;
-            if (nc > LC - 64) {
-                flush_cells(nc);
+            if (nc + nf > LC - 64) {
+                if (nc > 0) flush_cells(nc);
+                if (FAR && nf > 0) flush_far(nf);
                 nc = 0;
+                nf = 0;
             }
             if (nb > LC - 64) {
                 flush_bodies(nb);
@@ -338,7 +442,31 @@ __global__ __launch_bounds__(64 * WPB) void k_walk_group(const NodeG* __restrict
             }
         }
         if (nc > 0) flush_cells(nc);
+        if (FAR && nf > 0) flush_far(nf);
         if (nb > 0) flush_bodies(nb);
+        if (use_far) {
+            /* the wave's sums (every lane holds the total), then each member
+             * body adds g - J u + K[u,u] / 2 */
+            for (int off = 32; off > 0; off >>= 1) {
+                for (int k = 0; k < 3; ++k) fg[k] += __shfl_xor(fg[k], off, 64);
+                for (int k = 0; k < 6; ++k) fj[k] += __shfl_xor(fj[k], off, 64);
+                for (int k = 0; k < 10; ++k) fk[k] += __shfl_xor(fk[k], off, 64);
+            }
+            if (me) {
+                const float ux = rx, uy = ry, uz = rz;
+                const float uxx = 0.5f * ux * ux, uyy = 0.5f * uy * uy, uzz = 0.5f * uz * uz;
+                const float uxy = ux * uy, uxz = ux * uz, uyz = uy * uz;
+                float tx = fg[0] - __builtin_fmaf(fj[0], ux, __builtin_fmaf(fj[1], uy, fj[2] * uz));
+                float ty = fg[1] - __builtin_fmaf(fj[1], ux, __builtin_fmaf(fj[3], uy, fj[4] * uz));
+                float tz = fg[2] - __builtin_fmaf(fj[2], ux, __builtin_fmaf(fj[4], uy, fj[5] * uz));
+                tx += fk[0] * uxx + fk[3] * uyy + fk[5] * uzz + fk[1] * uxy + fk[2] * uxz + fk[4] * uyz;
+                ty += fk[1] * uxx + fk[6] * uyy + fk[8] * uzz + fk[3] * uxy + fk[4] * uxz + fk[7] * uyz;
+                tz += fk[2] * uxx + fk[7] * uyy + fk[9] * uzz + fk[4] * uxy + fk[5] * uxz + fk[8] * uyz;
+                nbh_kahan_add(ax, kx, (double) tx);
+                nbh_kahan_add(ay, ky, (double) ty);
+                nbh_kahan_add(az, kz, (double) tz);
+            }
+        }
         if (COUNT) self_hits += (unsigned) own_seen;
         if (own_seen < expected && lane == 0) atomicOr(&sims[sim_u].incest, 1u);
         if (bad) break;
@@ -355,6 +483,7 @@ __global__ __launch_bounds__(64 * WPB) void k_walk_group(const NodeG* __restrict
             atomicAdd(&scal->counters[1], c1);
             atomicAdd(&scal->counters[2], (unsigned long long) nop);
             atomicAdd(&scal->counters[3], (unsigned long long) iters);
+            atomicAdd(&scal->counters[4], (unsigned long long) nfar);
         }
     }
     if (!valid) return;
@@ -372,10 +501,13 @@ void nbh_walk_group(nbhip_state* st, const WalkOut& o, double* oax, double* oay,
     dim3 grid(nbh_gblocks(st->n, bs)), block(bs);
     const bool quad = st->ctx.use_quad != 0;
     const bool eps0 = !o.all_eps2_pos;
+    const bool far = o.far2 > 0.0;
+#define NBH_GWF(W, L, Q, E, C, F)                                                             \
+    hipLaunchKernelGGL((k_walk_group<W, L, Q, E, C, F>), grid, block, 0, st->stream,           \
+                       st->nodeg, st->spos, st->perm, st->simid_s, st->dadj, st->sims, oax,    \
+                       oay, oaz, st->vx, st->vy, st->vz, st->scal, o)
 #define NBH_GWL(W, L, Q, E, C)                                                                \
-    hipLaunchKernelGGL((k_walk_group<W, L, Q, E, C>), grid, block, 0, st->stream, st->nodeg,   \
-                       st->spos, st->perm, st->simid_s, st->dadj, st->sims, oax, oay, oaz,     \
-                       st->vx, st->vy, st->vz, st->scal, o)
+    do { if (far) NBH_GWF(W, L, Q, E, C, true); else NBH_GWF(W, L, Q, E, C, false); } while (0)
 #define NBH_GWC(W, L, Q, E)                                                                   \
     do { if (count) NBH_GWL(W, L, Q, E, true); else NBH_GWL(W, L, Q, E, false); } while (0)
 #define NBH_GWE(W, L, Q)                                                                      \
@@ -393,4 +525,5 @@ void nbh_walk_group(nbhip_state* st, const WalkOut& o, double* oax, double* oay,
 #undef NBH_GWE
 #undef NBH_GWC
 #undef NBH_GWL
+#undef NBH_GWF
 }

@@ -212,6 +212,7 @@ EXPORTED = [
     "nbhip_run_system", "nbhip_best_bodies", "nbhip_checkpoint_pack", "nbhip_checkpoint_unpack",
     "nbhip_checkpoint_write", "nbhip_checkpoint_read", "nbhip_mixed_dwarf",
     "nbhip_mixed_dwarf_state", "nbhip_dwarf_df", "nbhip_philox4x32_10",
+    "nbhip_set_group_expansion", "nbhip_walk_counters5",
 ]
 
 _LIB: Optional[C.CDLL] = None
@@ -288,6 +289,8 @@ def load_library() -> C.CDLL:
                                         _dp, _dp, _dp]),
         "nbhip_checkpoint_write": (u, [st, C.c_char_p, C.POINTER(_CkptIO)]),
         "nbhip_checkpoint_read": (u, [st, C.c_char_p, C.POINTER(_CkptIO)]),
+        "nbhip_set_group_expansion": (u, [st, C.c_double]),
+        "nbhip_walk_counters5": (u, [st, C.POINTER(C.c_ulonglong)]),
         "nbhip_mixed_dwarf": (u, [C.POINTER(_DwarfParams), C.c_int, _dp, _dp, _dp, _dp]),
         "nbhip_mixed_dwarf_state": (u, [st, C.c_int, C.POINTER(_DwarfParams), _dp]),
         "nbhip_dwarf_df": (u, [C.POINTER(_DwarfParams), C.c_int, C.c_int, C.c_int, _dp, _dp]),
@@ -838,6 +841,18 @@ class NBodyState:
         c = (C.c_ulonglong * 4)()
         self._check(self.lib.nbhip_walk_counters(self._h, c))
         return int(c[0]), int(c[1]), int(c[2]), int(c[3])
+
+    def walk_counters5(self):
+        """walk_counters() and the group walk's far-field (cell, group) expansions."""
+        c = (C.c_ulonglong * 5)()
+        self._check(self.lib.nbhip_walk_counters5(self._h, c))
+        return tuple(int(x) for x in c)
+
+    def set_group_expansion(self, ratio: float) -> None:
+        """Group walk only: evaluate an accepted cell once per group, as a
+        Taylor expansion about the group's centre, when the group's half
+        box diagonal is at most `ratio` times its distance (0: off)."""
+        self._check(self.lib.nbhip_set_group_expansion(self._h, float(ratio)))
 
     def set_graph(self, enable: bool = True) -> None:
         """Replay steps as a captured hipGraph (no host sync inside a step)."""

@@ -201,3 +201,111 @@ class GroupWalk:
             nbb += (b - a) * nbody - own
             nbc += (b - a) * ncell
         return nbb, nbc
+
+
+# ---------------------------------------------------------------------------
+# Far-field expansion (nbhip_set_group_expansion)
+# ---------------------------------------------------------------------------
+# Phi(d) = -M / s - (d^T Q d) / (2 s^5), s^2 = |d|^2 + eps2, is the softened
+# potential whose gradient the reference's interaction evaluates
+# (a = M d / s^3 + 2.5 (d^T Q d) d / s^7 - Q d / s^5, nbody_grav.c:53-99).
+# A far cell seen from a group's centre at D contributes, at body offset u,
+# a(D - u) ~ g - J u + K[u, u] / 2 with g, J and K the first three
+# derivatives of Phi at D (K from the monopole only).
+
+def _derivs(D, m, Q, eps2):
+    """g (n,3), J (n,3,3), K (n,3,3,3) of each cell at D (FP64)."""
+    I = np.eye(3)
+    s2 = (D * D).sum(1) + eps2
+    s = np.sqrt(s2)
+    r3, r5 = 1.0 / (s2 * s), 1.0 / (s2 * s2 * s)
+    r7, r9 = r5 / s2, r5 / (s2 * s2)
+    Qm = np.empty((D.shape[0], 3, 3))
+    Qm[:, 0, 0], Qm[:, 0, 1], Qm[:, 0, 2] = Q[:, 0], Q[:, 1], Q[:, 2]
+    Qm[:, 1, 0], Qm[:, 1, 1], Qm[:, 1, 2] = Q[:, 1], Q[:, 3], Q[:, 4]
+    Qm[:, 2, 0], Qm[:, 2, 1], Qm[:, 2, 2] = Q[:, 2], Q[:, 4], Q[:, 5]
+    q = np.einsum("nij,nj->ni", Qm, D)
+    dqd = (q * D).sum(1)
+    DD = np.einsum("ni,nj->nij", D, D)
+    g = (m * r3)[:, None] * D + (2.5 * dqd * r7)[:, None] * D - r5[:, None] * q
+    J = ((m * r3)[:, None, None] * I - (3 * m * r5)[:, None, None] * DD
+         - r5[:, None, None] * Qm
+         + (5 * r7)[:, None, None] * (np.einsum("ni,nj->nij", q, D) + np.einsum("ni,nj->nij", D, q))
+         + (2.5 * dqd * r7)[:, None, None] * I - (17.5 * dqd * r9)[:, None, None] * DD)
+    sym = (np.einsum("ij,nk->nijk", I, D) + np.einsum("ik,nj->nijk", I, D)
+           + np.einsum("jk,ni->nijk", I, D))
+    K = (-3 * m * r5)[:, None, None, None] * sym + \
+        (15 * m * r7)[:, None, None, None] * np.einsum("ni,nj,nk->nijk", D, D, D)
+    return g, J, K
+
+
+def taylor_field(D, m, Q, eps2, u):
+    """sum over cells of g - J u + K[u,u]/2 at offsets u (k,3)."""
+    g, J, K = _derivs(np.atleast_2d(D), np.atleast_1d(m), np.atleast_2d(Q), eps2)
+    G, JJ, KK = g.sum(0), J.sum(0), K.sum(0)
+    return G[None, :] - u @ JJ.T + 0.5 * np.einsum("ijk,nj,nk->ni", KK, u, u)
+
+
+def exact_field(D, m, Q, eps2, u):
+    """sum over cells of the reference interaction at d = D - u (k,3)."""
+    out = np.zeros((u.shape[0], 3))
+    for c in range(np.atleast_2d(D).shape[0]):
+        d = np.atleast_2d(D)[c] - u
+        qq = np.atleast_2d(Q)[c]
+        s2 = (d * d).sum(1) + eps2
+        s = np.sqrt(s2)
+        qx = qq[0] * d[:, 0] + qq[1] * d[:, 1] + qq[2] * d[:, 2]
+        qy = qq[1] * d[:, 0] + qq[3] * d[:, 1] + qq[4] * d[:, 2]
+        qz = qq[2] * d[:, 0] + qq[4] * d[:, 1] + qq[5] * d[:, 2]
+        dqd = qx * d[:, 0] + qy * d[:, 1] + qz * d[:, 2]
+        r5 = 1.0 / (s2 * s2 * s)
+        out += (np.atleast_1d(m)[c] / (s2 * s) + 2.5 * r5 * dqd / s2)[:, None] * d \
+            - r5[:, None] * np.stack([qx, qy, qz], 1)
+    return out
+
+
+def _split(self, a, b, ratio):
+    """(near list, far cells) of sub-group [a, b): a cell is far when the
+    group's half box diagonal is at most ratio times its distance from the
+    group's centre (the GPU's FP64 test, same association)."""
+    lst = self.accepted(a, b)
+    p = self.spos[a:b]
+    lo, hi = p.min(0), p.max(0)
+    g = 0.5 * (lo + hi)
+    h = 0.5 * (hi - lo)
+    umax2 = h[2] * h[2] + (h[1] * h[1] + h[0] * h[0])
+    D = self.cpos[lst] - g
+    dist2 = D[:, 2] * D[:, 2] + (D[:, 1] * D[:, 1] + D[:, 0] * D[:, 0])
+    far = (self.kind[lst] == 0) & (umax2 <= (ratio * ratio) * dist2) & (ratio > 0)
+    return lst[~far], lst[far]
+
+
+def _expanded_forces(self, a, b, ratio):
+    """FP64 self-gravity of sorted bodies [a, b) with far cells expanded."""
+    near, far = self.split(a, b, ratio)
+    out = self.forces(a, b, near)
+    if far.size:
+        p = self.spos[a:b]
+        g = 0.5 * (p.min(0) + p.max(0))
+        out = out + taylor_field(self.cpos[far] - g, self.mass[far], self.quad[far], self.eps2,
+                                 p - g)
+    return out
+
+
+def _expanded_counts(self, ratio, groups=None):
+    """(body-body without self, near body-cell, far (cell, group) pairs)."""
+    nbb = nbc = nfar = 0
+    for a, b in (groups if groups is not None else self.groups()):
+        near, far = self.split(a, b, ratio)
+        ncell = int((self.kind[near] == 0).sum())
+        nbody = near.size - ncell
+        own = int(np.isin(near, self.leaf[a:b]).sum())
+        nbb += (b - a) * nbody - own
+        nbc += (b - a) * ncell
+        nfar += far.size
+    return nbb, nbc, nfar
+
+
+GroupWalk.split = _split
+GroupWalk.expanded_forces = _expanded_forces
+GroupWalk.expanded_counts = _expanded_counts

@@ -142,3 +142,40 @@ def test_group_walk_in_a_batch_matches_direct_sum():
         assert eg < ef
         one.close()
     bt.close()
+
+
+@pytest.mark.parametrize("ratio", [0.05, 0.1])
+def test_group_expansion_matches_its_restatement(dwarf_case, ratio):
+    """Far cells expanded about each group's centre (nbhip_set_group_expansion):
+    the near / far split is decided in FP64 by the same test as the numpy
+    restatement, so the counts agree exactly; the accelerations (near cells
+    per body in FP32, far cells as an FP32 second-order expansion) agree
+    with the restatement's FP64 evaluation of the same expansion to FP32
+    rounding."""
+    ctx, pot, pos, vel, m = dwarf_case
+    st = nb.NBodyState(ctx, None, pos, vel, m, precision="mixed_group", prestep=False)
+    st.set_group_expansion(ratio)
+    st.gravity()
+    gw = _model(st, ctx)
+    groups = gw.groups()
+    c = st.walk_counters5()
+    want = gw.expanded_counts(ratio, groups)
+    print(f"ratio {ratio}: gpu (bb, near cells, far pairs) {(c[0], c[1], c[4])} model {want}",
+          flush=True)
+    assert (c[0], c[1], c[4]) == want and c[4] > 0
+    acc = st.walk_accel()
+    rng = np.random.default_rng(3)
+    errs = []
+    for gi in rng.choice(len(groups), size=40, replace=False):
+        a, b = groups[gi]
+        errs.append(rel_err(acc[gw.order[a:b]], gw.expanded_forces(a, b, ratio), 0))
+    e = np.concatenate(errs)
+    print(f"FP32 vs FP64 expansion: median {np.median(e):.3e} p99 {np.percentile(e, 99):.3e}",
+          flush=True)
+    assert np.median(e) <= 1e-6 and np.percentile(e, 99) <= 1e-5
+    # the expansion's own error against the per-body lists (same accepted cells)
+    full = nb.NBodyState(ctx, None, pos, vel, m, precision="mixed_group").walk_accel()
+    d = rel_err(acc, full, 0)
+    print(f"expanded vs per-body lists: median {np.median(d):.3e} p99 {np.percentile(d, 99):.3e}",
+          flush=True)
+    st.close()
