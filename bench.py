@@ -28,7 +28,8 @@ METRIC = "body-steps/s at N=1M θ=0.5: mixed vs FP64 GPU at ≤1e-5 rel accel er
 PEAK = {"fp64": 78.6, "fp32": 157.3}     # TFLOP/s, MI355X published
 ERR_BOUND = 1e-5                         # median relative acceleration error of the metric
 # walk forms timed by default: 'precision[:open scale[:far-field ratio]]'
-ALL_FORMS = ["fp64", "mixed", "mixed_coherent", "mixed_group"]
+ALL_FORMS = ["fp64", "mixed", "mixed_coherent", "mixed_group", "mixed_coherent:1.6",
+             "mixed_group:1.3:0.05"]
 HBM_PEAK = 8000.0                        # GB/s
 
 
@@ -579,7 +580,7 @@ def main():
                      "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak,
                      "traffic": None if traffic is None else
-                     traffic.get("walk", {}).get(head, {}).get("bytes_per_launch"),
+                     traffic.get("walk", {}).get(parse_form(head)[0], {}).get("bytes_per_launch"),
                      "traffic_source": "profile constant: profiles/walk_traffic.json "
                                        "(rocprofv3 FETCH_SIZE + WRITE_SIZE passes, copy-"
                                        "calibrated), not measured in this run",

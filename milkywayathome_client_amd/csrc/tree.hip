@@ -1094,10 +1094,10 @@ __global__ __launch_bounds__(256) void k_level_scatter(const int* __restrict__ n
  * indices of its children, sorted into octant order and checked against the
  * Next links, then loaded together instead of one after another. */
 
-/* The group walk's table (precision MIXED_GROUP; T is null otherwise):
- * every cell writes its children's records into its child block
- * (slot nsim + kbase[cell] + rank, rank in octant order), and a root also
- * writes its own record into slot sim. */
+/* The group walk's table (precision MIXED_GROUP): after the summaries,
+ * every node's record goes to its parent's child block (slot nsim +
+ * kbase[parent] + rank, rank in octant (= pre-order) order among the
+ * siblings) and every root's to slot sim (k_group_table). */
 struct GroupTab {
     NodeG* T;
     const unsigned* nkid;
@@ -1218,14 +1218,6 @@ __device__ __forceinline__ bool nbh_summarize_cell(const SumView& v, int p)
             cz += w[j].m * w[j].z;
         }
     }
-    if (v.g.T) {
-        const int tb = v.g.nsim + kb;
-        #pragma unroll
-        for (int j = 0; j < 8; ++j)
-            if (j < nk)
-                v.g.T[tb + j] = nbh_group_record(w[j], ch[j], v.nquad, v.g, v.rc_scale,
-                                                 v.ctx.use_quad);
-    }
     const double mx = v.nmid[3 * p + 0], my = v.nmid[3 * p + 1], mz = v.nmid[3 * p + 2];
     if (mass > 0.0) {
         cx /= mass;
@@ -1283,8 +1275,6 @@ __device__ __forceinline__ bool nbh_summarize_cell(const SumView& v, int p)
     r.next = end;
     r.body = -1;
     node[p] = r;
-    if (v.g.T && level == 0)
-        v.g.T[sim] = nbh_group_record(r, p, v.nquad, v.g, v.rc_scale, v.ctx.use_quad);
     if (v.nodef) {
         /* the mixed walk's record: FP64 position, the rest in FP32 */
         NodeF& f = v.nodef[p];
@@ -1507,6 +1497,34 @@ __global__ __launch_bounds__(NBH_SPAN_THREADS) void k_sum_top(SumView v,
                                      fill, &nspan, scal);
 }
 
+/* The group walk's table from the finished summaries: one thread per node
+ * (no level order needed: every record is final). */
+__global__ __launch_bounds__(256) void k_group_table(const NodeW* __restrict__ node,
+                                                     const double* __restrict__ nquad,
+                                                     const int* __restrict__ nparent,
+                                                     const int* __restrict__ nsim,
+                                                     const int* __restrict__ kid, GroupTab g,
+                                                     double rc_scale, int use_quad, int node_cap,
+                                                     DevScalars* __restrict__ scal)
+{
+    if (scal->status & (NBHIP_CELL_OVERFLOW_ERROR | NBHIP_ERROR)) return;
+    const int nn = min(scal->nnode, node_cap);
+    for (int q = blockIdx.x * 256 + threadIdx.x; q < nn; q += gridDim.x * 256) {
+        const NodeW w = node[q];
+        const int par = nparent[q];
+        const NodeG r = nbh_group_record(w, q, nquad, g, rc_scale, use_quad != 0);
+        if (par < 0) {
+            g.T[nsim[q]] = r;
+            continue;
+        }
+        const int kb = (int) g.kbase[par];
+        const int nk = min((int) g.nkid[par], 8);
+        int rank = 0;
+        for (int j = 0; j < nk; ++j) rank += kid[kb + j] < q;
+        g.T[g.nsim + kb + rank] = r;
+    }
+}
+
 /* End of a KDK step (nbRunSystemPlain's loop, nbody/src/nbody_plain.c:
  * 409-448): counts the step and, when a simulation's tree or walk reported
  * a fatal status (a structure, depth or internal error, or incest without
@@ -1661,7 +1679,7 @@ void nbh_build(nbhip_state* st, int graph_safe)
     v.kid = st->kid;
     v.sims = st->sims;
     v.nodef = st->nodef;
-    v.g = GroupTab{ st->nodeg, st->nkid, st->kbase, st->nsim };
+    v.g = GroupTab{ nullptr, st->nkid, st->kbase, st->nsim };   /* table: k_group_table */
     v.ctx = st->ctx;
     v.rc_scale = st->open_scale;
     v.nn = st->node_cap;
@@ -1681,5 +1699,11 @@ void nbh_build(nbhip_state* st, int graph_safe)
                        st->span, span_n, span2, span2_n, st->scal);
     hipLaunchKernelGGL(k_sum_top, dim3(1), dim3(NBH_SPAN_THREADS), 0, s, v, span2, span2_n, nsup,
                        top_cand, top_list, st->scal);
+    if (st->nodeg) {
+        const GroupTab g{ st->nodeg, st->nkid, st->kbase, st->nsim };
+        hipLaunchKernelGGL(k_group_table, dim3(nbh_blocks(st->node_cap, 256 * 4)), dim3(256), 0,
+                           s, st->node, st->nquad, st->nparent, st->nsim_node, st->kid, g,
+                           st->open_scale, st->ctx.use_quad, st->node_cap, st->scal);
+    }
     nbh_mark(st, 4);
 }
