@@ -367,41 +367,51 @@ __device__ __forceinline__ bool nbh_words_gt(const unsigned long long* a,
     return a[2] > b[2];
 }
 
-__global__ __launch_bounds__(256) void k_tiefix(const unsigned long long* __restrict__ key,
-                                                unsigned* __restrict__ perm,
-                                                const double* __restrict__ px,
-                                                const double* __restrict__ py,
-                                                const double* __restrict__ pz,
-                                                const int* __restrict__ simid,
-                                                const SimTab* __restrict__ sims, int top,
-                                                const DevScalars* __restrict__ scal)
+__global__ __launch_bounds__(256) void k_tiefix_rank(const unsigned long long* __restrict__ key,
+                                                     const unsigned* __restrict__ perm,
+                                                     unsigned* __restrict__ perm_out,
+                                                     const double* __restrict__ px,
+                                                     const double* __restrict__ py,
+                                                     const double* __restrict__ pz,
+                                                     const int* __restrict__ simid,
+                                                     const SimTab* __restrict__ sims, int top,
+                                                     const DevScalars* __restrict__ scal)
 {
     const int i = blockIdx.x * 256 + threadIdx.x;
     const int M = scal->nmassive;
     if (i >= M) return;
     const unsigned long long k = key[i];
-    if (i > 0 && key[i - 1] == k) return;
-    if (i + 1 >= M || key[i + 1] != k) return;
+    const bool tie = (i > 0 && key[i - 1] == k) || (i + 1 < M && key[i + 1] == k);
+    if (!tie) return;
+    int a = i;
+    while (a > 0 && key[a - 1] == k) --a;
     int e = i + 1;
     while (e < M && key[e] == k) ++e;
     const double ks = sims[simid[perm[i]]].key_scale;
-    for (int a = i + 1; a < e; ++a) {
-        const unsigned v = perm[a];
-        unsigned long long wv[3];
-        nbh_deep_words(px, py, pz, ks, v, top, wv);
-        int b = a - 1;
-        while (b >= i) {
-            unsigned long long wb[3];
-            nbh_deep_words(px, py, pz, ks, perm[b], top, wb);
-            if (nbh_words_gt(wb, wv)) {
-                perm[b + 1] = perm[b];
-                --b;
-            } else {
-                break;
-            }
-        }
-        perm[b + 1] = v;
+    unsigned long long wi[3];
+    nbh_deep_words(px, py, pz, ks, perm[i], top, wi);
+    /* rank in the run: members with smaller deeper words, and members with
+     * equal words that come earlier (the stable order of the sort) */
+    int rank = 0;
+    for (int j = a; j < e; ++j) {
+        if (j == i) continue;
+        unsigned long long wj[3];
+        nbh_deep_words(px, py, pz, ks, perm[j], top, wj);
+        if (nbh_words_gt(wi, wj) || (j < i && !nbh_words_gt(wj, wi))) ++rank;
     }
+    perm_out[a + rank] = perm[i];
+}
+
+__global__ __launch_bounds__(256) void k_tiefix_copy(const unsigned long long* __restrict__ key,
+                                                     unsigned* __restrict__ perm,
+                                                     const unsigned* __restrict__ perm_out,
+                                                     const DevScalars* __restrict__ scal)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int M = scal->nmassive;
+    if (i >= M) return;
+    const unsigned long long k = key[i];
+    if ((i > 0 && key[i - 1] == k) || (i + 1 < M && key[i + 1] == k)) perm[i] = perm_out[i];
 }
 
 /* ---------------------------------------------------------------------- */
@@ -1627,8 +1637,11 @@ void nbh_build(nbhip_state* st, int graph_safe)
     }
     /* 8 passes: sorted data is back in st->key / st->perm */
     hipLaunchKernelGGL(k_count_massive, dim3(g256), dim3(256), 0, s, n, st->key, st->scal);
-    hipLaunchKernelGGL(k_tiefix, dim3(g256), dim3(256), 0, s, st->key, st->perm, st->px, st->py,
-                       st->pz, st->simid, st->sims, st->top_levels, st->scal);
+    /* perm_alt is free after the eighth pass */
+    hipLaunchKernelGGL(k_tiefix_rank, dim3(g256), dim3(256), 0, s, st->key, st->perm, st->perm_alt,
+                       st->px, st->py, st->pz, st->simid, st->sims, st->top_levels, st->scal);
+    hipLaunchKernelGGL(k_tiefix_copy, dim3(g256), dim3(256), 0, s, st->key, st->perm,
+                       st->perm_alt, st->scal);
     nbh_mark(st, 2);
 
     hipLaunchKernelGGL(k_gather, dim3(g256), dim3(256), 0, s, n, st->perm, st->pos4, st->key,
