@@ -29,7 +29,7 @@ PEAK = {"fp64": 78.6, "fp32": 157.3}     # TFLOP/s, MI355X published
 ERR_BOUND = 1e-5                         # median relative acceleration error of the metric
 # walk forms timed by default: 'precision[:open scale[:far-field ratio]]'
 ALL_FORMS = ["fp64", "mixed", "mixed_coherent", "mixed_group", "mixed_coherent:1.6",
-             "mixed_group:1.3:0.05"]
+             "mixed_group:1.3:0.08"]
 HBM_PEAK = 8000.0                        # GB/s
 
 

@@ -105,7 +105,8 @@ static void nbh_free_all(nbhip_state* st)
                      st->nlevel, st->nparent, st->nsim_node, st->level_list, st->nodef,
                      st->scal, st->sims, st->ignore, st->hist_buf, st->disp_buf, st->pos4,
                      st->exact_part, st->nodeg, st->nkid, st->kbase, st->best_buf,
-                     st->nrank, st->kid, st->blist, st->span, st->span_sorted };
+                     st->nrank, st->kid, st->blist, st->span, st->span_sorted, st->gcost,
+                     st->gorder, st->bcost };
     for (void* p : ptrs)
         if (p) (void) hipFree(p);
     if (st->graph_exec) (void) hipGraphExecDestroy(st->graph_exec);
@@ -213,7 +214,15 @@ static unsigned nbh_create(const nbhip_ctx* ctx, int device, int nsim, const int
     A(nsim_node, nc); A(level_list, nc);
     if (ctx->precision == NBHIP_PRECISION_MIXED || ctx->precision == NBHIP_PRECISION_MIXED_COHERENT)
         A(nodef, nc + 1);   /* + 1: walk prefetch */
-    if (ctx->precision == NBHIP_PRECISION_MIXED_GROUP) A(nodeg, nc + (size_t) nsim);
+    if (ctx->precision == NBHIP_PRECISION_MIXED_GROUP) {
+        const size_t ng = ((size_t) st->n + 63) / 64 + 1;
+        A(nodeg, nc + (size_t) nsim);
+        A(bcost, n ? n : 1);
+        A(gcost, ng);
+        A(gorder, ng);
+        /* the first walk has no costs yet: all zero, so every group in one bucket */
+        if (e == hipSuccess) e = hipMemset(st->bcost, 0, (n ? n : 1) * sizeof(unsigned));
+    }
     A(nkid, nc); A(kbase, nc); A(nrank, nc); A(kid, nc); A(blist, nc); A(span, nc);
     A(span_sorted, nc);
     if (ctx->criterion == NBHIP_EXACT) A(exact_part, 4 * (size_t) st->exact_seg * n);

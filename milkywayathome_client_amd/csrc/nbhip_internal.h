@@ -156,6 +156,10 @@ struct nbhip_state {
     int *level_list;              /* cell pre-order indices grouped by level */
     NodeF *nodef;                 /* mixed-precision walk records (precision MIXED) */
     NodeG *nodeg;                 /* group walk table (precision MIXED_GROUP): nsim + node_cap */
+    unsigned *bcost;              /* group walk: per body (original order), its group's work
+                                     in the last walk */
+    unsigned *gcost;              /* group walk: predicted work of each 64-body group */
+    int *gorder;                  /* group walk: dispatch order, costliest group first */
     unsigned *nkid;               /* children per node */
     unsigned *kbase;              /* exclusive scan of nkid: each cell's child block */
     int *nrank;                   /* a node's rank among its parent's children (arrival order) */

@@ -133,6 +133,8 @@ __global__ __launch_bounds__(64 * WPB) void k_walk_group(const NodeG* __restrict
                                                          double* __restrict__ vy,
                                                          double* __restrict__ vz,
                                                          DevScalars* __restrict__ scal,
+                                                         const int* __restrict__ gorder,
+                                                         unsigned* __restrict__ bcost,
                                                          WalkOut o)
 {
     __shared__ int s_fr[WPB][NBH_GW_FCAP];
@@ -150,7 +152,11 @@ __global__ __launch_bounds__(64 * WPB) void k_walk_group(const NodeG* __restrict
     float2* c2 = s_c2[w];
     float4* bl = s_b[w];
 
-    const int s = blockIdx.x * (64 * WPB) + threadIdx.x;
+    /* this wave's group: the costliest groups of the last walk go first */
+    const int ngroup = (o.n + 63) >> 6;
+    const int slot = blockIdx.x * WPB + w;
+    const int grp = slot < ngroup ? gorder[slot] : slot;
+    const int s = grp * 64 + lane;
     const bool valid = s < o.n;
     const unsigned long long valid_m = __builtin_amdgcn_ballot_w64(valid);
     if (valid_m == 0ull) return;
@@ -179,6 +185,7 @@ __global__ __launch_bounds__(64 * WPB) void k_walk_group(const NodeG* __restrict
 
     double ax = 0.0, ay = 0.0, az = 0.0, kx = 0.0, ky = 0.0, kz = 0.0;
     unsigned nbb = 0, nbc = 0, nop = 0, iters = 0, nfar = 0;
+    unsigned work = 0;                 /* dense-loop entries + 4 per traversal round */
     unsigned self_hits = 0;
     bool bad = false;
     unsigned long long remaining = valid_m;
@@ -264,6 +271,7 @@ __global__ __launch_bounds__(64 * WPB) void k_walk_group(const NodeG* __restrict
                 nbc += me ? (unsigned) cnt : 0u;
                 iters += (unsigned) cnt;
             }
+            work += (unsigned) cnt;
         };
         /* far cells: slots LC-1, LC-2, ...; one cell per lane */
         auto flush_far = [&](int cnt) {
@@ -367,6 +375,7 @@ __global__ __launch_bounds__(64 * WPB) void k_walk_group(const NodeG* __restrict
                 nbb += me ? (unsigned) cnt : 0u;
                 iters += (unsigned) cnt;
             }
+            work += (unsigned) cnt;
         };
 
         while (fsize > 0) {
@@ -381,6 +390,7 @@ __global__ __launch_bounds__(64 * WPB) void k_walk_group(const NodeG* __restrict
             const bool act = eidx < E && sub < cnt;
             fsize -= E;
             __builtin_amdgcn_wave_barrier();
+            work += 4u;
             const NodeG r = T[act ? (ent >> 3) + sub : 0];
             /* FP64 test against the sub-group's box */
             const double ddx = fmax(fmax(lox - r.x, r.x - hix), 0.0);
@@ -473,6 +483,7 @@ __global__ __launch_bounds__(64 * WPB) void k_walk_group(const NodeG* __restrict
     }
 
     if (bad && lane == 0) atomicOr(&scal->status, NBHIP_ERROR);
+
     if (COUNT) {
         /* the self interactions (d = 0) are not counted, as in the reference;
          * nop and iters are wave-uniform */
@@ -487,7 +498,56 @@ __global__ __launch_bounds__(64 * WPB) void k_walk_group(const NodeG* __restrict
         }
     }
     if (!valid) return;
-    nbh_walk_finish(o, p.x, p.y, p.z, ax, ay, az, perm[s], oax, oay, oaz, vx, vy, vz);
+    const unsigned j = perm[s];
+    bcost[j] = work;                   /* the body carries its group's cost to the next walk */
+    nbh_walk_finish(o, p.x, p.y, p.z, ax, ay, az, j, oax, oay, oaz, vx, vy, vz);
+}
+
+/* Predicted cost of each group: the largest last cost of its bodies (the
+ * costs follow the bodies, so a re-sorted order keeps its predictions). */
+__global__ __launch_bounds__(256) void k_group_cost(const unsigned* __restrict__ bcost,
+                                                    const unsigned* __restrict__ perm,
+                                                    unsigned* __restrict__ gcost, int n)
+{
+    const int s = blockIdx.x * 256 + threadIdx.x;
+    const unsigned c = s < n ? bcost[perm[s]] : 0u;
+    unsigned m = c;
+    for (int off = 32; off > 0; off >>= 1) m = max(m, (unsigned) __shfl_xor((int) m, off, 64));
+    if ((threadIdx.x & 63) == 0 && s < n) gcost[s >> 6] = m;
+}
+
+/* Dispatch order of the next group walk: groups by their predicted cost,
+ * costliest first (a counting sort over 256 logarithmic cost buckets, one
+ * workgroup; the order within a bucket is whatever the atomics give -- the
+ * results of the walk do not depend on the order, only its tail does). */
+#define NBH_ORDER_THREADS 1024
+__global__ __launch_bounds__(NBH_ORDER_THREADS) void k_group_order(const unsigned* __restrict__ gcost,
+                                                                  int* __restrict__ gorder,
+                                                                  int ngroup)
+{
+    __shared__ int cnt[256];
+    __shared__ int off[256];
+    const int tid = threadIdx.x;
+    if (tid < 256) cnt[tid] = 0;
+    __syncthreads();
+    auto bucket = [](unsigned c) {
+        const unsigned v = c + 1u;
+        const int e = 31 - __clz(v);                         /* floor(log2 v), 0..31 */
+        const int f = e >= 3 ? (int) ((v >> (e - 3)) & 7u) : (int) ((v << (3 - e)) & 7u);
+        return 255 - min(255, e * 8 + f);                    /* descending cost */
+    };
+    for (int g = tid; g < ngroup; g += NBH_ORDER_THREADS) atomicAdd(&cnt[bucket(gcost[g])], 1);
+    __syncthreads();
+    if (tid == 0) {
+        int acc = 0;
+        for (int b = 0; b < 256; ++b) {
+            off[b] = acc;
+            acc += cnt[b];
+        }
+    }
+    __syncthreads();
+    for (int g = tid; g < ngroup; g += NBH_ORDER_THREADS)
+        gorder[atomicAdd(&off[bucket(gcost[g])], 1)] = g;
 }
 
 static inline int nbh_gblocks(int n, int per) { return n > 0 ? (n + per - 1) / per : 1; }
@@ -499,13 +559,17 @@ void nbh_walk_group(nbhip_state* st, const WalkOut& o, double* oax, double* oay,
     int lc = (t.batch == 64 || t.batch == 128) ? t.batch : 256;
     if (bs == 256 && lc > 128) lc = 128;       /* LDS: at most 64 KiB per workgroup */
     dim3 grid(nbh_gblocks(st->n, bs)), block(bs);
+    hipLaunchKernelGGL(k_group_cost, dim3(nbh_gblocks(st->n, 256)), dim3(256), 0, st->stream,
+                       st->bcost, st->perm, st->gcost, st->n);
+    hipLaunchKernelGGL(k_group_order, dim3(1), dim3(NBH_ORDER_THREADS), 0, st->stream, st->gcost,
+                       st->gorder, (st->n + 63) / 64);
     const bool quad = st->ctx.use_quad != 0;
     const bool eps0 = !o.all_eps2_pos;
     const bool far = o.far2 > 0.0;
 #define NBH_GWF(W, L, Q, E, C, F)                                                             \
     hipLaunchKernelGGL((k_walk_group<W, L, Q, E, C, F>), grid, block, 0, st->stream,           \
                        st->nodeg, st->spos, st->perm, st->simid_s, st->dadj, st->sims, oax,    \
-                       oay, oaz, st->vx, st->vy, st->vz, st->scal, o)
+                       oay, oaz, st->vx, st->vy, st->vz, st->scal, st->gorder, st->bcost, o)
 #define NBH_GWL(W, L, Q, E, C)                                                                \
     do { if (far) NBH_GWF(W, L, Q, E, C, true); else NBH_GWF(W, L, Q, E, C, false); } while (0)
 #define NBH_GWC(W, L, Q, E)                                                                   \
