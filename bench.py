@@ -29,7 +29,7 @@ PEAK = {"fp64": 78.6, "fp32": 157.3}     # TFLOP/s, MI355X published
 ERR_BOUND = 1e-5                         # median relative acceleration error of the metric
 # walk forms timed by default: 'precision[:open scale[:far-field ratio]]'
 ALL_FORMS = ["fp64", "mixed", "mixed_coherent", "mixed_group", "mixed_coherent:1.6",
-             "mixed_group:1.3:0.08"]
+             "mixed_group:1.45:0.1"]
 HBM_PEAK = 8000.0                        # GB/s
 
 
@@ -445,6 +445,16 @@ def load_traffic():
         return json.load(f)
 
 
+def walk_traffic(traffic, form):
+    """HBM bytes per walk launch of a form from the profile constants: the
+    entry of the form itself, else of its precision."""
+    if traffic is None:
+        return None
+    walks = traffic.get("walk", {})
+    entry = walks.get(form) or walks.get(parse_form(form)[0]) or {}
+    return entry.get("bytes_per_launch")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -579,8 +589,7 @@ def main():
         "roofline": {"bound": "valu", "kernel": "k_walk (tree walk + potential + kick)",
                      "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak,
-                     "traffic": None if traffic is None else
-                     traffic.get("walk", {}).get(parse_form(head)[0], {}).get("bytes_per_launch"),
+                     "traffic": walk_traffic(traffic, head),
                      "traffic_source": "profile constant: profiles/walk_traffic.json "
                                        "(rocprofv3 FETCH_SIZE + WRITE_SIZE passes, copy-"
                                        "calibrated), not measured in this run",

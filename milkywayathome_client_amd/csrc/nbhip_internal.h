@@ -91,6 +91,7 @@ struct DevScalars {
     int level_count[64];           /* cells per level */
     int level_fill[64];
     int level_off[65];             /* exclusive scan of level_count */
+    unsigned group_mean;           /* group walk: mean predicted group cost of the last walk */
 };
 
 /* Walk tuning picked by the autotuner (nbhip_autotune). */
@@ -99,6 +100,14 @@ struct WalkTune {
     int batch;                     /* mixed walk: node block per FP32 partial sum (power of 2);
                                       group walk: LDS interaction-list depth (64, 128, 256) */
 };
+
+/* Group walk tasks: the sub-groups of a 64-body group predicted to cost
+ * more than NBH_GW_SPLIT_COST times the mean group are shared out among up
+ * to 8 waves (runs of whole sub-groups, each run at most that cost where a
+ * sub-group allows); at most NBH_GW_EXTRA_TASKS such extra waves per walk. */
+#define NBH_GW_EXTRA_TASKS 1024
+#define NBH_GW_SPLIT_COST 2
+__host__ __device__ inline int nbh_group_tasks(int n) { return (n + 63) / 64 + NBH_GW_EXTRA_TASKS; }
 
 #define NBH_SORT_TILE 2048   /* keys per radix-sort block: 256 threads x 8 */
 #define NBH_SCAN_TILE 4096   /* items per scan block: 1024 threads x 4 */
@@ -156,10 +165,13 @@ struct nbhip_state {
     int *level_list;              /* cell pre-order indices grouped by level */
     NodeF *nodef;                 /* mixed-precision walk records (precision MIXED) */
     NodeG *nodeg;                 /* group walk table (precision MIXED_GROUP): nsim + node_cap */
-    unsigned *bcost;              /* group walk: per body (original order), its group's work
+    unsigned *bcost;              /* group walk: per body (original order), its sub-group's work
                                      in the last walk */
-    unsigned *gcost;              /* group walk: predicted work of each 64-body group */
-    int *gorder;                  /* group walk: dispatch order, costliest group first */
+    unsigned *gcost;              /* group walk: predicted work of each 64-body group, and
+                                     its task count - 1 in bits 28..30 */
+    uint2 *gtask;                 /* group walk: up to 8 tasks per group: (cost, lo | hi << 8) */
+    int2 *gorder;                 /* group walk: [0].x = task count, then the tasks (group,
+                                     lo | hi << 8) in dispatch order, costliest first */
     unsigned *nkid;               /* children per node */
     unsigned *kbase;              /* exclusive scan of nkid: each cell's child block */
     int *nrank;                   /* a node's rank among its parent's children (arrival order) */

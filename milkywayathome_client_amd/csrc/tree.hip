@@ -1177,22 +1177,25 @@ struct SumView {
     int nn;
 };
 
-/* One cell's summary; false on a child list that disagrees with the Next
- * links (a corrupt table). */
-__device__ __forceinline__ bool nbh_summarize_cell(const SumView& v, int p)
+#define NBH_NO_KID 0x7fffffff   /* an unused slot of a sorted child list */
+
+/* The structure half of a cell's summary: its children, sorted into
+ * pre-order (= octant order), into ch[8] (unused slots NBH_NO_KID), checked
+ * against the Next links; false on a child list that disagrees with them (a
+ * corrupt table).  Reads nothing that a summary writes except Next, which a
+ * summary rewrites with the same value, so it may run before the children
+ * are summarised. */
+__device__ __forceinline__ bool nbh_cell_children(const SumView& v, int p, int* __restrict__ out)
 {
-    NodeW* __restrict__ node = v.node;
+    const NodeW* __restrict__ node = v.node;
     const int level = v.nlevel[p];
-    const int sim = v.nsim[p];
-    const double rsize = v.sims[sim].rsize;
     const int end = node[p].next;
     const int nk = (int) v.g.nkid[p];
     if (level < 0 || end <= p + 1 || end > v.nn || nk < 1 || nk > 8) return false;
     const int kb = (int) v.g.kbase[p];
-    /* the children, sorted into pre-order (= octant order) */
     int ch[8];
     #pragma unroll
-    for (int j = 0; j < 8; ++j) ch[j] = j < nk ? v.kid[kb + j] : 0x7fffffff;
+    for (int j = 0; j < 8; ++j) ch[j] = j < nk ? v.kid[kb + j] : NBH_NO_KID;
     /* Batcher's odd-even merge network for 8 keys (19 compare-exchanges,
      * static indices: the list stays in registers) */
 #define NBH_CE(a, b)                                                    \
@@ -1208,15 +1211,39 @@ __device__ __forceinline__ bool nbh_summarize_cell(const SumView& v, int p)
     NBH_CE(2, 4); NBH_CE(3, 5);
     NBH_CE(1, 2); NBH_CE(3, 4); NBH_CE(5, 6);
 #undef NBH_CE
-    NodeW w[8];
+    int nx[8];
     #pragma unroll
-    for (int j = 0; j < 8; ++j)
-        if (j < nk) w[j] = node[ch[j]];
+    for (int j = 0; j < 8; ++j) nx[j] = j < nk ? node[ch[j]].next : 0;
     /* the list must be the Next chain p+1 -> ... -> end */
     if (ch[0] != p + 1) return false;
     #pragma unroll
     for (int j = 0; j < 8; ++j)
-        if (j < nk && w[j].next != (j + 1 < nk ? ch[j + 1] : end)) return false;
+        if (j < nk && nx[j] != (j + 1 < nk ? ch[j + 1] : end)) return false;
+    #pragma unroll
+    for (int j = 0; j < 8; ++j) out[j] = ch[j];
+    return true;
+}
+
+/* The value half: the cell's record (centre of mass, mass, critical
+ * radius, quadrupole, and the mixed walk's record) from its children's
+ * finished records; ch is the cell's checked, sorted child list. */
+__device__ __forceinline__ void nbh_cell_summary(const SumView& v, int p, const int* __restrict__ chl)
+{
+    NodeW* __restrict__ node = v.node;
+    int ch[8];
+    #pragma unroll
+    for (int j = 0; j < 8; ++j) ch[j] = chl[j];
+    const int level = v.nlevel[p];
+    const int sim = v.nsim[p];
+    const double rsize = v.sims[sim].rsize;
+    const int end = node[p].next;
+    int nk = 0;
+    #pragma unroll
+    for (int j = 0; j < 8; ++j) nk += ch[j] != NBH_NO_KID;
+    NodeW w[8];
+    #pragma unroll
+    for (int j = 0; j < 8; ++j)
+        if (j < nk) w[j] = node[ch[j]];
 
     double mass = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
     #pragma unroll
@@ -1297,6 +1324,15 @@ __device__ __forceinline__ bool nbh_summarize_cell(const SumView& v, int p)
         f.next = end;
         f.body = -1;
     }
+}
+
+/* One cell's summary; false on a child list that disagrees with the Next
+ * links (a corrupt table). */
+__device__ __forceinline__ bool nbh_summarize_cell(const SumView& v, int p)
+{
+    int ch[8];
+    if (!nbh_cell_children(v, p, ch)) return false;
+    nbh_cell_summary(v, p, ch);
     return true;
 }
 
@@ -1305,6 +1341,54 @@ __device__ __forceinline__ bool nbh_summarize_cell(const SumView& v, int p)
 #define NBH_SUM_GROUP 64       /* blocks per k_sum_super workgroup */
 #define NBH_SPAN_SLOTS 64      /* spanning cells one block can hand up (one per level) */
 #define NBH_SPAN_THREADS 512
+#define NBH_SUM_CAP 1024       /* cells whose child lists a workgroup keeps in LDS */
+
+/* Summaries of the cells list[0 .. off[64]), grouped by level (level l at
+ * list[off[l] .. off[l+1])), deepest level first, one barrier per level.
+ * Up to NBH_SUM_CAP cells, every cell's child list is read, sorted and
+ * checked first, all levels at once, and kept in LDS (s_ch: two int4 per
+ * cell), so that a level then waits on one round of loads -- its children's
+ * records -- instead of the chain nlevel/kbase -> kid -> children.  More
+ * cells: each level does both halves.  False on a corrupt child list. */
+template <int NT>
+__device__ bool nbh_sum_levels(const SumView& v, const int* list, const int* off, int4* s_ch)
+{
+    const int tid = threadIdx.x;
+    const int total = off[64];
+    bool ok = true;
+    if (total <= NBH_SUM_CAP) {
+        for (int i = tid; i < total; i += NT) {
+            int ch[8];
+            if (nbh_cell_children(v, list[i], ch)) {
+                s_ch[2 * i] = make_int4(ch[0], ch[1], ch[2], ch[3]);
+                s_ch[2 * i + 1] = make_int4(ch[4], ch[5], ch[6], ch[7]);
+            } else {
+                ok = false;
+                s_ch[2 * i] = make_int4(-1, -1, -1, -1);
+            }
+        }
+        __syncthreads();
+        for (int l = 63; l >= 0; --l) {
+            if (off[l + 1] == off[l]) continue;           /* uniform */
+            for (int i = off[l] + tid; i < off[l + 1]; i += NT) {
+                const int4 a = s_ch[2 * i];
+                if (a.x < 0) continue;
+                const int4 b = s_ch[2 * i + 1];
+                const int ch[8] = { a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w };
+                nbh_cell_summary(v, list[i], ch);
+            }
+            __syncthreads();
+        }
+    } else {
+        for (int l = 63; l >= 0; --l) {
+            if (off[l + 1] == off[l]) continue;           /* uniform */
+            for (int i = off[l] + tid; i < off[l + 1]; i += NT)
+                ok = nbh_summarize_cell(v, list[i]) && ok;
+            __syncthreads();
+        }
+    }
+    return ok;
+}
 
 /* Levels holding cells in [lo, hi): counts into cnt[64] (LDS), then an
  * exclusive scan into off[65]; callers add their own cells. */
@@ -1319,6 +1403,8 @@ __global__ __launch_bounds__(NBH_SUM_THREADS) void k_sum_blocks(SumView v,
     __shared__ int off[65];
     __shared__ int fill[64];
     __shared__ int nspan;
+    __shared__ int s_list[NBH_SUM_CAP];
+    __shared__ int4 s_ch[2 * NBH_SUM_CAP];
     const int tid = threadIdx.x;
     if (scal->status & (NBHIP_CELL_OVERFLOW_ERROR | NBHIP_ERROR)) return;
     const int M = scal->nmassive;
@@ -1362,23 +1448,17 @@ __global__ __launch_bounds__(NBH_SUM_THREADS) void k_sum_blocks(SumView v,
         off[64] = acc;
     }
     __syncthreads();
-    /* the block's inside cells grouped by level, stored in its own slice
-     * [P0, P0 + count) of blist (slices of different blocks are disjoint) */
+    /* the block's inside cells grouped by level: in LDS, or, past
+     * NBH_SUM_CAP, in the block's own slice [P0, P0 + count) of blist
+     * (slices of different blocks are disjoint) */
+    int* lst = off[64] <= NBH_SUM_CAP ? s_list : blist + P0;
     for (int q = P0 + tid; q < P1; q += NBH_SUM_THREADS) {
         const int l = v.nlevel[q];
         if (l < 0 || l > 63 || v.node[q].next > P1) continue;
-        blist[P0 + off[l] + atomicAdd(&fill[l], 1)] = q;
+        lst[off[l] + atomicAdd(&fill[l], 1)] = q;
     }
     __syncthreads();
-    bool ok = true;
-    for (int l = 63; l >= 0; --l) {
-        const int c = off[l + 1] - off[l];
-        if (c == 0) continue;                     /* uniform */
-        for (int i = tid; i < c; i += NBH_SUM_THREADS)
-            ok = nbh_summarize_cell(v, blist[P0 + off[l] + i]) && ok;
-        __syncthreads();
-    }
-    if (!ok) atomicOr(&scal->status, NBHIP_ERROR);
+    if (!nbh_sum_levels<NBH_SUM_THREADS>(v, lst, off, s_ch)) atomicOr(&scal->status, NBHIP_ERROR);
 }
 
 /* Cells handed up by several lower workgroups (cand[0..nc) in LDS or
@@ -1388,7 +1468,7 @@ __global__ __launch_bounds__(NBH_SUM_THREADS) void k_sum_blocks(SumView v,
 template <int NT>
 __device__ void nbh_sum_handed(const SumView& v, const int* cand, int nc, int P1, int* out,
                                int* out_n, int* list, int* cnt, int* off, int* fill, int* nspan,
-                               DevScalars* scal)
+                               int4* s_ch, DevScalars* scal)
 {
     const int tid = threadIdx.x;
     if (tid < 64) {
@@ -1430,14 +1510,7 @@ __device__ void nbh_sum_handed(const SumView& v, const int* cand, int nc, int P1
         list[off[l] + atomicAdd(&fill[l], 1)] = q;
     }
     __syncthreads();
-    bool ok = true;
-    for (int l = 63; l >= 0; --l) {
-        const int c = off[l + 1] - off[l];
-        if (c == 0) continue;                     /* uniform */
-        for (int i = tid; i < c; i += NT) ok = nbh_summarize_cell(v, list[off[l] + i]) && ok;
-        __syncthreads();
-    }
-    if (!ok) atomicOr(&scal->status, NBHIP_ERROR);
+    if (!nbh_sum_levels<NT>(v, list, off, s_ch)) atomicOr(&scal->status, NBHIP_ERROR);
 }
 
 /* Workgroup s: the spanning cells of blocks [s*NBH_SUM_GROUP, ...). */
@@ -1456,6 +1529,7 @@ __global__ __launch_bounds__(NBH_SUM_THREADS) void k_sum_super(SumView v,
     __shared__ int fill[64];
     __shared__ int nspan;
     __shared__ int ncand;
+    __shared__ int4 s_ch[2 * NBH_SUM_CAP];
     const int tid = threadIdx.x;
     if (scal->status & (NBHIP_CELL_OVERFLOW_ERROR | NBHIP_ERROR)) return;
     const int M = scal->nmassive;
@@ -1474,7 +1548,8 @@ __global__ __launch_bounds__(NBH_SUM_THREADS) void k_sum_super(SumView v,
     }
     __syncthreads();
     nbh_sum_handed<NBH_SUM_THREADS>(v, cand, ncand, P1, span2 + (size_t) blockIdx.x * NBH_SPAN_SLOTS,
-                                    span2_n + blockIdx.x, list, cnt, off, fill, &nspan, scal);
+                                    span2_n + blockIdx.x, list, cnt, off, fill, &nspan, s_ch,
+                                    scal);
 }
 
 /* One workgroup: every super-block's spanning cells. */
@@ -1490,6 +1565,7 @@ __global__ __launch_bounds__(NBH_SPAN_THREADS) void k_sum_top(SumView v,
     __shared__ int fill[64];
     __shared__ int nspan;
     __shared__ int ncand;
+    __shared__ int4 s_ch[2 * NBH_SUM_CAP];
     const int tid = threadIdx.x;
     if (scal->status & (NBHIP_CELL_OVERFLOW_ERROR | NBHIP_ERROR)) return;
     const int M = scal->nmassive;
@@ -1504,7 +1580,7 @@ __global__ __launch_bounds__(NBH_SPAN_THREADS) void k_sum_top(SumView v,
     }
     __syncthreads();
     nbh_sum_handed<NBH_SPAN_THREADS>(v, cand, ncand, v.nn, nullptr, nullptr, list, cnt, off,
-                                     fill, &nspan, scal);
+                                     fill, &nspan, s_ch, scal);
 }
 
 /* The group walk's table from the finished summaries: one thread per node

@@ -2,10 +2,14 @@
  * LDS-staged group walk (precision NBHIP_PRECISION_MIXED_GROUP) on gfx950.
  *
  * A wavefront owns 64 consecutive bodies in Morton order.  Where the order
- * jumps between distant cells (the weakest common-prefix link of the wave
- * lies three or more levels above the wave's median link; applied twice, so
- * at most four parts) and where a batch changes simulation, the wave is cut
- * into sub-groups, walked one after another.  For a sub-group:
+ * jumps between distant cells (the weakest common-prefix link of a run lies
+ * more than two levels above the run's median link, or at level 4 or nearer
+ * the root: bodies that share no cell smaller than a sixteenth of the root;
+ * applied three times, so at most eight parts per run) and where a batch changes simulation, the wave is cut into
+ * sub-groups, walked one after another.  (A group spanning two sibling
+ * cells has a box twice theirs and walks several times their lists: at N =
+ * 10^6 such groups, cut at three levels only, made single waves as long as
+ * the whole walk.)  For a sub-group:
  *
  *  1. its bounding box is reduced across the member lanes (FP64);
  *  2. the tree is walked breadth-first 64 nodes at a time: the frontier is a
@@ -43,7 +47,15 @@
 #include "nbhip_walk.h"
 
 #define NBH_GW_FCAP 512    /* range-stack entries per wavefront */
-#define NBH_GW_SPLIT_T 3   /* split when min link level < median link level - 3 */
+#ifndef NBH_GW_SPLIT_T
+#define NBH_GW_SPLIT_T 2   /* split when min link level < median link level - 2 */
+#endif
+#ifndef NBH_GW_SPLIT_ABS
+#define NBH_GW_SPLIT_ABS 4 /* and always when min link level <= 4 (cells of rsize / 16) */
+#endif
+#ifndef NBH_GW_SPLIT_PASSES
+#define NBH_GW_SPLIT_PASSES 3   /* splitting passes: at most 2^passes sub-groups */
+#endif
 
 __device__ __forceinline__ double nbh_uniform_d(double v)
 {
@@ -80,13 +92,15 @@ __device__ __forceinline__ unsigned nbh_mbcnt(unsigned long long m)
  * level of the common prefix of this lane's body and the next lane's (-1
  * where the next lane is in another simulation, massless, invalid or past
  * the wave).  Runs between hard cuts are split at their weakest link (first
- * minimum) when that link's level lies NBH_GW_SPLIT_T or more levels below
- * the lower median of the run's link levels; twice. */
+ * minimum) when that link's level is more than NBH_GW_SPLIT_T below (nearer
+ * the root than) the lower median of the run's link levels, or is at most
+ * NBH_GW_SPLIT_ABS (a sparse run: a few bodies far apart, whose box would
+ * open most of the tree); NBH_GW_SPLIT_PASSES times. */
 __device__ __forceinline__ unsigned long long nbh_group_cuts(unsigned long long hard, int lvl,
                                                              int lane)
 {
     unsigned long long cuts = hard;
-    for (int pass = 0; pass < 2; ++pass) {
+    for (int pass = 0; pass < NBH_GW_SPLIT_PASSES; ++pass) {
         unsigned long long next = cuts;
         int a = 0;
         unsigned long long rest = cuts;
@@ -94,7 +108,7 @@ __device__ __forceinline__ unsigned long long nbh_group_cuts(unsigned long long 
             const int e = __builtin_ctzll(rest);      /* run [a, e] */
             rest &= rest - 1;
             const int nl = e - a;                      /* links a .. e-1 */
-            if (nl >= 2) {
+            if (nl >= 1) {
                 const bool in = lane >= a && lane < e;
                 const bool ok = __builtin_amdgcn_ballot_w64(in && lvl >= 0) ==
                                 (((~0ull) >> (64 - nl)) << a);
@@ -109,7 +123,7 @@ __device__ __forceinline__ unsigned long long nbh_group_cuts(unsigned long long 
                         const int c = __popcll(__builtin_amdgcn_ballot_w64(in && lvl <= mid));
                         if (c >= need) hi = mid; else lo = mid + 1;
                     }
-                    if (lmin < lo - NBH_GW_SPLIT_T) next |= 1ull << j;
+                    if (lmin < lo - NBH_GW_SPLIT_T || lmin <= NBH_GW_SPLIT_ABS) next |= 1ull << j;
                 }
             }
             a = e + 1;
@@ -117,6 +131,33 @@ __device__ __forceinline__ unsigned long long nbh_group_cuts(unsigned long long 
         cuts = next;
     }
     return cuts;
+}
+
+/* The sub-group cuts of the wave that holds sorted body s: hard cuts where
+ * the next lane is invalid, in another simulation or of the other kind
+ * (massive / massless), then nbh_group_cuts on the common-prefix levels. */
+__device__ __forceinline__ unsigned long long nbh_wave_cuts(int s, bool valid, bool massive,
+                                                            int sim, unsigned long long valid_m,
+                                                            unsigned long long massive_m,
+                                                            const int* __restrict__ dadj, int sb,
+                                                            int lane)
+{
+    const int sim_next = __shfl_down(sim, 1, 64);
+    const bool massive_next = lane < 63 && ((massive_m >> (lane + 1)) & 1ull);
+    const bool valid_next = lane < 63 && ((valid_m >> (lane + 1)) & 1ull);
+    const bool hard = valid && (!valid_next || sim_next != sim || massive_next != massive);
+    int lvl = -1;
+    if (massive && massive_next && !hard) lvl = nbh_levels(dadj[s], sb);
+    return nbh_group_cuts(__builtin_amdgcn_ballot_w64(hard) | (1ull << 63), lvl, lane);
+}
+
+/* Members of the sub-group that starts at the lowest lane of `remaining`. */
+__device__ __forceinline__ unsigned long long nbh_next_subgroup(unsigned long long remaining,
+                                                                unsigned long long cuts)
+{
+    const int lead = __builtin_ctzll(remaining);
+    const int last = __builtin_ctzll(cuts & (~0ull << lead));
+    return (last == 63 ? ~0ull : ((1ull << (last + 1)) - 1ull)) & (~0ull << lead) & remaining;
 }
 
 template <int WPB, int LC, bool QUAD, bool EPS0, bool COUNT, bool FAR>
@@ -133,7 +174,7 @@ __global__ __launch_bounds__(64 * WPB) void k_walk_group(const NodeG* __restrict
                                                          double* __restrict__ vy,
                                                          double* __restrict__ vz,
                                                          DevScalars* __restrict__ scal,
-                                                         const int* __restrict__ gorder,
+                                                         const int2* __restrict__ gorder,
                                                          unsigned* __restrict__ bcost,
                                                          WalkOut o)
 {
@@ -152,14 +193,17 @@ __global__ __launch_bounds__(64 * WPB) void k_walk_group(const NodeG* __restrict
     float2* c2 = s_c2[w];
     float4* bl = s_b[w];
 
-    /* this wave's group: the costliest groups of the last walk go first */
-    const int ngroup = (o.n + 63) >> 6;
+    /* this wave's task, costliest first: lanes [lo, hi) of a group, whole
+     * sub-groups (the group itself, or a share of a costly one).  The cuts
+     * are those of the whole group, so a body's list does not depend on how
+     * its group was shared out. */
     const int slot = blockIdx.x * WPB + w;
-    const int grp = slot < ngroup ? gorder[slot] : slot;
-    const int s = grp * 64 + lane;
-    const bool valid = s < o.n;
-    const unsigned long long valid_m = __builtin_amdgcn_ballot_w64(valid);
-    if (valid_m == 0ull) return;
+    if (slot >= gorder[0].x) return;
+    const int2 task = gorder[1 + slot];
+    const int lo = task.y & 255, hi = task.y >> 8;
+    const int s = task.x * 64 + lane;
+    const bool in_group = s < o.n;
+    const unsigned long long group_m = __builtin_amdgcn_ballot_w64(in_group);
     const int M = scal->nmassive;
     const int sb = scal->sim_bits;
     /* A depth-first descent pushes at most 7 entries more than it pops per
@@ -168,33 +212,28 @@ __global__ __launch_bounds__(64 * WPB) void k_walk_group(const NodeG* __restrict
      * always proceed without overflowing. */
     const int slack = 7 * (min(scal->maxdepth, 63) + 2);
     const int wave_base = s - lane;
-    const double4 p = valid ? spos[s] : make_double4(0.0, 0.0, 0.0, 0.0);
-    const int sim = valid ? simid_s[s] : -1;
-    const bool massive = valid && s < M;
-    const unsigned long long massive_m = __builtin_amdgcn_ballot_w64(massive);
-
-    /* hard cuts: next lane invalid, in another simulation or of the other kind */
-    const int sim_next = __shfl_down(sim, 1, 64);
-    const bool massive_next = ((massive_m >> (lane + 1)) & 1ull) && lane < 63;
-    const bool valid_next = ((valid_m >> (lane + 1)) & 1ull) && lane < 63;
-    const bool hard = valid && (!valid_next || sim_next != sim || massive_next != massive);
-    int lvl = -1;
-    if (massive && massive_next && !hard) lvl = nbh_levels(dadj[s], sb);
+    const double4 p = in_group ? spos[s] : make_double4(0.0, 0.0, 0.0, 0.0);
+    const int sim = in_group ? simid_s[s] : -1;
+    const bool gmassive = in_group && s < M;
+    const unsigned long long massive_m = __builtin_amdgcn_ballot_w64(gmassive);
     const unsigned long long cuts =
-        nbh_group_cuts(__builtin_amdgcn_ballot_w64(hard) | (1ull << 63), lvl, lane);
+        nbh_wave_cuts(s, in_group, gmassive, sim, group_m, massive_m, dadj, sb, lane);
+    const bool valid = in_group && lane >= lo && lane < hi;
+    const unsigned long long valid_m = __builtin_amdgcn_ballot_w64(valid);
+    if (valid_m == 0ull) return;
 
     double ax = 0.0, ay = 0.0, az = 0.0, kx = 0.0, ky = 0.0, kz = 0.0;
     unsigned nbb = 0, nbc = 0, nop = 0, iters = 0, nfar = 0;
     unsigned work = 0;                 /* dense-loop entries + 4 per traversal round */
+    unsigned my_work = 0;              /* the work of this lane's sub-group */
     unsigned self_hits = 0;
     bool bad = false;
     unsigned long long remaining = valid_m;
     while (remaining) {
         const int lead = __builtin_ctzll(remaining);
-        const int last = __builtin_ctzll(cuts & (~0ull << lead));
-        const unsigned long long mem = (last == 63 ? ~0ull : ((1ull << (last + 1)) - 1ull)) &
-                                       (~0ull << lead) & remaining;
+        const unsigned long long mem = nbh_next_subgroup(remaining, cuts);
         remaining &= ~mem;
+        const unsigned work0 = work;
         const bool me = (mem >> lane) & 1ull;
         const int sim_u = __builtin_amdgcn_readlane(sim, lead);
         const int root = sims[sim_u].root;
@@ -479,6 +518,7 @@ __global__ __launch_bounds__(64 * WPB) void k_walk_group(const NodeG* __restrict
         }
         if (COUNT) self_hits += (unsigned) own_seen;
         if (own_seen < expected && lane == 0) atomicOr(&sims[sim_u].incest, 1u);
+        if (me) my_work = work - work0;
         if (bad) break;
     }
 
@@ -499,55 +539,136 @@ __global__ __launch_bounds__(64 * WPB) void k_walk_group(const NodeG* __restrict
     }
     if (!valid) return;
     const unsigned j = perm[s];
-    bcost[j] = work;                   /* the body carries its group's cost to the next walk */
+    bcost[j] = my_work;                /* the body carries its sub-group's cost to the next walk */
     nbh_walk_finish(o, p.x, p.y, p.z, ax, ay, az, j, oax, oay, oaz, vx, vy, vz);
 }
 
-/* Predicted cost of each group: the largest last cost of its bodies (the
- * costs follow the bodies, so a re-sorted order keeps its predictions). */
+/* Predicted cost of each group (wave) and its tasks.  A sub-group (cut as
+ * the walk cuts it) is priced at the largest last cost among its bodies --
+ * the costs follow the bodies, and a wave cut into parts walks each part --
+ * and a group at the sum over its sub-groups.  A group above
+ * NBH_GW_SPLIT_COST times the last walk's mean group is shared out: runs of
+ * whole sub-groups, a run closed before it would pass that cost, at most 8.
+ * gcost[g] = min(cost, 2^28 - 1) | (tasks - 1) << 28; gtask[8 g + k] =
+ * (cost, lo | hi << 8) of task k (lanes [lo, hi)). */
 __global__ __launch_bounds__(256) void k_group_cost(const unsigned* __restrict__ bcost,
                                                     const unsigned* __restrict__ perm,
-                                                    unsigned* __restrict__ gcost, int n)
+                                                    const int* __restrict__ simid_s,
+                                                    const int* __restrict__ dadj,
+                                                    const DevScalars* __restrict__ scal,
+                                                    unsigned* __restrict__ gcost,
+                                                    uint2* __restrict__ gtask, int n)
 {
     const int s = blockIdx.x * 256 + threadIdx.x;
-    const unsigned c = s < n ? bcost[perm[s]] : 0u;
-    unsigned m = c;
-    for (int off = 32; off > 0; off >>= 1) m = max(m, (unsigned) __shfl_xor((int) m, off, 64));
-    if ((threadIdx.x & 63) == 0 && s < n) gcost[s >> 6] = m;
+    const int lane = threadIdx.x & 63;
+    const int g = s >> 6;
+    const bool valid = s < n;
+    const unsigned long long valid_m = __builtin_amdgcn_ballot_w64(valid);
+    if (valid_m == 0ull) return;
+    const int M = scal->nmassive;
+    const int sim = valid ? simid_s[s] : -1;
+    const bool massive = valid && s < M;
+    const unsigned long long massive_m = __builtin_amdgcn_ballot_w64(massive);
+    const unsigned long long cuts =
+        nbh_wave_cuts(s, valid, massive, sim, valid_m, massive_m, dadj, scal->sim_bits, lane);
+    const unsigned c = valid ? bcost[perm[s]] : 0u;
+    const double limit = NBH_GW_SPLIT_COST * (double) scal->group_mean;
+    unsigned total = 0, tcost = 0;
+    int ntask = 0, tlo = 0;
+    unsigned long long remaining = valid_m;
+    while (remaining) {
+        const unsigned long long mem = nbh_next_subgroup(remaining, cuts);
+        remaining &= ~mem;
+        unsigned m = ((mem >> lane) & 1ull) ? c : 0u;
+        for (int off = 32; off > 0; off >>= 1) m = max(m, (unsigned) __shfl_xor((int) m, off, 64));
+        const int a = __builtin_ctzll(mem);
+        /* wave-uniform from here */
+        if (limit > 0.0 && tcost > 0u && (double) tcost + (double) m > limit && ntask < 7) {
+            if (lane == 0) gtask[8 * (size_t) g + ntask] = make_uint2(tcost, (unsigned) (tlo | (a << 8)));
+            ++ntask;
+            tlo = a;
+            tcost = 0u;
+        }
+        tcost += m;
+        total += m;
+    }
+    if (lane == 0) {
+        gtask[8 * (size_t) g + ntask] = make_uint2(tcost, (unsigned) (tlo | (64 << 8)));
+        gcost[g] = min(total, (1u << 28) - 1u) | ((unsigned) ntask << 28);
+    }
 }
 
-/* Dispatch order of the next group walk: groups by their predicted cost,
- * costliest first (a counting sort over 256 logarithmic cost buckets, one
- * workgroup; the order within a bucket is whatever the atomics give -- the
- * results of the walk do not depend on the order, only its tail does). */
+/* Tasks of the next group walk in dispatch order, costliest first (a
+ * counting sort over 256 logarithmic cost buckets, one workgroup; the order
+ * within a bucket is whatever the atomics give -- the results of the walk
+ * depend neither on the order nor on the sharing, only its tail does).
+ * gorder[0].x = the task count; entries (group, lo | hi << 8).  Also keeps
+ * the mean group cost for the next walk's sharing. */
 #define NBH_ORDER_THREADS 1024
-__global__ __launch_bounds__(NBH_ORDER_THREADS) void k_group_order(const unsigned* __restrict__ gcost,
-                                                                  int* __restrict__ gorder,
+__global__ __launch_bounds__(NBH_ORDER_THREADS) void k_group_order(unsigned* __restrict__ gcost,
+                                                                  const uint2* __restrict__ gtask,
+                                                                  int2* __restrict__ gorder,
+                                                                  DevScalars* __restrict__ scal,
                                                                   int ngroup)
 {
     __shared__ int cnt[256];
     __shared__ int off[256];
+    __shared__ unsigned long long wsum[NBH_ORDER_THREADS / 64];
+    __shared__ int extra;
     const int tid = threadIdx.x;
     if (tid < 256) cnt[tid] = 0;
-    __syncthreads();
+    if (tid == 0) extra = 0;
     auto bucket = [](unsigned c) {
         const unsigned v = c + 1u;
         const int e = 31 - __clz(v);                         /* floor(log2 v), 0..31 */
         const int f = e >= 3 ? (int) ((v >> (e - 3)) & 7u) : (int) ((v << (3 - e)) & 7u);
         return 255 - min(255, e * 8 + f);                    /* descending cost */
     };
-    for (int g = tid; g < ngroup; g += NBH_ORDER_THREADS) atomicAdd(&cnt[bucket(gcost[g])], 1);
+    const unsigned cmask = (1u << 28) - 1u;
+    unsigned long long sum = 0;
+    for (int g = tid; g < ngroup; g += NBH_ORDER_THREADS) sum += gcost[g] & cmask;
+    for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o, 64);
+    if ((tid & 63) == 0) wsum[tid >> 6] = sum;
+    __syncthreads();
+    /* every task entry a group will have; shares past the extra-task budget
+     * fall back to the whole group */
+    for (int g = tid; g < ngroup; g += NBH_ORDER_THREADS) {
+        const unsigned e = gcost[g];
+        int k = (int) (e >> 28) + 1;
+        if (k > 1 && atomicAdd(&extra, k - 1) + k - 1 > NBH_GW_EXTRA_TASKS) {
+            k = 1;
+            gcost[g] = e & cmask;
+        }
+        if (k == 1) atomicAdd(&cnt[bucket(e & cmask)], 1);
+        else
+            for (int t = 0; t < k; ++t) atomicAdd(&cnt[bucket(gtask[8 * (size_t) g + t].x)], 1);
+    }
     __syncthreads();
     if (tid == 0) {
+        unsigned long long total = 0;
+        for (int k = 0; k < NBH_ORDER_THREADS / 64; ++k) total += wsum[k];
+        scal->group_mean = (unsigned) min(total / (unsigned long long) max(ngroup, 1),
+                                          (unsigned long long) cmask);
         int acc = 0;
         for (int b = 0; b < 256; ++b) {
             off[b] = acc;
             acc += cnt[b];
         }
+        gorder[0] = make_int2(acc, 0);
     }
     __syncthreads();
-    for (int g = tid; g < ngroup; g += NBH_ORDER_THREADS)
-        gorder[atomicAdd(&off[bucket(gcost[g])], 1)] = g;
+    for (int g = tid; g < ngroup; g += NBH_ORDER_THREADS) {
+        const unsigned e = gcost[g];
+        const int k = (int) (e >> 28) + 1;
+        if (k == 1) {
+            gorder[1 + atomicAdd(&off[bucket(e & cmask)], 1)] = make_int2(g, 64 << 8);
+        } else {
+            for (int t = 0; t < k; ++t) {
+                const uint2 tk = gtask[8 * (size_t) g + t];
+                gorder[1 + atomicAdd(&off[bucket(tk.x)], 1)] = make_int2(g, (int) tk.y);
+            }
+        }
+    }
 }
 
 static inline int nbh_gblocks(int n, int per) { return n > 0 ? (n + per - 1) / per : 1; }
@@ -558,11 +679,12 @@ void nbh_walk_group(nbhip_state* st, const WalkOut& o, double* oax, double* oay,
     const int bs = (t.block == 64 || t.block == 128) ? t.block : 256;
     int lc = (t.batch == 64 || t.batch == 128) ? t.batch : 256;
     if (bs == 256 && lc > 128) lc = 128;       /* LDS: at most 64 KiB per workgroup */
-    dim3 grid(nbh_gblocks(st->n, bs)), block(bs);
+    dim3 grid(nbh_gblocks(nbh_group_tasks(st->n) * 64, bs)), block(bs);
     hipLaunchKernelGGL(k_group_cost, dim3(nbh_gblocks(st->n, 256)), dim3(256), 0, st->stream,
-                       st->bcost, st->perm, st->gcost, st->n);
+                       st->bcost, st->perm, st->simid_s, st->dadj, st->scal, st->gcost, st->gtask,
+                       st->n);
     hipLaunchKernelGGL(k_group_order, dim3(1), dim3(NBH_ORDER_THREADS), 0, st->stream, st->gcost,
-                       st->gorder, (st->n + 63) / 64);
+                       st->gtask, st->gorder, st->scal, (st->n + 63) / 64);
     const bool quad = st->ctx.use_quad != 0;
     const bool eps0 = !o.all_eps2_pos;
     const bool far = o.far2 > 0.0;

@@ -23,7 +23,9 @@ from __future__ import annotations
 
 import numpy as np
 
-SPLIT_T = 3
+SPLIT_T = 2
+SPLIT_ABS = 4
+SPLIT_PASSES = 3
 WAVE = 64
 
 
@@ -79,9 +81,10 @@ def link_levels(pos_sorted, rsize):
 
 def _split_run(lvl, a, e, cuts):
     """One pass over run [a, e] (inclusive lanes): cut after the first
-    weakest link when it is SPLIT_T levels below the lower median."""
+    weakest link when its level is more than SPLIT_T below the lower median,
+    or at most SPLIT_ABS."""
     nl = e - a
-    if nl < 2:
+    if nl < 1:
         return
     d = lvl[a:e]
     if (d < 0).any():
@@ -89,7 +92,7 @@ def _split_run(lvl, a, e, cuts):
     j = int(np.argmin(d))
     need = (nl + 1) // 2
     med = int(np.sort(d)[need - 1])
-    if d[j] < med - SPLIT_T:
+    if d[j] < med - SPLIT_T or d[j] <= SPLIT_ABS:
         cuts.add(a + j)
 
 
@@ -98,7 +101,7 @@ def wave_groups(lvl_wave, nvalid):
     the link between lanes i and i+1 (-1 = hard cut)."""
     cuts = {nvalid - 1}
     cuts.update(i for i in range(nvalid - 1) if lvl_wave[i] < 0)
-    for _ in range(2):
+    for _ in range(SPLIT_PASSES):
         new = set(cuts)
         a = 0
         for e in sorted(cuts):
