@@ -106,7 +106,7 @@ static void nbh_free_all(nbhip_state* st)
                      st->scal, st->sims, st->ignore, st->hist_buf, st->disp_buf, st->pos4,
                      st->exact_part, st->nodeg, st->nkid, st->kbase, st->best_buf,
                      st->nrank, st->kid, st->blist, st->span, st->span_sorted, st->gcost,
-                     st->gorder, st->bcost, st->gtask };
+                     st->gorder, st->bcost, st->gtask, st->nnext };
     for (void* p : ptrs)
         if (p) (void) hipFree(p);
     if (st->graph_exec) (void) hipGraphExecDestroy(st->graph_exec);
@@ -211,7 +211,7 @@ static unsigned nbh_create(const nbhip_ctx* ctx, int device, int nsim, const int
     A(scan_part, std::max<size_t>(4096, (nc + NBH_SCAN_TILE - 1) / NBH_SCAN_TILE));
     A(blo, n); A(bhi, n); A(bdelta, n); A(bparent, 2 * n);
     A(node, nc); A(nquad, 6 * nc); A(nmid, 3 * nc); A(nlevel, nc); A(nparent, nc);
-    A(nsim_node, nc); A(level_list, nc);
+    A(nsim_node, nc); A(level_list, nc); A(nnext, nc);
     if (ctx->precision == NBHIP_PRECISION_MIXED || ctx->precision == NBHIP_PRECISION_MIXED_COHERENT)
         A(nodef, nc + 1);   /* + 1: walk prefetch */
     if (ctx->precision == NBHIP_PRECISION_MIXED_GROUP) {

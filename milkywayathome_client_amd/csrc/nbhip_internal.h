@@ -161,6 +161,8 @@ struct nbhip_state {
     NodeW *node;
     double *nquad;                /* 6 per node */
     double *nmid;                 /* 3 per node (cells) */
+    int *nnext;                   /* cells: pre-order index of Next(cell) (the walk record's
+                                     next, kept compact for the summaries) */
     int *nlevel, *nparent, *nsim_node;
     int *level_list;              /* cell pre-order indices grouped by level */
     NodeF *nodef;                 /* mixed-precision walk records (precision MIXED) */

@@ -844,7 +844,8 @@ __device__ __forceinline__ void nbh_emit_cell(int pre, int l, int next, int pare
                                               unsigned long long uy, unsigned long long uz,
                                               NodeW* __restrict__ node, double* __restrict__ nmid,
                                               int* __restrict__ nlevel, int* __restrict__ nparent,
-                                              int* __restrict__ nsim, SimTab* __restrict__ sims)
+                                              int* __restrict__ nsim, SimTab* __restrict__ sims,
+                                              int* __restrict__ nnext)
 {
     NodeW c;
     c.x = c.y = c.z = c.m = 0.0;
@@ -852,6 +853,7 @@ __device__ __forceinline__ void nbh_emit_cell(int pre, int l, int next, int pare
     c.next = next;
     c.body = -1;
     node[pre] = c;
+    nnext[pre] = next;
     nlevel[pre] = l;
     nparent[pre] = parent;
     nsim[pre] = sim;
@@ -880,7 +882,7 @@ __device__ __forceinline__ void nbh_emit_cells_body(
     const int* __restrict__ simid_s, int n, NodeW* __restrict__ node,
     double* __restrict__ nmid, int* __restrict__ nlevel, int* __restrict__ nparent,
     int* __restrict__ nsim, SimTab* __restrict__ sims, unsigned* __restrict__ nkid,
-    int* __restrict__ nrank, DevScalars* __restrict__ scal, int* lh)
+    int* __restrict__ nrank, int* __restrict__ nnext, DevScalars* __restrict__ scal, int* lh)
 {
     const int k = blockIdx.x * 256 + threadIdx.x;
     const int M = scal->nmassive;
@@ -914,7 +916,7 @@ __device__ __forceinline__ void nbh_emit_cells_body(
         }
         const int par = (l == Lp + 1) ? top_parent : pre - 1;
         nbh_emit_cell(pre, l, next, par, sim, R, ux, uy, uz, node, nmid, nlevel, nparent, nsim,
-                      sims);
+                      sims, nnext);
         /* the cell's rank among its parent's children (arrival order; the
          * summaries sort each child list into octant order) */
         if (par >= 0) nrank[pre] = (int) atomicAdd(&nkid[par], 1u);
@@ -938,13 +940,14 @@ __global__ __launch_bounds__(256) void k_emit_cells(const int* __restrict__ blo,
                                                     SimTab* __restrict__ sims,
                                                     unsigned* __restrict__ nkid,
                                                     int* __restrict__ nrank,
+                                                    int* __restrict__ nnext,
                                                     DevScalars* __restrict__ scal)
 {
     __shared__ int lh[64];
     if (threadIdx.x < 64) lh[threadIdx.x] = 0;
     __syncthreads();
     nbh_emit_cells_body(blo, bhi, bdelta, bparent, dadj, cincl, su, simid_s, n, node, nmid,
-                        nlevel, nparent, nsim, sims, nkid, nrank, scal, lh);
+                        nlevel, nparent, nsim, sims, nkid, nrank, nnext, scal, lh);
     __syncthreads();
     if (threadIdx.x < 64 && lh[threadIdx.x])
         atomicAdd(&scal->level_count[threadIdx.x], lh[threadIdx.x]);
@@ -968,6 +971,7 @@ __global__ __launch_bounds__(256) void k_emit_leaves(const int* __restrict__ blo
                                                      NodeF* __restrict__ nodef,
                                                      unsigned* __restrict__ nkid,
                                                      int* __restrict__ nrank,
+                                                     int* __restrict__ nnext,
                                                      DevScalars* __restrict__ scal)
 {
     const int s = blockIdx.x * 256 + threadIdx.x;
@@ -983,7 +987,7 @@ __global__ __launch_bounds__(256) void k_emit_leaves(const int* __restrict__ blo
         parent = pre - 1;
         if (parent >= 0)
             nbh_emit_cell(parent, 0, pre + 1, -1, sim, sims[sim].rsize, 0ull, 0ull, 0ull, node,
-                          nmid, nlevel, nparent, nsim, sims);
+                          nmid, nlevel, nparent, nsim, sims, nnext);
         atomicAdd(&scal->level_count[0], 1);
     } else {
         parent = nbh_owner_pre(v, bparent[(M - 1) + s], M);
@@ -1166,6 +1170,7 @@ struct SumView {
     NodeW* node;
     double* nquad;
     const double* nmid;
+    const int* nnext;
     const int* nsim;
     const int* nlevel;
     const int* kid;
@@ -1178,18 +1183,18 @@ struct SumView {
 };
 
 #define NBH_NO_KID 0x7fffffff   /* an unused slot of a sorted child list */
+#define NBH_KID_CELL 0x80000000u /* flag of a child list entry: the child is a cell */
 
 /* The structure half of a cell's summary: its children, sorted into
- * pre-order (= octant order), into ch[8] (unused slots NBH_NO_KID), checked
- * against the Next links; false on a child list that disagrees with them (a
- * corrupt table).  Reads nothing that a summary writes except Next, which a
- * summary rewrites with the same value, so it may run before the children
- * are summarised. */
+ * pre-order (= octant order), into ch[8] (unused slots NBH_NO_KID; cells
+ * flagged NBH_KID_CELL), checked against the Next links; false on a child
+ * list that disagrees with them (a corrupt table).  Reads structure only
+ * (levels, Next links, child lists), so it may run before the children are
+ * summarised. */
 __device__ __forceinline__ bool nbh_cell_children(const SumView& v, int p, int* __restrict__ out)
 {
-    const NodeW* __restrict__ node = v.node;
     const int level = v.nlevel[p];
-    const int end = node[p].next;
+    const int end = v.nnext[p];
     const int nk = (int) v.g.nkid[p];
     if (level < 0 || end <= p + 1 || end > v.nn || nk < 1 || nk > 8) return false;
     const int kb = (int) v.g.kbase[p];
@@ -1211,48 +1216,66 @@ __device__ __forceinline__ bool nbh_cell_children(const SumView& v, int p, int* 
     NBH_CE(2, 4); NBH_CE(3, 5);
     NBH_CE(1, 2); NBH_CE(3, 4); NBH_CE(5, 6);
 #undef NBH_CE
+    if (ch[0] != p + 1) return false;
+    for (int j = 0; j < nk; ++j)
+        if (ch[j] < 0 || ch[j] >= v.nn) return false;
+    int lv[8];
+    #pragma unroll
+    for (int j = 0; j < 8; ++j) lv[j] = j < nk ? v.nlevel[ch[j]] : -1;
     int nx[8];
     #pragma unroll
-    for (int j = 0; j < 8; ++j) nx[j] = j < nk ? node[ch[j]].next : 0;
+    for (int j = 0; j < 8; ++j) nx[j] = j < nk ? (lv[j] < 0 ? ch[j] + 1 : v.nnext[ch[j]]) : 0;
     /* the list must be the Next chain p+1 -> ... -> end */
-    if (ch[0] != p + 1) return false;
     #pragma unroll
     for (int j = 0; j < 8; ++j)
         if (j < nk && nx[j] != (j + 1 < nk ? ch[j + 1] : end)) return false;
     #pragma unroll
-    for (int j = 0; j < 8; ++j) out[j] = ch[j];
+    for (int j = 0; j < 8; ++j)
+        out[j] = j < nk ? (int) ((unsigned) ch[j] | (lv[j] >= 0 ? NBH_KID_CELL : 0u)) : NBH_NO_KID;
     return true;
 }
 
 /* The value half: the cell's record (centre of mass, mass, critical
  * radius, quadrupole, and the mixed walk's record) from its children's
- * finished records; ch is the cell's checked, sorted child list. */
+ * finished records; chl is the cell's checked, sorted, flagged child list.
+ * The children's positions and masses and the quadrupoles of the child
+ * cells are loaded together (the flags say which children are cells). */
 __device__ __forceinline__ void nbh_cell_summary(const SumView& v, int p, const int* __restrict__ chl)
 {
     NodeW* __restrict__ node = v.node;
     int ch[8];
+    bool cell[8];
+    int nk = 0;
     #pragma unroll
-    for (int j = 0; j < 8; ++j) ch[j] = chl[j];
+    for (int j = 0; j < 8; ++j) {
+        const int e = chl[j];
+        nk += e != NBH_NO_KID;
+        cell[j] = e != NBH_NO_KID && ((unsigned) e & NBH_KID_CELL);
+        ch[j] = e & 0x7fffffff;
+    }
     const int level = v.nlevel[p];
     const int sim = v.nsim[p];
     const double rsize = v.sims[sim].rsize;
-    const int end = node[p].next;
-    int nk = 0;
+    const int end = v.nnext[p];
+    const bool quad = v.ctx.use_quad != 0;
+    double4 w[8];                                  /* x y z m */
+    double cq[8][6];
     #pragma unroll
-    for (int j = 0; j < 8; ++j) nk += ch[j] != NBH_NO_KID;
-    NodeW w[8];
-    #pragma unroll
-    for (int j = 0; j < 8; ++j)
-        if (j < nk) w[j] = node[ch[j]];
+    for (int j = 0; j < 8; ++j) {
+        if (j < nk) w[j] = *reinterpret_cast<const double4*>(&node[ch[j]]);
+        #pragma unroll
+        for (int k = 0; k < 6; ++k)
+            cq[j][k] = (quad && j < nk && cell[j]) ? v.nquad[6 * (size_t) ch[j] + k] : 0.0;
+    }
 
     double mass = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
     #pragma unroll
     for (int j = 0; j < 8; ++j) {
         if (j < nk) {
-            mass += w[j].m;
-            cx += w[j].m * w[j].x;
-            cy += w[j].m * w[j].y;
-            cz += w[j].m * w[j].z;
+            mass += w[j].w;
+            cx += w[j].w * w[j].x;
+            cy += w[j].w * w[j].y;
+            cz += w[j].w * w[j].z;
         }
     }
     const double mx = v.nmid[3 * p + 0], my = v.nmid[3 * p + 1], mz = v.nmid[3 * p + 2];
@@ -1271,24 +1294,23 @@ __device__ __forceinline__ void nbh_cell_summary(const SumView& v, int p, const 
     const double rc2 = nbh_rcrit2(v.ctx, mx, my, mz, rsize, cx, cy, cz, psize);
 
     double* pq = &v.nquad[6 * (size_t) p];
-    if (v.ctx.use_quad) {
+    if (quad) {
         double q0 = 0.0, q1 = 0.0, q2 = 0.0, q3 = 0.0, q4 = 0.0, q5 = 0.0;
         #pragma unroll
         for (int j = 0; j < 8; ++j) {
             if (j >= nk) continue;
             const double dx = w[j].x - cx, dy = w[j].y - cy, dz = w[j].z - cz;
             const double drsq = dz * dz + (dy * dy + dx * dx);
-            const double m = w[j].m;
+            const double m = w[j].w;
             double e0 = m * (3.0 * (dx * dx) - drsq);
             double e1 = m * (3.0 * (dx * dy));
             double e2 = m * (3.0 * (dx * dz));
             double e3 = m * (3.0 * (dy * dy) - drsq);
             double e4 = m * (3.0 * (dy * dz));
             double e5 = m * (3.0 * (dz * dz) - drsq);
-            if (w[j].body < 0) {
-                const double* cq = &v.nquad[6 * (size_t) ch[j]];
-                e0 += cq[0]; e1 += cq[1]; e2 += cq[2];
-                e3 += cq[3]; e4 += cq[4]; e5 += cq[5];
+            if (cell[j]) {
+                e0 += cq[j][0]; e1 += cq[j][1]; e2 += cq[j][2];
+                e3 += cq[j][3]; e4 += cq[j][4]; e5 += cq[j][5];
             }
             q0 += e0; q1 += e1; q2 += e2; q3 += e3; q4 += e4; q5 += e5;
         }
@@ -1363,8 +1385,8 @@ __device__ bool nbh_sum_levels(const SumView& v, const int* list, const int* off
                 s_ch[2 * i] = make_int4(ch[0], ch[1], ch[2], ch[3]);
                 s_ch[2 * i + 1] = make_int4(ch[4], ch[5], ch[6], ch[7]);
             } else {
-                ok = false;
-                s_ch[2 * i] = make_int4(-1, -1, -1, -1);
+                ok = false;                               /* marked: a list never starts empty */
+                s_ch[2 * i] = make_int4(NBH_NO_KID, NBH_NO_KID, NBH_NO_KID, NBH_NO_KID);
             }
         }
         __syncthreads();
@@ -1372,7 +1394,7 @@ __device__ bool nbh_sum_levels(const SumView& v, const int* list, const int* off
             if (off[l + 1] == off[l]) continue;           /* uniform */
             for (int i = off[l] + tid; i < off[l + 1]; i += NT) {
                 const int4 a = s_ch[2 * i];
-                if (a.x < 0) continue;
+                if (a.x == NBH_NO_KID) continue;
                 const int4 b = s_ch[2 * i + 1];
                 const int ch[8] = { a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w };
                 nbh_cell_summary(v, list[i], ch);
@@ -1427,7 +1449,7 @@ __global__ __launch_bounds__(NBH_SUM_THREADS) void k_sum_blocks(SumView v,
             atomicOr(&scal->status, NBHIP_ERROR);
             continue;
         }
-        if (v.node[q].next <= P1) {
+        if (v.nnext[q] <= P1) {
             atomicAdd(&cnt[l], 1);
         } else {
             const int k = atomicAdd(&nspan, 1);
@@ -1454,7 +1476,7 @@ __global__ __launch_bounds__(NBH_SUM_THREADS) void k_sum_blocks(SumView v,
     int* lst = off[64] <= NBH_SUM_CAP ? s_list : blist + P0;
     for (int q = P0 + tid; q < P1; q += NBH_SUM_THREADS) {
         const int l = v.nlevel[q];
-        if (l < 0 || l > 63 || v.node[q].next > P1) continue;
+        if (l < 0 || l > 63 || v.nnext[q] > P1) continue;
         lst[off[l] + atomicAdd(&fill[l], 1)] = q;
     }
     __syncthreads();
@@ -1484,7 +1506,7 @@ __device__ void nbh_sum_handed(const SumView& v, const int* cand, int nc, int P1
             atomicOr(&scal->status, NBHIP_ERROR);
             continue;
         }
-        if (v.node[q].next <= P1) {
+        if (v.nnext[q] <= P1) {
             atomicAdd(&cnt[l], 1);
         } else {
             const int k = atomicAdd(nspan, 1);
@@ -1506,7 +1528,7 @@ __device__ void nbh_sum_handed(const SumView& v, const int* cand, int nc, int P1
     for (int i = tid; i < nc; i += NT) {
         const int q = cand[i];
         const int l = v.nlevel[q];
-        if (l < 0 || l > 63 || v.node[q].next > P1) continue;
+        if (l < 0 || l > 63 || v.nnext[q] > P1) continue;
         list[off[l] + atomicAdd(&fill[l], 1)] = q;
     }
     __syncthreads();
@@ -1739,11 +1761,11 @@ void nbh_build(nbhip_state* st, int graph_safe)
     hipLaunchKernelGGL(k_emit_cells, dim3(g256), dim3(256), 0, s, st->blo, st->bhi, st->bdelta,
                        st->bparent, st->dadj, st->cincl, st->su, st->simid_s, n, st->node,
                        st->nmid, st->nlevel, st->nparent, st->nsim_node, st->sims, st->nkid,
-                       st->nrank, st->scal);
+                       st->nrank, st->nnext, st->scal);
     hipLaunchKernelGGL(k_emit_leaves, dim3(g256), dim3(256), 0, s, st->blo, st->bdelta,
                        st->bparent, st->dadj, st->cincl, st->spos, st->simid_s, st->node,
                        st->nmid, st->nlevel, st->nparent, st->nsim_node, st->sims, st->nodef,
-                       st->nkid, st->nrank, st->scal);
+                       st->nkid, st->nrank, st->nnext, st->scal);
     /* each cell's child block (kbase) and child list (kid) */
     const int nt = nbh_blocks(st->node_cap, NBH_SCAN_TILE);
     hipLaunchKernelGGL(k_nscan_tiles, dim3(nt), dim3(1024), 0, s, st->nkid, st->kbase,
@@ -1763,6 +1785,7 @@ void nbh_build(nbhip_state* st, int graph_safe)
     v.node = st->node;
     v.nquad = st->nquad;
     v.nmid = st->nmid;
+    v.nnext = st->nnext;
     v.nsim = st->nsim_node;
     v.nlevel = st->nlevel;
     v.kid = st->kid;
