@@ -509,8 +509,12 @@ __global__ __launch_bounds__(256) void k_delta_count(int n, const unsigned long 
     const int wmax = nbh_wave_max_i(sim >= 0 ? depth : -1);
     if (__lane_id() == 0 && wmax >= 0) atomicMax(&bmax, wmax);
     __syncthreads();
-    if (sim >= 0 && mixed) atomicMax(&sims[sim].maxdepth, depth);
-    if (threadIdx.x == 0 && !mixed && bsim >= 0) atomicMax(&sims[bsim].maxdepth, bmax);
+    /* the maximum only grows, so a value already at least as deep (even a
+     * stale read of it) makes the atomic unnecessary: few of the workgroups
+     * that share a simulation reach it */
+    if (sim >= 0 && mixed && depth > sims[sim].maxdepth) atomicMax(&sims[sim].maxdepth, depth);
+    if (threadIdx.x == 0 && !mixed && bsim >= 0 && bmax > sims[bsim].maxdepth)
+        atomicMax(&sims[bsim].maxdepth, bmax);
 }
 
 /* ---------------------------------------------------------------------- */
@@ -847,12 +851,9 @@ __device__ __forceinline__ void nbh_emit_cell(int pre, int l, int next, int pare
                                               int* __restrict__ nsim, SimTab* __restrict__ sims,
                                               int* __restrict__ nnext)
 {
-    NodeW c;
-    c.x = c.y = c.z = c.m = 0.0;
-    c.rcrit2 = 0.0;
-    c.next = next;
-    c.body = -1;
-    node[pre] = c;
+    /* the cell's walk record is written whole by its summary; until then
+     * its Next link lives in nnext */
+    (void) node;
     nnext[pre] = next;
     nlevel[pre] = l;
     nparent[pre] = parent;
@@ -882,7 +883,7 @@ __device__ __forceinline__ void nbh_emit_cells_body(
     const int* __restrict__ simid_s, int n, NodeW* __restrict__ node,
     double* __restrict__ nmid, int* __restrict__ nlevel, int* __restrict__ nparent,
     int* __restrict__ nsim, SimTab* __restrict__ sims, unsigned* __restrict__ nkid,
-    int* __restrict__ nrank, int* __restrict__ nnext, DevScalars* __restrict__ scal, int* lh)
+    int* __restrict__ nrank, int* __restrict__ nnext, DevScalars* __restrict__ scal)
 {
     const int k = blockIdx.x * 256 + threadIdx.x;
     const int M = scal->nmassive;
@@ -920,7 +921,6 @@ __device__ __forceinline__ void nbh_emit_cells_body(
         /* the cell's rank among its parent's children (arrival order; the
          * summaries sort each child list into octant order) */
         if (par >= 0) nrank[pre] = (int) atomicAdd(&nkid[par], 1u);
-        atomicAdd(&lh[l], 1);
     }
 }
 
@@ -943,14 +943,8 @@ __global__ __launch_bounds__(256) void k_emit_cells(const int* __restrict__ blo,
                                                     int* __restrict__ nnext,
                                                     DevScalars* __restrict__ scal)
 {
-    __shared__ int lh[64];
-    if (threadIdx.x < 64) lh[threadIdx.x] = 0;
-    __syncthreads();
     nbh_emit_cells_body(blo, bhi, bdelta, bparent, dadj, cincl, su, simid_s, n, node, nmid,
-                        nlevel, nparent, nsim, sims, nkid, nrank, nnext, scal, lh);
-    __syncthreads();
-    if (threadIdx.x < 64 && lh[threadIdx.x])
-        atomicAdd(&scal->level_count[threadIdx.x], lh[threadIdx.x]);
+                        nlevel, nparent, nsim, sims, nkid, nrank, nnext, scal);
 }
 
 /* Body leaves; a simulation with a single massive body gets its root cell
@@ -988,7 +982,6 @@ __global__ __launch_bounds__(256) void k_emit_leaves(const int* __restrict__ blo
         if (parent >= 0)
             nbh_emit_cell(parent, 0, pre + 1, -1, sim, sims[sim].rsize, 0ull, 0ull, 0ull, node,
                           nmid, nlevel, nparent, nsim, sims, nnext);
-        atomicAdd(&scal->level_count[0], 1);
     } else {
         parent = nbh_owner_pre(v, bparent[(M - 1) + s], M);
     }
@@ -1689,13 +1682,35 @@ static void nbh_mark(nbhip_state* st, int k)
  * top-down pass over the cells needs them; the summaries do not. */
 __global__ void k_level_fill_reset(DevScalars* __restrict__ scal)
 {
-    if (threadIdx.x < 64) scal->level_fill[threadIdx.x] = 0;
+    if (threadIdx.x < 64) {
+        scal->level_fill[threadIdx.x] = 0;
+        scal->level_count[threadIdx.x] = 0;
+    }
+}
+
+/* Cells per level of the current tree (grid-stride over the nodes built). */
+__global__ __launch_bounds__(256) void k_level_count(const int* __restrict__ nlevel,
+                                                     DevScalars* __restrict__ scal)
+{
+    __shared__ int lh[64];
+    const int tid = threadIdx.x;
+    if (tid < 64) lh[tid] = 0;
+    __syncthreads();
+    const int nn = (scal->status & NBHIP_CELL_OVERFLOW_ERROR) ? 0 : scal->nnode;
+    for (int i = blockIdx.x * 256 + tid; i < nn; i += gridDim.x * 256) {
+        const int l = nlevel[i];
+        if (l >= 0 && l <= 63) atomicAdd(&lh[l], 1);
+    }
+    __syncthreads();
+    if (tid < 64 && lh[tid]) atomicAdd(&scal->level_count[tid], lh[tid]);
 }
 
 unsigned nbh_level_lists(nbhip_state* st)
 {
     hipStream_t s = st->stream;
     hipLaunchKernelGGL(k_level_fill_reset, dim3(1), dim3(64), 0, s, st->scal);
+    hipLaunchKernelGGL(k_level_count, dim3(nbh_blocks(2 * st->n + 1024, NBH_LS_CHUNK)), dim3(256),
+                       0, s, st->nlevel, st->scal);
     hipLaunchKernelGGL(k_level_scatter, dim3(nbh_blocks(2 * st->n + 1024, NBH_LS_CHUNK)),
                        dim3(256), 0, s, st->nlevel, st->level_list, st->scal);
     hipError_t e = hipMemcpyAsync(st->h_counts_pinned, st->scal->level_count, 64 * sizeof(int),
