@@ -106,7 +106,7 @@ static void nbh_free_all(nbhip_state* st)
                      st->scal, st->sims, st->ignore, st->hist_buf, st->disp_buf, st->pos4,
                      st->exact_part, st->nodeg, st->nkid, st->kbase, st->best_buf,
                      st->nrank, st->kid, st->blist, st->span, st->span_sorted, st->gcost,
-                     st->gorder, st->bcost, st->gtask, st->nnext };
+                     st->gorder, st->bcost, st->gtask, st->nnext, st->bbox };
     for (void* p : ptrs)
         if (p) (void) hipFree(p);
     if (st->graph_exec) (void) hipGraphExecDestroy(st->graph_exec);
@@ -218,11 +218,13 @@ static unsigned nbh_create(const nbhip_ctx* ctx, int device, int nsim, const int
         const size_t ng = ((size_t) st->n + 63) / 64 + 1;
         A(nodeg, nc + (size_t) nsim);
         A(bcost, n ? n : 1);
+        A(bbox, n ? n : 1);
         A(gcost, ng);
         A(gtask, 8 * ng);
         A(gorder, (size_t) nbh_group_tasks(st->n) + 1);
         /* the first walk has no costs yet: all zero, so every group in one bucket */
         if (e == hipSuccess) e = hipMemset(st->bcost, 0, (n ? n : 1) * sizeof(unsigned));
+        if (e == hipSuccess) e = hipMemset(st->bbox, 0, (n ? n : 1) * sizeof(float));
     }
     A(nkid, nc); A(kbase, nc); A(nrank, nc); A(kid, nc); A(blist, nc); A(span, nc);
     A(span_sorted, nc);

@@ -169,6 +169,8 @@ struct nbhip_state {
     NodeG *nodeg;                 /* group walk table (precision MIXED_GROUP): nsim + node_cap */
     unsigned *bcost;              /* group walk: per body (original order), its sub-group's work
                                      in the last walk */
+    float *bbox;                  /* group walk: per body, its sub-group's box half-diagonal in
+                                     the last walk */
     unsigned *gcost;              /* group walk: predicted work of each 64-body group, and
                                      its task count - 1 in bits 28..30 */
     uint2 *gtask;                 /* group walk: up to 8 tasks per group: (cost, lo | hi << 8) */
