@@ -262,6 +262,15 @@ __global__ __launch_bounds__(256) void k_radix_scan(unsigned* __restrict__ hist,
     if (t == 255) dtot[blockIdx.x] = v + before;
 }
 
+/* Stable scatter of one pass.  Wave w of the block owns the tile's keys
+ * [w * T/4, (w + 1) * T/4) as T/256 rows of 64; it ranks each row with a
+ * wave64 multisplit (one ballot per digit bit) on top of its own per-digit
+ * counts in LDS (the row's leader lane per digit reads and advances them,
+ * so no barrier is needed between rows), then the four waves' counts are
+ * scanned per digit.  Ranks therefore follow key order within the tile.
+ * The tile is then laid out in digit order in LDS and written run by run
+ * (each digit's run at the digit's base plus the block's offset, from the
+ * histogram scan), so stores to the output are contiguous per run. */
 __global__ __launch_bounds__(256) void k_radix_scatter(int n, int shift,
                                                        const unsigned long long* __restrict__ kin,
                                                        const unsigned* __restrict__ vin,
@@ -270,68 +279,104 @@ __global__ __launch_bounds__(256) void k_radix_scatter(int n, int shift,
                                                        const unsigned* __restrict__ hist,
                                                        const unsigned* __restrict__ dtot, int nb)
 {
+    constexpr int R = NBH_SORT_TILE / 256;           /* rows per wave */
     __shared__ unsigned goff[256];
-    __shared__ unsigned running[256];
-    __shared__ unsigned wcnt[4][256];
+    __shared__ unsigned whist[4][256];
+    __shared__ unsigned tbase[256];
     __shared__ unsigned wsum[4];
+    __shared__ unsigned long long skey[NBH_SORT_TILE];
+    __shared__ unsigned sval[NBH_SORT_TILE];
     const int t = threadIdx.x;
     const int w = t >> 6;
+    const int lane = t & 63;
     /* digit bases: exclusive scan of the 256 digit totals */
     {
         const unsigned s0 = dtot[t];
         unsigned v = s0;
         for (int off = 1; off < 64; off <<= 1) {
             unsigned u = __shfl_up(v, off, 64);
-            if ((int) __lane_id() >= off) v += u;
+            if (lane >= off) v += u;
         }
-        if (__lane_id() == 63) wsum[w] = v;
+        if (lane == 63) wsum[w] = v;
+        whist[0][t] = whist[1][t] = whist[2][t] = whist[3][t] = 0;
         __syncthreads();
         unsigned before = 0;
         for (int ww = 0; ww < w; ++ww) before += wsum[ww];
         goff[t] = v - s0 + before + hist[t * nb + blockIdx.x];
     }
-    running[t] = 0;
-    wcnt[0][t] = wcnt[1][t] = wcnt[2][t] = wcnt[3][t] = 0;
-    __syncthreads();
     const unsigned long long lt = nbh_lanemask_lt();
-    const int base = blockIdx.x * NBH_SORT_TILE;
-    /* all of the thread's keys and values in flight at once (one memory
-     * latency per tile instead of one per row), then the rows in order */
-    unsigned long long kr[NBH_SORT_TILE / 256];
-    unsigned vr[NBH_SORT_TILE / 256];
+    const int base = blockIdx.x * NBH_SORT_TILE + w * (NBH_SORT_TILE / 4);
+    unsigned long long kr[R];
+    unsigned vr[R];
+    unsigned rk[R];
     #pragma unroll
-    for (int r = 0; r < NBH_SORT_TILE / 256; ++r) {
-        const int idx = base + r * 256 + t;
+    for (int r = 0; r < R; ++r) {
+        const int idx = base + r * 64 + lane;
         kr[r] = idx < n ? kin[idx] : 0ull;
         vr[r] = idx < n ? vin[idx] : 0u;
     }
+    unsigned* wh = whist[w];
     #pragma unroll
-    for (int r = 0; r < NBH_SORT_TILE / 256; ++r) {
-        const int idx = base + r * 256 + t;
-        const bool valid = idx < n;
-        const unsigned long long k = kr[r];
-        const unsigned v = vr[r];
-        unsigned d = (unsigned) (k >> shift) & 255u;
+    for (int r = 0; r < R; ++r) {
+        const bool valid = base + r * 64 + lane < n;
+        const unsigned d = (unsigned) (kr[r] >> shift) & 255u;
         unsigned long long peers = __ballot(valid);
         #pragma unroll
         for (int b = 0; b < 8; ++b) {
-            unsigned long long bb = __ballot(valid && ((d >> b) & 1u));
+            const unsigned long long bb = __ballot(valid && ((d >> b) & 1u));
             peers &= ((d >> b) & 1u) ? bb : ~bb;
         }
-        unsigned rank = (unsigned) __popcll(peers & lt);
-        if (valid && rank == 0) wcnt[w][d] = (unsigned) __popcll(peers);
-        __syncthreads();
-        if (valid) {
-            unsigned before = running[d];
-            for (int ww = 0; ww < w; ++ww) before += wcnt[ww][d];
-            unsigned pos = goff[d] + before + rank;
-            kout[pos] = k;
-            vout[pos] = v;
+        const int leader = valid ? __builtin_ctzll(peers) : lane;
+        unsigned before = 0;
+        if (valid && lane == leader) {
+            before = wh[d];
+            wh[d] = before + (unsigned) __popcll(peers);
         }
+        before = (unsigned) __shfl((int) before, leader, 64);
+        rk[r] = before + (unsigned) __popcll(peers & lt);
+        __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+    /* digit d = t: the waves' offsets within the tile's run of digit d, and
+     * the run's start in the tile (an exclusive scan over digits) */
+    unsigned acc = 0;
+    #pragma unroll
+    for (int ww = 0; ww < 4; ++ww) {
+        const unsigned c = whist[ww][t];
+        whist[ww][t] = acc;
+        acc += c;
+    }
+    {
+        unsigned v = acc;
+        for (int off = 1; off < 64; off <<= 1) {
+            unsigned u = __shfl_up(v, off, 64);
+            if (lane >= off) v += u;
+        }
+        if (lane == 63) wsum[w] = v;
         __syncthreads();
-        running[t] += wcnt[0][t] + wcnt[1][t] + wcnt[2][t] + wcnt[3][t];
-        wcnt[0][t] = wcnt[1][t] = wcnt[2][t] = wcnt[3][t] = 0;
-        __syncthreads();
+        unsigned before = 0;
+        for (int ww = 0; ww < w; ++ww) before += wsum[ww];
+        tbase[t] = v - acc + before;
+    }
+    __syncthreads();
+    /* the tile in digit order in LDS, then written out run by run: lanes
+     * that follow each other write neighbouring addresses of one run */
+    #pragma unroll
+    for (int r = 0; r < R; ++r) {
+        if (base + r * 64 + lane >= n) continue;
+        const unsigned d = (unsigned) (kr[r] >> shift) & 255u;
+        const unsigned lp = tbase[d] + wh[d] + rk[r];
+        skey[lp] = kr[r];
+        sval[lp] = vr[r];
+    }
+    __syncthreads();
+    const int cnt = min(NBH_SORT_TILE, n - blockIdx.x * NBH_SORT_TILE);
+    for (int i = t; i < cnt; i += 256) {
+        const unsigned long long k = skey[i];
+        const unsigned d = (unsigned) (k >> shift) & 255u;
+        const unsigned pos = goff[d] + ((unsigned) i - tbase[d]);
+        kout[pos] = k;
+        vout[pos] = sval[i];
     }
 }
 
