@@ -179,3 +179,37 @@ def test_group_expansion_matches_its_restatement(dwarf_case, ratio):
     print(f"expanded vs per-body lists: median {np.median(d):.3e} p99 {np.percentile(d, 99):.3e}",
           flush=True)
     st.close()
+
+
+def test_group_walk_results_do_not_depend_on_dispatch_order_or_sharing():
+    """The first walk of a state orders its groups by nothing (no costs yet)
+    and shares none out; later walks order them by the last walk's costs
+    and share the costliest groups out among several waves, by whole
+    sub-groups.  A dense clump with a sparse outskirt makes a few groups far
+    costlier than the mean, so later walks do share; every walk's lists and
+    accelerations must be the first walk's, bit for bit, and the lists the
+    restatement's."""
+    rng = np.random.default_rng(41)
+    pos, vel, m = plummer_bodies(20000, seed=41, radius=0.05)
+    far = rng.normal(size=(300, 3))
+    far *= (rng.uniform(5.0, 40.0, 300) / np.linalg.norm(far, axis=1))[:, None]
+    pos = np.concatenate([pos, far])
+    vel = np.concatenate([vel, np.zeros((300, 3))])
+    m = np.concatenate([m, np.full(300, m[0])])
+    ctx = nb.NBodyCtx.create(timestep=1e-4, timeEvolve=1e-2, eps2=1e-6, theta=0.5)
+    st = nb.NBodyState(ctx, None, pos, vel, m, precision="mixed_group", prestep=False)
+    st.set_group_expansion(0.1)
+    st.gravity()
+    first = st.walk_accel()
+    c_first = st.walk_counters5()
+    for rep in range(4):
+        st.gravity()
+        a = st.walk_accel()
+        c = st.walk_counters5()
+        print("walk", rep + 2, "identical", np.array_equal(a, first), c[:2], c[4], flush=True)
+        assert np.array_equal(a, first)
+        assert c == c_first
+    gw = _model(st, ctx)
+    want = gw.expanded_counts(0.1, gw.groups())
+    assert (c_first[0], c_first[1], c_first[4]) == want
+    st.close()
