@@ -1398,6 +1398,9 @@ __device__ __forceinline__ bool nbh_summarize_cell(const SumView& v, int p)
 
 #define NBH_SUM_BODIES 1024    /* massive bodies per k_sum_blocks workgroup */
 #define NBH_SUM_THREADS 256
+#define NBH_SUMB_THREADS 128   /* k_sum_blocks: two waves, so that with its ~220 VGPRs
+                                  four workgroups share a CU and all blocks of a
+                                  10^6-body tree run at once */
 #define NBH_SUM_GROUP 64       /* blocks per k_sum_super workgroup */
 #define NBH_SPAN_SLOTS 64      /* spanning cells one block can hand up (one per level) */
 #define NBH_SPAN_THREADS 512
@@ -1452,7 +1455,7 @@ __device__ bool nbh_sum_levels(const SumView& v, const int* list, const int* off
 
 /* Levels holding cells in [lo, hi): counts into cnt[64] (LDS), then an
  * exclusive scan into off[65]; callers add their own cells. */
-__global__ __launch_bounds__(NBH_SUM_THREADS) void k_sum_blocks(SumView v,
+__global__ __launch_bounds__(NBH_SUMB_THREADS) void k_sum_blocks(SumView v,
                                                                 const unsigned* __restrict__ cincl,
                                                                 int* __restrict__ blist,
                                                                 int* __restrict__ span,
@@ -1480,7 +1483,7 @@ __global__ __launch_bounds__(NBH_SUM_THREADS) void k_sum_blocks(SumView v,
     if (tid == 0) nspan = 0;
     __syncthreads();
     /* inside cells per level; spanning cells to the second launch */
-    for (int q = P0 + tid; q < P1; q += NBH_SUM_THREADS) {
+    for (int q = P0 + tid; q < P1; q += NBH_SUMB_THREADS) {
         const int l = v.nlevel[q];
         if (l < 0) continue;                      /* a body's leaf */
         if (l > 63) {
@@ -1512,13 +1515,13 @@ __global__ __launch_bounds__(NBH_SUM_THREADS) void k_sum_blocks(SumView v,
      * NBH_SUM_CAP, in the block's own slice [P0, P0 + count) of blist
      * (slices of different blocks are disjoint) */
     int* lst = off[64] <= NBH_SUM_CAP ? s_list : blist + P0;
-    for (int q = P0 + tid; q < P1; q += NBH_SUM_THREADS) {
+    for (int q = P0 + tid; q < P1; q += NBH_SUMB_THREADS) {
         const int l = v.nlevel[q];
         if (l < 0 || l > 63 || v.nnext[q] > P1) continue;
         lst[off[l] + atomicAdd(&fill[l], 1)] = q;
     }
     __syncthreads();
-    if (!nbh_sum_levels<NBH_SUM_THREADS>(v, lst, off, s_ch)) atomicOr(&scal->status, NBHIP_ERROR);
+    if (!nbh_sum_levels<NBH_SUMB_THREADS>(v, lst, off, s_ch)) atomicOr(&scal->status, NBHIP_ERROR);
 }
 
 /* Cells handed up by several lower workgroups (cand[0..nc) in LDS or
@@ -1865,7 +1868,7 @@ void nbh_build(nbhip_state* st, int graph_safe)
     int* span2_n = span_n + nblk;
     int* top_cand = span2_n + nsup;
     int* top_list = top_cand + (size_t) nsup * NBH_SPAN_SLOTS;
-    hipLaunchKernelGGL(k_sum_blocks, dim3(nblk), dim3(NBH_SUM_THREADS), 0, s, v, st->cincl,
+    hipLaunchKernelGGL(k_sum_blocks, dim3(nblk), dim3(NBH_SUMB_THREADS), 0, s, v, st->cincl,
                        st->blist, st->span, span_n, st->scal);
     hipLaunchKernelGGL(k_sum_super, dim3(nsup), dim3(NBH_SUM_THREADS), 0, s, v, st->cincl,
                        st->span, span_n, span2, span2_n, st->scal);
