@@ -63,15 +63,17 @@ extern "C" {
  * refines its reference list (more accurate, not the reference's lists) */
 #define NBHIP_PRECISION_MIXED_COHERENT 2
 /* LDS-staged group walk: each wavefront takes 64 consecutive bodies in
- * Morton order (split into at most four sub-groups where the order jumps
- * between distant cells), tests the tree's nodes 64 at a time against the
- * group's bounding box with the context's criterion (a node is accepted
- * when the box's nearest point passes it, so every body's interaction list
- * refines its reference list), stages the accepted nodes in LDS as FP32
- * positions relative to the group's centre and runs a dense FP32
- * interaction loop over them; each LDS batch is one FP32 partial sum added
- * into FP64 with Kahan compensation.  More accurate than the reference
- * lists; results depend on which bodies share a wavefront (not
+ * Morton order (cut into sub-groups where the order jumps between distant
+ * cells), tests the tree's nodes 64 at a time against each sub-group's
+ * bounding box with the context's criterion (a node is accepted when the
+ * box's nearest point passes it, so every body's interaction list refines
+ * its reference list), stages the accepted nodes in LDS as FP32 positions
+ * relative to the sub-group's centre and runs a dense FP32 interaction loop
+ * over them; each LDS batch is one FP32 partial sum added into FP64 with
+ * Kahan compensation.  Groups run costliest first by a prediction from the
+ * last walk, and a costly group's sub-groups may be shared out among
+ * several waves; neither changes any result.  More accurate than the
+ * reference lists; results depend on which bodies share a wavefront (not
  * bit-identical across batching). */
 #define NBHIP_PRECISION_MIXED_GROUP 3
 
