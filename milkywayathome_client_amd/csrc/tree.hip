@@ -821,21 +821,53 @@ __global__ __launch_bounds__(256) void k_karras(int n, const unsigned long long*
     const int dl = i > 0 ? dadj[i - 1] : -1;
     const int d = dr > dl ? 1 : -1;
     const int dmin = d > 0 ? dl : dr;
-    int lmax = 2;
-    while (KD(i, i + lmax * d) > dmin) lmax <<= 1;
-    int l = 0;
-    for (int t = lmax >> 1; t >= 1; t >>= 1)
-        if (KD(i, i + (l + t) * d) > dmin) l += t;
-    const int j = i + l * d;
-    const int dnode = KD(i, j);
-    int s = 0;
-    for (int div = 2;; div <<= 1) {
-        int t = (l + div - 1) / div;
-        if (KD(i, i + (s + t) * d) > dnode) s += t;
-        if (t <= 1) break;
+    /* The common prefix of sorted keys i < j is the least adjacent prefix
+     * between them (the index tie-break of equal keys included), so a short
+     * range is found in the next NBH_KW adjacent prefixes in direction d:
+     * its length l is the first of them <= dmin, its prefix the least of
+     * the l before, and its split the position of that least one (unique:
+     * two neighbouring adjacent prefixes never agree).  Longer ranges take
+     * Karras's searches over the keys, from a length the window has shown. */
+    constexpr int NBH_KW = 16;
+    int w[NBH_KW];
+    #pragma unroll
+    for (int k = 0; k < NBH_KW; ++k) {
+        const int idx = d > 0 ? i + k : i - 1 - k;
+        w[k] = (idx >= 0 && idx < M) ? dadj[idx] : -1;
+    }
+    int l = -1;
+    #pragma unroll
+    for (int k = NBH_KW - 1; k >= 0; --k)
+        if (w[k] <= dmin) l = k;
+    int j, dnode, gamma;
+    if (l >= 1) {
+        dnode = w[0];
+        int ks = 0;
+        #pragma unroll
+        for (int k = 1; k < NBH_KW; ++k)
+            if (k < l && w[k] < dnode) {
+                dnode = w[k];
+                ks = k;
+            }
+        j = i + l * d;
+        gamma = d > 0 ? i + ks : i - 1 - ks;
+    } else {
+        int lmax = 2 * NBH_KW;
+        while (KD(i, i + lmax * d) > dmin) lmax <<= 1;
+        int ll = 0;
+        for (int t = lmax >> 1; t >= 1; t >>= 1)
+            if (KD(i, i + (ll + t) * d) > dmin) ll += t;
+        j = i + ll * d;
+        dnode = KD(i, j);
+        int s = 0;
+        for (int div = 2;; div <<= 1) {
+            int t = (ll + div - 1) / div;
+            if (KD(i, i + (s + t) * d) > dnode) s += t;
+            if (t <= 1) break;
+        }
+        gamma = i + s * d + min(d, 0);
     }
 #undef KD
-    const int gamma = i + s * d + min(d, 0);
     const int lo = min(i, j), hi = max(i, j);
     blo[i] = lo;
     bhi[i] = hi;
