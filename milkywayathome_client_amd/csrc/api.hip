@@ -207,7 +207,7 @@ static unsigned nbh_create(const nbhip_ctx* ctx, int device, int nsim, const int
     A(key, n); A(key_alt, n); A(perm, n); A(perm_alt, n);
     A(hist, 256 * (size_t) st->sort_blocks);
     A(dtot, 256);
-    A(pos4, n); A(spos, n); A(simid_s, n); A(su, 3 * n); A(dadj, n); A(cincl, n);
+    A(pos4, n); A(spos, n); A(simid_s, n); A(su, 3 * n); A(dadj, nbh_dadj_cap(n)); A(cincl, n);
     A(scan_part, std::max<size_t>(4096, (nc + NBH_SCAN_TILE - 1) / NBH_SCAN_TILE));
     A(blo, n); A(bhi, n); A(bdelta, n); A(bparent, 2 * n);
     A(node, nc); A(nquad, 6 * nc); A(nmid, 3 * nc); A(nlevel, nc); A(nparent, nc);

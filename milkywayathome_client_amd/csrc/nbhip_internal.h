@@ -109,6 +109,42 @@ struct WalkTune {
 #define NBH_GW_SPLIT_COST 2
 __host__ __device__ inline int nbh_group_tasks(int n) { return (n + 63) / 64 + NBH_GW_EXTRA_TASKS; }
 
+/* Min-hierarchy over the adjacent prefixes, in dadj's allocation after its
+ * n entries: level 0 is dadj[0..M), level L >= 1 holds the minimum of each 16
+ * entries of level L-1 (size ceil(M / 16^L)), every level starting on a
+ * multiple of 16 entries, up to the first level of at most 16 entries. */
+struct NbhDHier {
+    int top;
+    int size[9];
+    int off[9];
+};
+__host__ __device__ inline NbhDHier nbh_dhier(int n, int M)
+{
+    NbhDHier h;
+    h.top = 0;
+    h.size[0] = M;
+    h.off[0] = 0;
+    int at = (n + 15) & ~15;
+    while (h.size[h.top] > 16 && h.top < 8) {
+        const int s = (h.size[h.top] + 15) / 16;
+        ++h.top;
+        h.size[h.top] = s;
+        h.off[h.top] = at;
+        at += (s + 15) & ~15;
+    }
+    return h;
+}
+/* dadj's allocation: n prefixes and the hierarchy's levels for M <= n */
+__host__ __device__ inline size_t nbh_dadj_cap(int n)
+{
+    size_t at = ((size_t) n + 15) & ~(size_t) 15, s = (size_t) n;
+    while (s > 16) {
+        s = (s + 15) / 16;
+        at += (s + 15) & ~(size_t) 15;
+    }
+    return at + 16;
+}
+
 #define NBH_SORT_TILE 2048   /* keys per radix-sort block: 256 threads x 8 */
 #define NBH_SCAN_TILE 4096   /* items per scan block: 1024 threads x 4 */
 

@@ -522,6 +522,7 @@ __global__ __launch_bounds__(256) void k_delta_count(int n, const unsigned long 
                                                      const DevScalars* __restrict__ scal)
 {
     __shared__ int bsim, bmax, mixed;
+    __shared__ int s_min16[16];
     if (threadIdx.x == 0) {
         bsim = -1;
         bmax = -1;
@@ -532,9 +533,11 @@ __global__ __launch_bounds__(256) void k_delta_count(int n, const unsigned long 
     const int M = scal->nmassive;
     const int sb = scal->sim_bits;
     int depth = -1, sim = -1;
+    int hv = INT_MAX;                  /* this prefix in the min-hierarchy (none: +inf) */
     if (a < M) {
         const int d = nbh_kdelta(su, simid_s, n, M, sb, a, a + 1);
         dadj[a] = d;
+        hv = d;
         int c;
         if (nbh_first_of_sim(simid_s, a)) {
             c = max(nbh_levels(d, sb), 0) + 1;
@@ -547,7 +550,19 @@ __global__ __launch_bounds__(256) void k_delta_count(int n, const unsigned long 
         sim = simid_s[a];
         if (threadIdx.x == 0) bsim = sim;
     }
+    /* the min-hierarchy's levels 1 and 2: minima of 16 and of 256 prefixes */
+    for (int o = 8; o > 0; o >>= 1) hv = min(hv, __shfl_xor(hv, o, 16));
+    const NbhDHier h = nbh_dhier(n, M);
+    if ((threadIdx.x & 15) == 0) {
+        if (h.top >= 1 && (a >> 4) < h.size[1]) dadj[h.off[1] + (a >> 4)] = hv;
+        s_min16[threadIdx.x >> 4] = hv;
+    }
     __syncthreads();
+    if (threadIdx.x == 0 && h.top >= 2 && (int) blockIdx.x < h.size[2]) {
+        int bm = INT_MAX;
+        for (int k = 0; k < 16; ++k) bm = min(bm, s_min16[k]);
+        dadj[h.off[2] + blockIdx.x] = bm;
+    }
     if (sim >= 0 && sim != bsim) mixed = 1;
     /* the workgroup's deepest level: a wave maximum, then one LDS atomic per
      * wave (not one per lane on a single address) */
@@ -803,80 +818,116 @@ __device__ __forceinline__ double nbh_rcrit2(const nbhip_ctx& c, double mx, doub
 /* ---------------------------------------------------------------------- */
 /* Karras binary radix tree (a forest across simulations)                  */
 /* ---------------------------------------------------------------------- */
-__global__ __launch_bounds__(256) void k_karras(int n, const unsigned long long* __restrict__ su,
-                                                const int* __restrict__ simid_s,
-                                                const int* __restrict__ dadj,
+/* Levels 3 and up of the adjacent prefixes' min-hierarchy (levels 1 and 2
+ * come from k_delta_count), one workgroup, level by level. */
+__global__ __launch_bounds__(1024) void k_dhier_top(int n, int* __restrict__ dadj,
+                                                    const DevScalars* __restrict__ scal)
+{
+    __shared__ NbhDHier h;
+    if (threadIdx.x == 0) h = nbh_dhier(n, scal->nmassive);
+    __syncthreads();
+    for (int L = 3; L <= h.top; ++L) {
+        for (int i = threadIdx.x; i < h.size[L]; i += 1024) {
+            int m = INT_MAX;
+            for (int t = 0; t < 16; ++t) {
+                const int c = 16 * i + t;
+                if (c < h.size[L - 1]) m = min(m, dadj[h.off[L - 1] + c]);
+            }
+            dadj[h.off[L] + i] = m;
+        }
+        __syncthreads();
+    }
+}
+
+/* Entries base .. base+15 of hierarchy level L below v (bit t: entry
+ * base + t), entries past the level's end excluded; base is a multiple of
+ * 16, so the chunk is four aligned 16-byte loads. */
+__device__ __forceinline__ unsigned nbh_dhier_below(const int* __restrict__ dadj, const NbhDHier& h,
+                                                    int L, int base, int v)
+{
+    const int4* q = reinterpret_cast<const int4*>(dadj + h.off[L] + base);
+    unsigned m = 0;
+    #pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int4 e = q[k];
+        m |= (unsigned) (e.x < v) << (4 * k) | (unsigned) (e.y < v) << (4 * k + 1) |
+             (unsigned) (e.z < v) << (4 * k + 2) | (unsigned) (e.w < v) << (4 * k + 3);
+    }
+    const int lim = h.size[L] - base;
+    return lim >= 16 ? m : (m & ((1u << max(lim, 0)) - 1u));
+}
+
+/* Nearest adjacent prefix smaller than v = dadj[i]: to the right (the
+ * first p > i; one exists, dadj[M-1] = -1) or to the left (the last p < i,
+ * -1 if none).  Up the hierarchy through the siblings of the chunk passed,
+ * then down into the first (last) chunk holding one: at most twice its
+ * height in rounds of one 64-byte chunk.  -2 on an inconsistent hierarchy. */
+__device__ __forceinline__ int nbh_nearest_smaller(const int* __restrict__ dadj, const NbhDHier& h,
+                                                   int i, int v, bool right)
+{
+    int c = i, L = 0;
+    for (;;) {
+        const int base = c & ~15, t = c & 15;
+        unsigned m = nbh_dhier_below(dadj, h, L, base, v);
+        m &= right ? ~((2u << t) - 1u) : ((1u << t) - 1u);
+        if (m) {
+            c = base + (right ? __builtin_ctz(m) : 31 - __builtin_clz(m));
+            break;
+        }
+        if (L >= h.top) return right ? -2 : -1;
+        c >>= 4;
+        ++L;
+    }
+    for (; L > 0; --L) {
+        const unsigned m = nbh_dhier_below(dadj, h, L - 1, c << 4, v);
+        if (!m) return -2;
+        c = (c << 4) + (right ? __builtin_ctz(m) : 31 - __builtin_clz(m));
+    }
+    return c;
+}
+
+/* Node of split k.  The binary radix tree over sorted keys is the minimum
+ * Cartesian tree of their adjacent prefixes dadj[0..M-2] (a key's prefix
+ * with the next; the index tie-break of equal keys makes the prefixes of a
+ * range's endpoints its least adjacent prefix, at a unique position): the
+ * node split at k covers bodies L+1 .. R, L and R the nearest smaller
+ * prefixes to its left and right, and is the left child of R or the right
+ * child of L, whichever prefix is larger.  Karras's numbering names a node
+ * by its range end it shares with its parent's split (bhi as a left child,
+ * blo as a right child, 0 for the root), so each node is written by the
+ * thread of its own split.  Identical to Karras's searches over the keys
+ * (T. Karras, HPG 2012), without reading a key. */
+__global__ __launch_bounds__(256) void k_karras(int n, const int* __restrict__ dadj,
                                                 int* __restrict__ blo, int* __restrict__ bhi,
                                                 int* __restrict__ bdelta, int* __restrict__ bparent,
-                                                const DevScalars* __restrict__ scal)
+                                                DevScalars* __restrict__ scal)
 {
-    const int i = blockIdx.x * 256 + threadIdx.x;
+    __shared__ NbhDHier h;             /* indexed by level: kept in LDS, not in scratch */
+    const int k = blockIdx.x * 256 + threadIdx.x;
     const int M = scal->nmassive;
-    if (i >= M - 1) return;
+    if (threadIdx.x == 0) h = nbh_dhier(n, M);
+    __syncthreads();
+    if (k >= M - 1) return;
     if (scal->status & NBHIP_CELL_OVERFLOW_ERROR) return;
-    const int sb = scal->sim_bits;
-#define KD(a, b) nbh_kdelta(su, simid_s, n, M, sb, (a), (b))
-    /* the neighbours' prefixes were formed by k_delta_count */
-    const int dr = dadj[i];
-    const int dl = i > 0 ? dadj[i - 1] : -1;
-    const int d = dr > dl ? 1 : -1;
-    const int dmin = d > 0 ? dl : dr;
-    /* The common prefix of sorted keys i < j is the least adjacent prefix
-     * between them (the index tie-break of equal keys included), so a short
-     * range is found in the next NBH_KW adjacent prefixes in direction d:
-     * its length l is the first of them <= dmin, its prefix the least of
-     * the l before, and its split the position of that least one (unique:
-     * two neighbouring adjacent prefixes never agree).  Longer ranges take
-     * Karras's searches over the keys, from a length the window has shown. */
-    constexpr int NBH_KW = 16;
-    int w[NBH_KW];
-    #pragma unroll
-    for (int k = 0; k < NBH_KW; ++k) {
-        const int idx = d > 0 ? i + k : i - 1 - k;
-        w[k] = (idx >= 0 && idx < M) ? dadj[idx] : -1;
+    const int v = dadj[k];
+    const int R = nbh_nearest_smaller(dadj, h, k, v, true);
+    const int Lp = nbh_nearest_smaller(dadj, h, k, v, false);
+    if (R < 0 || Lp < -1) {
+        atomicOr(&scal->status, NBHIP_ERROR);
+        return;
     }
-    int l = -1;
-    #pragma unroll
-    for (int k = NBH_KW - 1; k >= 0; --k)
-        if (w[k] <= dmin) l = k;
-    int j, dnode, gamma;
-    if (l >= 1) {
-        dnode = w[0];
-        int ks = 0;
-        #pragma unroll
-        for (int k = 1; k < NBH_KW; ++k)
-            if (k < l && w[k] < dnode) {
-                dnode = w[k];
-                ks = k;
-            }
-        j = i + l * d;
-        gamma = d > 0 ? i + ks : i - 1 - ks;
-    } else {
-        int lmax = 2 * NBH_KW;
-        while (KD(i, i + lmax * d) > dmin) lmax <<= 1;
-        int ll = 0;
-        for (int t = lmax >> 1; t >= 1; t >>= 1)
-            if (KD(i, i + (ll + t) * d) > dmin) ll += t;
-        j = i + ll * d;
-        dnode = KD(i, j);
-        int s = 0;
-        for (int div = 2;; div <<= 1) {
-            int t = (ll + div - 1) / div;
-            if (KD(i, i + (s + t) * d) > dnode) s += t;
-            if (t <= 1) break;
-        }
-        gamma = i + s * d + min(d, 0);
-    }
-#undef KD
-    const int lo = min(i, j), hi = max(i, j);
-    blo[i] = lo;
-    bhi[i] = hi;
-    bdelta[i] = dnode;
-    const int left = (gamma == lo) ? (M - 1) + gamma : gamma;
-    const int right = (gamma + 1 == hi) ? (M - 1) + gamma + 1 : gamma + 1;
-    bparent[left] = i;
-    bparent[right] = i;
-    if (i == 0) bparent[0] = -1;
+    const int lo = Lp + 1, hi = R;
+    const int vl = Lp >= 0 ? dadj[Lp] : -1;
+    const int vr = R < M - 1 ? dadj[R] : -1;
+    const int node = (vl < 0 && vr < 0) ? 0 : (vr > vl ? hi : lo);
+    blo[node] = lo;
+    bhi[node] = hi;
+    bdelta[node] = v;
+    const int left = (k == lo) ? (M - 1) + k : k;
+    const int right = (k + 1 == hi) ? (M - 1) + k + 1 : k + 1;
+    bparent[left] = node;
+    bparent[right] = node;
+    if (node == 0) bparent[0] = -1;
 }
 
 /* ---------------------------------------------------------------------- */
@@ -1851,8 +1902,9 @@ void nbh_build(nbhip_state* st, int graph_safe)
     hipLaunchKernelGGL(k_sim_roots, dim3(g256), dim3(256), 0, s, st->simid_s, st->cincl,
                        st->node_cap, st->sims, st->nkid, st->scal);
     hipLaunchKernelGGL(k_sim_ends, dim3(1), dim3(256), 0, s, st->nsim, st->sims, st->scal);
-    hipLaunchKernelGGL(k_karras, dim3(g256), dim3(256), 0, s, n, st->su, st->simid_s, st->dadj,
-                       st->blo, st->bhi, st->bdelta, st->bparent, st->scal);
+    hipLaunchKernelGGL(k_dhier_top, dim3(1), dim3(1024), 0, s, n, st->dadj, st->scal);
+    hipLaunchKernelGGL(k_karras, dim3(g256), dim3(256), 0, s, n, st->dadj, st->blo, st->bhi,
+                       st->bdelta, st->bparent, st->scal);
     hipLaunchKernelGGL(k_emit_cells, dim3(g256), dim3(256), 0, s, st->blo, st->bhi, st->bdelta,
                        st->bparent, st->dadj, st->cincl, st->su, st->simid_s, n, st->node,
                        st->nmid, st->nlevel, st->nparent, st->nsim_node, st->sims, st->nkid,
