@@ -100,8 +100,8 @@ static void nbh_free_all(nbhip_state* st)
     void* ptrs[] = { st->px,   st->py,     st->pz,    st->vx,      st->vy,     st->vz,
                      st->m,    st->ax,     st->ay,    st->az,      st->simid,  st->key,
                      st->key_alt, st->perm, st->perm_alt, st->hist, st->dtot,  st->spos,
-                     st->simid_s, st->su,  st->dadj,  st->cincl,   st->scan_part, st->blo,
-                     st->bhi,  st->bdelta, st->bparent, st->node,  st->nquad,  st->nmid,
+                     st->simid_s, st->su,  st->dadj,  st->cincl,   st->scan_part,
+                     st->node,  st->nquad,  st->nmid,
                      st->nlevel, st->nparent, st->nsim_node, st->level_list, st->nodef,
                      st->scal, st->sims, st->ignore, st->hist_buf, st->disp_buf, st->pos4,
                      st->exact_part, st->nodeg, st->nkid, st->kbase, st->best_buf,
@@ -209,7 +209,6 @@ static unsigned nbh_create(const nbhip_ctx* ctx, int device, int nsim, const int
     A(dtot, 256);
     A(pos4, n); A(spos, n); A(simid_s, n); A(su, 3 * n); A(dadj, nbh_dadj_cap(n)); A(cincl, n);
     A(scan_part, std::max<size_t>(4096, (nc + NBH_SCAN_TILE - 1) / NBH_SCAN_TILE));
-    A(blo, n); A(bhi, n); A(bdelta, n); A(bparent, 2 * n);
     A(node, nc); A(nquad, 6 * nc); A(nmid, 3 * nc); A(nlevel, nc); A(nparent, nc);
     A(nsim_node, nc); A(level_list, nc); A(nnext, nc);
     if (ctx->precision == NBHIP_PRECISION_MIXED || ctx->precision == NBHIP_PRECISION_MIXED_COHERENT)

@@ -6,8 +6,9 @@
  *   sort: 64-bit Morton word (levels 1-21) + 32-bit body index, ping-pong
  *   sorted bodies: spos (x y z m, 32 B) and 63-bit fixed-point coordinates
  *     su (3 x 8 B) of the massive bodies, used for the 189-bit Morton prefix
- *   binary radix tree over the sorted massive bodies (Karras 2012):
- *     blo bhi bdelta (internal nodes), bparent (internal + leaves)
+ *   adjacent prefixes dadj of the sorted massive bodies and their
+ *     min-hierarchy (the binary radix tree, Karras 2012, is never stored:
+ *     it is the Cartesian tree of dadj, found by nearest-smaller searches)
  *   node table in pre-order (the reference's More/Next threading order):
  *     NodeW (48 B: centre of mass, mass, rcrit2, Next index, body) + quad
  *     (48 B) + build-only level/parent/child counters/midpoint.
@@ -192,7 +193,6 @@ struct nbhip_state {
     unsigned *cincl;              /* inclusive scan of cells-per-first-body */
     unsigned *scan_part;
 
-    int *blo, *bhi, *bdelta, *bparent;
 
     NodeW *node;
     double *nquad;                /* 6 per node */

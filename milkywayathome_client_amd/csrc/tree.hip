@@ -1,6 +1,7 @@
 /*
  * Tree construction on gfx950: root-size update, Morton keys, LSD radix sort,
- * Karras binary radix tree, octree emission in pre-order and level-by-level
+ * Karras binary radix tree (as the Cartesian tree of the adjacent prefixes),
+ * octree emission in pre-order and level-by-level
  * summarisation (centre of mass, critical radius, quadrupole).
  *
  * The octree is exactly the reference's (nbody/src/nbody_tree.c): root at the
@@ -886,57 +887,10 @@ __device__ __forceinline__ int nbh_nearest_smaller(const int* __restrict__ dadj,
     return c;
 }
 
-/* Node of split k.  The binary radix tree over sorted keys is the minimum
- * Cartesian tree of their adjacent prefixes dadj[0..M-2] (a key's prefix
- * with the next; the index tie-break of equal keys makes the prefixes of a
- * range's endpoints its least adjacent prefix, at a unique position): the
- * node split at k covers bodies L+1 .. R, L and R the nearest smaller
- * prefixes to its left and right, and is the left child of R or the right
- * child of L, whichever prefix is larger.  Karras's numbering names a node
- * by its range end it shares with its parent's split (bhi as a left child,
- * blo as a right child, 0 for the root), so each node is written by the
- * thread of its own split.  Identical to Karras's searches over the keys
- * (T. Karras, HPG 2012), without reading a key. */
-__global__ __launch_bounds__(256) void k_karras(int n, const int* __restrict__ dadj,
-                                                int* __restrict__ blo, int* __restrict__ bhi,
-                                                int* __restrict__ bdelta, int* __restrict__ bparent,
-                                                DevScalars* __restrict__ scal)
-{
-    __shared__ NbhDHier h;             /* indexed by level: kept in LDS, not in scratch */
-    const int k = blockIdx.x * 256 + threadIdx.x;
-    const int M = scal->nmassive;
-    if (threadIdx.x == 0) h = nbh_dhier(n, M);
-    __syncthreads();
-    if (k >= M - 1) return;
-    if (scal->status & NBHIP_CELL_OVERFLOW_ERROR) return;
-    const int v = dadj[k];
-    const int R = nbh_nearest_smaller(dadj, h, k, v, true);
-    const int Lp = nbh_nearest_smaller(dadj, h, k, v, false);
-    if (R < 0 || Lp < -1) {
-        atomicOr(&scal->status, NBHIP_ERROR);
-        return;
-    }
-    const int lo = Lp + 1, hi = R;
-    const int vl = Lp >= 0 ? dadj[Lp] : -1;
-    const int vr = R < M - 1 ? dadj[R] : -1;
-    const int node = (vl < 0 && vr < 0) ? 0 : (vr > vl ? hi : lo);
-    blo[node] = lo;
-    bhi[node] = hi;
-    bdelta[node] = v;
-    const int left = (k == lo) ? (M - 1) + k : k;
-    const int right = (k + 1 == hi) ? (M - 1) + k + 1 : k + 1;
-    bparent[left] = node;
-    bparent[right] = node;
-    if (node == 0) bparent[0] = -1;
-}
-
 /* ---------------------------------------------------------------------- */
 /* octree emission in pre-order                                            */
 /* ---------------------------------------------------------------------- */
 struct EmitView {
-    const int* blo;
-    const int* bdelta;
-    const int* bparent;
     const int* dadj;
     const unsigned* cincl;
     int sb;
@@ -952,36 +906,28 @@ __device__ __forceinline__ int nbh_excl(const EmitView& v, int lo)
     return lo == 0 ? 0 : (int) v.cincl[lo - 1];
 }
 
-__device__ __forceinline__ int nbh_parent_levels(const EmitView& v, int k)
+/* Pre-order index of the level-P cell that contains sorted body i: its first
+ * body a follows the nearest adjacent prefix left of i below level P (a
+ * prefix reaches level P iff it is at least sb + 3P), and the cells whose
+ * first body is a precede a's leaf level by level.  -1 on an inconsistent
+ * hierarchy. */
+__device__ __forceinline__ int nbh_cell_at(const EmitView& v, const NbhDHier& h, int i, int P)
 {
-    return k == 0 ? -1 : nbh_levels(v.bdelta[v.bparent[k]], v.sb);
-}
-
-/* Pre-order index of the deepest octree cell owned by the nearest binary
- * ancestor-or-self of k that owns any cell (-1 on a corrupt tree). */
-__device__ __forceinline__ int nbh_owner_pre(const EmitView& v, int k, int M)
-{
-    for (int guard = 0; k > 0 && nbh_levels(v.bdelta[k], v.sb) <= nbh_parent_levels(v, k);
-         ++guard) {
-        k = v.bparent[k];
-        if (guard > 4 * 256 || k < 0 || k >= M - 1) return -1;
-    }
-    if (k < 0 || k >= M - 1 || nbh_levels(v.bdelta[k], v.sb) < 0) return -1;
-    const int lo = v.blo[k];
-    return lo + nbh_excl(v, lo) + (nbh_levels(v.bdelta[k], v.sb) - nbh_first_level(v, lo));
+    const int a = nbh_nearest_smaller(v.dadj, h, i, v.sb + 3 * P, false) + 1;
+    if (a < 0) return -1;
+    return a + nbh_excl(v, a) + (P - nbh_first_level(v, a));
 }
 
 __device__ __forceinline__ void nbh_emit_cell(int pre, int l, int next, int parent, int sim,
                                               double R, unsigned long long ux,
                                               unsigned long long uy, unsigned long long uz,
-                                              NodeW* __restrict__ node, double* __restrict__ nmid,
+                                              double* __restrict__ nmid,
                                               int* __restrict__ nlevel, int* __restrict__ nparent,
                                               int* __restrict__ nsim, SimTab* __restrict__ sims,
                                               int* __restrict__ nnext)
 {
     /* the cell's walk record is written whole by its summary; until then
      * its Next link lives in nnext */
-    (void) node;
     nnext[pre] = next;
     nlevel[pre] = l;
     nparent[pre] = parent;
@@ -1004,34 +950,59 @@ __device__ __forceinline__ void nbh_emit_cell(int pre, int l, int next, int pare
     else if (l >= 63) atomicOr(&sims[sim].flags, NBHIP_MAX_DEPTH_ERROR);
 }
 
-__device__ __forceinline__ void nbh_emit_cells_body(
-    const int* __restrict__ blo, const int* __restrict__ bhi, const int* __restrict__ bdelta,
-    const int* __restrict__ bparent, const int* __restrict__ dadj,
-    const unsigned* __restrict__ cincl, const unsigned long long* __restrict__ su,
-    const int* __restrict__ simid_s, int n, NodeW* __restrict__ node,
-    double* __restrict__ nmid, int* __restrict__ nlevel, int* __restrict__ nparent,
-    int* __restrict__ nsim, SimTab* __restrict__ sims, unsigned* __restrict__ nkid,
-    int* __restrict__ nrank, int* __restrict__ nnext, DevScalars* __restrict__ scal)
+/* The octree cells of split k of the binary radix tree, thread per split.
+ * That tree is the minimum Cartesian tree of the adjacent prefixes dadj[0..
+ * M-2] (a sorted key's common prefix with the next; the index tie-break of
+ * equal keys puts every range's least prefix at one position; T. Karras,
+ * HPG 2012, builds the same tree by binary searches over the keys): the
+ * node split at k covers bodies L+1 .. R, L and R the nearest prefixes
+ * smaller than dadj[k] to its left and right, and its parent is split at
+ * whichever of the two is larger.  The node owns the octree levels of its
+ * prefix below its parent's, each a cell [lo, hi] whose first body is lo;
+ * the topmost one's parent is the level-Lp cell holding lo. */
+__global__ __launch_bounds__(256) void k_emit_cells(int n, const int* __restrict__ dadj,
+                                                    const unsigned* __restrict__ cincl,
+                                                    const unsigned long long* __restrict__ su,
+                                                    const int* __restrict__ simid_s,
+                                                    double* __restrict__ nmid,
+                                                    int* __restrict__ nlevel,
+                                                    int* __restrict__ nparent,
+                                                    int* __restrict__ nsim,
+                                                    SimTab* __restrict__ sims,
+                                                    unsigned* __restrict__ nkid,
+                                                    int* __restrict__ nrank,
+                                                    int* __restrict__ nnext,
+                                                    DevScalars* __restrict__ scal)
 {
+    __shared__ NbhDHier h;             /* indexed by level: kept in LDS, not in scratch */
     const int k = blockIdx.x * 256 + threadIdx.x;
     const int M = scal->nmassive;
+    if (threadIdx.x == 0) h = nbh_dhier(n, M);
+    __syncthreads();
     if (k >= M - 1) return;
     if (scal->status & NBHIP_CELL_OVERFLOW_ERROR) return;
-    EmitView v = { blo, bdelta, bparent, dadj, cincl, scal->sim_bits };
-    const int Lk = nbh_levels(bdelta[k], v.sb);
-    const int Lp = nbh_parent_levels(v, k);
+    const EmitView v = { dadj, cincl, scal->sim_bits };
+    const int d = dadj[k];
+    const int R = nbh_nearest_smaller(dadj, h, k, d, true);
+    const int Lft = nbh_nearest_smaller(dadj, h, k, d, false);
+    if (R < 0 || Lft < -1) {
+        atomicOr(&scal->status, NBHIP_ERROR);
+        return;
+    }
+    const int lo = Lft + 1, hi = R;
+    const int Lk = nbh_levels(d, v.sb);
+    const int Lp = nbh_levels(max(Lft >= 0 ? dadj[Lft] : -1, R < M - 1 ? dadj[R] : -1), v.sb);
     if (Lk <= Lp) return;
-    const int lo = blo[k], hi = bhi[k];
     const int base = lo + nbh_excl(v, lo) - nbh_first_level(v, lo);
     const int next = hi + 1 + (int) cincl[hi];
     const int sim = simid_s[lo];
-    const double R = sims[sim].rsize;
+    const double Rs = sims[sim].rsize;
     const unsigned long long ux = su[3 * (size_t) lo], uy = su[3 * (size_t) lo + 1],
                              uz = su[3 * (size_t) lo + 2];
     const int nn = scal->nnode;
     int top_parent = -1;
-    if (Lp + 1 > 0) {
-        top_parent = nbh_owner_pre(v, bparent[k], M);
+    if (Lp >= 0) {
+        top_parent = nbh_cell_at(v, h, lo, Lp);
         if (top_parent < 0) {
             atomicOr(&scal->status, NBHIP_ERROR);
             return;
@@ -1044,43 +1015,17 @@ __device__ __forceinline__ void nbh_emit_cells_body(
             return;
         }
         const int par = (l == Lp + 1) ? top_parent : pre - 1;
-        nbh_emit_cell(pre, l, next, par, sim, R, ux, uy, uz, node, nmid, nlevel, nparent, nsim,
-                      sims, nnext);
+        nbh_emit_cell(pre, l, next, par, sim, Rs, ux, uy, uz, nmid, nlevel, nparent, nsim, sims,
+                      nnext);
         /* the cell's rank among its parent's children (arrival order; the
          * summaries sort each child list into octant order) */
         if (par >= 0) nrank[pre] = (int) atomicAdd(&nkid[par], 1u);
     }
 }
 
-__global__ __launch_bounds__(256) void k_emit_cells(const int* __restrict__ blo,
-                                                    const int* __restrict__ bhi,
-                                                    const int* __restrict__ bdelta,
-                                                    const int* __restrict__ bparent,
-                                                    const int* __restrict__ dadj,
-                                                    const unsigned* __restrict__ cincl,
-                                                    const unsigned long long* __restrict__ su,
-                                                    const int* __restrict__ simid_s, int n,
-                                                    NodeW* __restrict__ node,
-                                                    double* __restrict__ nmid,
-                                                    int* __restrict__ nlevel,
-                                                    int* __restrict__ nparent,
-                                                    int* __restrict__ nsim,
-                                                    SimTab* __restrict__ sims,
-                                                    unsigned* __restrict__ nkid,
-                                                    int* __restrict__ nrank,
-                                                    int* __restrict__ nnext,
-                                                    DevScalars* __restrict__ scal)
-{
-    nbh_emit_cells_body(blo, bhi, bdelta, bparent, dadj, cincl, su, simid_s, n, node, nmid,
-                        nlevel, nparent, nsim, sims, nkid, nrank, nnext, scal);
-}
-
 /* Body leaves; a simulation with a single massive body gets its root cell
  * here (it owns no binary node). */
-__global__ __launch_bounds__(256) void k_emit_leaves(const int* __restrict__ blo,
-                                                     const int* __restrict__ bdelta,
-                                                     const int* __restrict__ bparent,
-                                                     const int* __restrict__ dadj,
+__global__ __launch_bounds__(256) void k_emit_leaves(int n, const int* __restrict__ dadj,
                                                      const unsigned* __restrict__ cincl,
                                                      const double4* __restrict__ spos,
                                                      const int* __restrict__ simid_s,
@@ -1096,11 +1041,14 @@ __global__ __launch_bounds__(256) void k_emit_leaves(const int* __restrict__ blo
                                                      int* __restrict__ nnext,
                                                      DevScalars* __restrict__ scal)
 {
+    __shared__ NbhDHier h;             /* indexed by level: kept in LDS, not in scratch */
     const int s = blockIdx.x * 256 + threadIdx.x;
     const int M = scal->nmassive;
+    if (threadIdx.x == 0) h = nbh_dhier(n, M);
+    __syncthreads();
     if (s >= M) return;
     if (scal->status & NBHIP_CELL_OVERFLOW_ERROR) return;
-    EmitView v = { blo, bdelta, bparent, dadj, cincl, scal->sim_bits };
+    const EmitView v = { dadj, cincl, scal->sim_bits };
     const int pre = s + (int) cincl[s];
     const int sim = simid_s[s];
     const bool singleton = nbh_first_of_sim(simid_s, s) && (s + 1 == M || simid_s[s + 1] != sim);
@@ -1108,10 +1056,12 @@ __global__ __launch_bounds__(256) void k_emit_leaves(const int* __restrict__ blo
     if (singleton) {
         parent = pre - 1;
         if (parent >= 0)
-            nbh_emit_cell(parent, 0, pre + 1, -1, sim, sims[sim].rsize, 0ull, 0ull, 0ull, node,
-                          nmid, nlevel, nparent, nsim, sims, nnext);
+            nbh_emit_cell(parent, 0, pre + 1, -1, sim, sims[sim].rsize, 0ull, 0ull, 0ull, nmid,
+                          nlevel, nparent, nsim, sims, nnext);
     } else {
-        parent = nbh_owner_pre(v, bparent[(M - 1) + s], M);
+        /* the deepest cell holding s: the deeper of its two prefixes' levels */
+        const int P = max(nbh_levels(s > 0 ? dadj[s - 1] : -1, v.sb), nbh_levels(dadj[s], v.sb));
+        parent = nbh_cell_at(v, h, s, P);
     }
     if (pre < 0 || pre >= scal->nnode || parent < 0 || parent >= pre) {
         atomicOr(&scal->status, NBHIP_ERROR);
@@ -1903,16 +1853,13 @@ void nbh_build(nbhip_state* st, int graph_safe)
                        st->node_cap, st->sims, st->nkid, st->scal);
     hipLaunchKernelGGL(k_sim_ends, dim3(1), dim3(256), 0, s, st->nsim, st->sims, st->scal);
     hipLaunchKernelGGL(k_dhier_top, dim3(1), dim3(1024), 0, s, n, st->dadj, st->scal);
-    hipLaunchKernelGGL(k_karras, dim3(g256), dim3(256), 0, s, n, st->dadj, st->blo, st->bhi,
-                       st->bdelta, st->bparent, st->scal);
-    hipLaunchKernelGGL(k_emit_cells, dim3(g256), dim3(256), 0, s, st->blo, st->bhi, st->bdelta,
-                       st->bparent, st->dadj, st->cincl, st->su, st->simid_s, n, st->node,
-                       st->nmid, st->nlevel, st->nparent, st->nsim_node, st->sims, st->nkid,
-                       st->nrank, st->nnext, st->scal);
-    hipLaunchKernelGGL(k_emit_leaves, dim3(g256), dim3(256), 0, s, st->blo, st->bdelta,
-                       st->bparent, st->dadj, st->cincl, st->spos, st->simid_s, st->node,
-                       st->nmid, st->nlevel, st->nparent, st->nsim_node, st->sims, st->nodef,
-                       st->nkid, st->nrank, st->nnext, st->scal);
+    hipLaunchKernelGGL(k_emit_cells, dim3(g256), dim3(256), 0, s, n, st->dadj, st->cincl,
+                       st->su, st->simid_s, st->nmid, st->nlevel, st->nparent, st->nsim_node,
+                       st->sims, st->nkid, st->nrank, st->nnext, st->scal);
+    hipLaunchKernelGGL(k_emit_leaves, dim3(g256), dim3(256), 0, s, n, st->dadj, st->cincl,
+                       st->spos, st->simid_s, st->node, st->nmid, st->nlevel, st->nparent,
+                       st->nsim_node, st->sims, st->nodef, st->nkid, st->nrank, st->nnext,
+                       st->scal);
     /* each cell's child block (kbase) and child list (kid) */
     const int nt = nbh_blocks(st->node_cap, NBH_SCAN_TILE);
     hipLaunchKernelGGL(k_nscan_tiles, dim3(nt), dim3(1024), 0, s, st->nkid, st->kbase,
