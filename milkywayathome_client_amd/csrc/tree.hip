@@ -683,7 +683,7 @@ __global__ __launch_bounds__(1024) void k_nscan_tiles(const unsigned* __restrict
     const int base = blockIdx.x * NBH_SCAN_TILE + threadIdx.x * 4;
     unsigned v[4];
     #pragma unroll
-    for (int k = 0; k < 4; ++k) v[k] = (base + k < nn) ? in[base + k] : 0u;
+    for (int k = 0; k < 4; ++k) v[k] = (base + k < nn) ? (unsigned) __popc(in[base + k]) : 0u;
     const unsigned sum = v[0] + v[1] + v[2] + v[3];
     unsigned tot;
     unsigned ex = nbh_block1024_excl(sum, wsum, &tot);
@@ -911,11 +911,29 @@ __device__ __forceinline__ int nbh_excl(const EmitView& v, int lo)
  * prefix reaches level P iff it is at least sb + 3P), and the cells whose
  * first body is a precede a's leaf level by level.  -1 on an inconsistent
  * hierarchy. */
+__device__ __forceinline__ int nbh_first_body(const EmitView& v, const NbhDHier& h, int i, int P)
+{
+    return nbh_nearest_smaller(v.dadj, h, i, v.sb + 3 * P, false) + 1;
+}
+__device__ __forceinline__ int nbh_cell_pre(const EmitView& v, int a, int P)
+{
+    return a < 0 ? -1 : a + nbh_excl(v, a) + (P - nbh_first_level(v, a));
+}
 __device__ __forceinline__ int nbh_cell_at(const EmitView& v, const NbhDHier& h, int i, int P)
 {
-    const int a = nbh_nearest_smaller(v.dadj, h, i, v.sb + 3 * P, false) + 1;
-    if (a < 0) return -1;
-    return a + nbh_excl(v, a) + (P - nbh_first_level(v, a));
+    return nbh_cell_pre(v, nbh_first_body(v, h, i, P), P);
+}
+
+/* Octant at level l (1..63) of a key: its level-l bit of each coordinate.
+ * The children of an octree cell sit in distinct octants, so the octant is
+ * a child's slot in its parent's child mask (nkid; a cell's child count is
+ * the mask's population, its children's order in the child list the order
+ * of their octants). */
+__device__ __forceinline__ int nbh_octant(unsigned long long ux, unsigned long long uy,
+                                          unsigned long long uz, int l)
+{
+    const int b = 63 - l;
+    return (int) (((ux >> b) & 1ull) << 2 | ((uy >> b) & 1ull) << 1 | ((uz >> b) & 1ull));
 }
 
 __device__ __forceinline__ void nbh_emit_cell(int pre, int l, int next, int parent, int sim,
@@ -1017,15 +1035,19 @@ __global__ __launch_bounds__(256) void k_emit_cells(int n, const int* __restrict
         const int par = (l == Lp + 1) ? top_parent : pre - 1;
         nbh_emit_cell(pre, l, next, par, sim, Rs, ux, uy, uz, nmid, nlevel, nparent, nsim, sims,
                       nnext);
-        /* the cell's rank among its parent's children (arrival order; the
-         * summaries sort each child list into octant order) */
-        if (par >= 0) nrank[pre] = (int) atomicAdd(&nkid[par], 1u);
+        /* the cell's octant in its parent, and the parent's octant mask */
+        if (par >= 0) {
+            const int o = nbh_octant(ux, uy, uz, l);
+            nrank[pre] = o;
+            atomicOr(&nkid[par], 1u << o);
+        }
     }
 }
 
 /* Body leaves; a simulation with a single massive body gets its root cell
  * here (it owns no binary node). */
 __global__ __launch_bounds__(256) void k_emit_leaves(int n, const int* __restrict__ dadj,
+                                                     const unsigned long long* __restrict__ su,
                                                      const unsigned* __restrict__ cincl,
                                                      const double4* __restrict__ spos,
                                                      const int* __restrict__ simid_s,
@@ -1052,7 +1074,7 @@ __global__ __launch_bounds__(256) void k_emit_leaves(int n, const int* __restric
     const int pre = s + (int) cincl[s];
     const int sim = simid_s[s];
     const bool singleton = nbh_first_of_sim(simid_s, s) && (s + 1 == M || simid_s[s + 1] != sim);
-    int parent;
+    int parent, P = 0, a = s;
     if (singleton) {
         parent = pre - 1;
         if (parent >= 0)
@@ -1060,14 +1082,22 @@ __global__ __launch_bounds__(256) void k_emit_leaves(int n, const int* __restric
                           nlevel, nparent, nsim, sims, nnext);
     } else {
         /* the deepest cell holding s: the deeper of its two prefixes' levels */
-        const int P = max(nbh_levels(s > 0 ? dadj[s - 1] : -1, v.sb), nbh_levels(dadj[s], v.sb));
-        parent = nbh_cell_at(v, h, s, P);
+        P = max(nbh_levels(s > 0 ? dadj[s - 1] : -1, v.sb), nbh_levels(dadj[s], v.sb));
+        a = nbh_first_body(v, h, s, P);
+        parent = nbh_cell_pre(v, a, P);
     }
     if (pre < 0 || pre >= scal->nnode || parent < 0 || parent >= pre) {
         atomicOr(&scal->status, NBHIP_ERROR);
         return;
     }
-    nrank[pre] = (int) atomicAdd(&nkid[parent], 1u);
+    /* the leaf's slot in its parent: its octant one level down; in a
+     * level-63 cell (equal keys, all children leaves) its place among them
+     * (past 8: the overflow slot 8, which the child lists reject) */
+    const int o = P < 63 ? nbh_octant(su[3 * (size_t) s], su[3 * (size_t) s + 1],
+                                      su[3 * (size_t) s + 2], P + 1)
+                         : min(s - a, 8);
+    nrank[pre] = o;
+    atomicOr(&nkid[parent], 1u << o);
     const double4 p = spos[s];
     NodeW b;
     b.x = p.x; b.y = p.y; b.z = p.z; b.m = p.w;
@@ -1208,14 +1238,15 @@ __device__ __forceinline__ NodeG nbh_group_record(const NodeW& w, int c,
     } else {
         r.rcrit2 = __double2float_ru(w.rcrit2 * rc_scale);
         r.link = g.nsim + (int) g.kbase[c];
-        r.count = (int) g.nkid[c];
+        r.count = __popc(g.nkid[c]);
         for (int k = 0; k < 6; ++k) r.q[k] = quad ? (float) nquad[6 * (size_t) c + k] : 0.0f;
     }
     return r;
 }
 
-/* Child lists: kid[kbase[p] .. kbase[p] + nkid[p]) holds the children of
- * cell p in arrival order (nrank, from the emission). */
+/* Child lists: kid[kbase[p] .. kbase[p] + popcount(nkid[p])) holds the
+ * children of cell p in octant order (nrank: a child's octant; nkid[p]: the
+ * mask of occupied octants, both from the emission). */
 __global__ __launch_bounds__(256) void k_fill_kids(const int* __restrict__ nparent,
                                                    const int* __restrict__ nrank,
                                                    const unsigned* __restrict__ nkid,
@@ -1229,11 +1260,12 @@ __global__ __launch_bounds__(256) void k_fill_kids(const int* __restrict__ npare
         const int par = nparent[q];
         if (par < 0) continue;
         const int r = nrank[q];
-        if (par >= q || r < 0 || r >= (int) nkid[par] || r > 7) {
+        const unsigned mask = nkid[par];
+        if (par >= q || r < 0 || r > 7 || !((mask >> r) & 1u)) {
             atomicOr(&scal->status, NBHIP_ERROR);
             continue;
         }
-        kid[kbase[par] + r] = q;
+        kid[kbase[par] + __popc(mask & ((1u << r) - 1u))] = q;
     }
 }
 
@@ -1266,7 +1298,7 @@ __device__ __forceinline__ bool nbh_cell_children(const SumView& v, int p, int* 
 {
     const int level = v.nlevel[p];
     const int end = v.nnext[p];
-    const int nk = (int) v.g.nkid[p];
+    const int nk = __popc(v.g.nkid[p]);
     if (level < 0 || end <= p + 1 || end > v.nn || nk < 1 || nk > 8) return false;
     const int kb = (int) v.g.kbase[p];
     int ch[8];
@@ -1700,7 +1732,7 @@ __global__ __launch_bounds__(256) void k_group_table(const NodeW* __restrict__ n
             continue;
         }
         const int kb = (int) g.kbase[par];
-        const int nk = min((int) g.nkid[par], 8);
+        const int nk = min(__popc(g.nkid[par]), 8);
         int rank = 0;
         for (int j = 0; j < nk; ++j) rank += kid[kb + j] < q;
         g.T[g.nsim + kb + rank] = r;
@@ -1856,7 +1888,7 @@ void nbh_build(nbhip_state* st, int graph_safe)
     hipLaunchKernelGGL(k_emit_cells, dim3(g256), dim3(256), 0, s, n, st->dadj, st->cincl,
                        st->su, st->simid_s, st->nmid, st->nlevel, st->nparent, st->nsim_node,
                        st->sims, st->nkid, st->nrank, st->nnext, st->scal);
-    hipLaunchKernelGGL(k_emit_leaves, dim3(g256), dim3(256), 0, s, n, st->dadj, st->cincl,
+    hipLaunchKernelGGL(k_emit_leaves, dim3(g256), dim3(256), 0, s, n, st->dadj, st->su, st->cincl,
                        st->spos, st->simid_s, st->node, st->nmid, st->nlevel, st->nparent,
                        st->nsim_node, st->sims, st->nodef, st->nkid, st->nrank, st->nnext,
                        st->scal);
