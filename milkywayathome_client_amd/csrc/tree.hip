@@ -1470,6 +1470,23 @@ __device__ __forceinline__ bool nbh_summarize_cell(const SumView& v, int p)
 #define NBH_SPAN_SLOTS 64      /* spanning cells one block can hand up (one per level) */
 #define NBH_SPAN_THREADS 512
 #define NBH_SUM_CAP 1024       /* cells whose child lists a workgroup keeps in LDS */
+#define NBH_SUMB_REG 16        /* k_sum_blocks: table entries per thread held in registers */
+
+/* off[l] = cells of levels < l, off[64] = all (exclusive scan of the 64
+ * level counts cnt[], by the first wave; callers synchronise after) */
+__device__ __forceinline__ void nbh_level_offsets(const int* cnt, int* off)
+{
+    if (threadIdx.x < 64) {
+        const int c = cnt[threadIdx.x];
+        int x = c;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(x, o, 64);
+            if ((int) threadIdx.x >= o) x += y;
+        }
+        off[threadIdx.x] = x - c;
+        if (threadIdx.x == 63) off[64] = x;
+    }
+}
 
 /* Summaries of the cells list[0 .. off[64]), grouped by level (level l at
  * list[off[l] .. off[l+1])), deepest level first, one barrier per level.
@@ -1547,15 +1564,26 @@ __global__ __launch_bounds__(NBH_SUMB_THREADS) void k_sum_blocks(SumView v,
     }
     if (tid == 0) nspan = 0;
     __syncthreads();
+    /* the range's levels and Next links, loaded once and all at once (up to
+     * NBH_SUMB_REG entries per thread; a longer range loads them per pass) */
+    constexpr int REG = NBH_SUMB_REG;
+    const bool in_regs = P1 - P0 <= REG * NBH_SUMB_THREADS;
+    int rl[REG], rn[REG];
+    #pragma unroll
+    for (int k = 0; k < REG; ++k) {
+        const int q = P0 + tid + k * NBH_SUMB_THREADS;
+        const bool h = in_regs && q < P1;
+        rl[k] = h ? v.nlevel[q] : -1;
+        rn[k] = h ? v.nnext[q] : 0;
+    }
     /* inside cells per level; spanning cells to the second launch */
-    for (int q = P0 + tid; q < P1; q += NBH_SUMB_THREADS) {
-        const int l = v.nlevel[q];
-        if (l < 0) continue;                      /* a body's leaf */
+    auto classify = [&](int q, int l, int nx) {
+        if (l < 0) return;                        /* a body's leaf */
         if (l > 63) {
             atomicOr(&scal->status, NBHIP_ERROR);
-            continue;
+            return;
         }
-        if (v.nnext[q] <= P1) {
+        if (nx <= P1) {
             atomicAdd(&cnt[l], 1);
         } else {
             const int k = atomicAdd(&nspan, 1);
@@ -1564,26 +1592,30 @@ __global__ __launch_bounds__(NBH_SUMB_THREADS) void k_sum_blocks(SumView v,
             else
                 atomicOr(&scal->status, NBHIP_ERROR);   /* not nested: a corrupt table */
         }
+    };
+    if (in_regs) {
+        #pragma unroll
+        for (int k = 0; k < REG; ++k) classify(P0 + tid + k * NBH_SUMB_THREADS, rl[k], rn[k]);
+    } else {
+        for (int q = P0 + tid; q < P1; q += NBH_SUMB_THREADS) classify(q, v.nlevel[q], v.nnext[q]);
     }
     __syncthreads();
     if (tid == 0) span_n[blockIdx.x] = min(nspan, NBH_SPAN_SLOTS);
-    if (tid == 0) {
-        int acc = 0;
-        for (int l = 0; l < 64; ++l) {
-            off[l] = acc;
-            acc += cnt[l];
-        }
-        off[64] = acc;
-    }
+    nbh_level_offsets(cnt, off);
     __syncthreads();
     /* the block's inside cells grouped by level: in LDS, or, past
      * NBH_SUM_CAP, in the block's own slice [P0, P0 + count) of blist
      * (slices of different blocks are disjoint) */
     int* lst = off[64] <= NBH_SUM_CAP ? s_list : blist + P0;
-    for (int q = P0 + tid; q < P1; q += NBH_SUMB_THREADS) {
-        const int l = v.nlevel[q];
-        if (l < 0 || l > 63 || v.nnext[q] > P1) continue;
+    auto place = [&](int q, int l, int nx) {
+        if (l < 0 || l > 63 || nx > P1) return;
         lst[off[l] + atomicAdd(&fill[l], 1)] = q;
+    };
+    if (in_regs) {
+        #pragma unroll
+        for (int k = 0; k < REG; ++k) place(P0 + tid + k * NBH_SUMB_THREADS, rl[k], rn[k]);
+    } else {
+        for (int q = P0 + tid; q < P1; q += NBH_SUMB_THREADS) place(q, v.nlevel[q], v.nnext[q]);
     }
     __syncthreads();
     if (!nbh_sum_levels<NBH_SUMB_THREADS>(v, lst, off, s_ch)) atomicOr(&scal->status, NBHIP_ERROR);
@@ -1621,15 +1653,8 @@ __device__ void nbh_sum_handed(const SumView& v, const int* cand, int nc, int P1
         }
     }
     __syncthreads();
-    if (tid == 0) {
-        int acc = 0;
-        for (int l = 0; l < 64; ++l) {
-            off[l] = acc;
-            acc += cnt[l];
-        }
-        off[64] = acc;
-        if (out_n) *out_n = min(*nspan, NBH_SPAN_SLOTS);
-    }
+    nbh_level_offsets(cnt, off);
+    if (tid == 0 && out_n) *out_n = min(*nspan, NBH_SPAN_SLOTS);
     __syncthreads();
     for (int i = tid; i < nc; i += NT) {
         const int q = cand[i];
