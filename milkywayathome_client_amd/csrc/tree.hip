@@ -1296,14 +1296,18 @@ struct SumView {
  * summarised. */
 __device__ __forceinline__ bool nbh_cell_children(const SumView& v, int p, int* __restrict__ out)
 {
+    /* without branches (a failed check clears ok and later loads use safe
+     * indices), so that two inlined calls overlap their three rounds of
+     * dependent loads */
     const int level = v.nlevel[p];
     const int end = v.nnext[p];
     const int nk = __popc(v.g.nkid[p]);
-    if (level < 0 || end <= p + 1 || end > v.nn || nk < 1 || nk > 8) return false;
-    const int kb = (int) v.g.kbase[p];
+    const int kbr = (int) v.g.kbase[p];
+    bool ok = !(level < 0 || end <= p + 1 || end > v.nn || nk < 1 || nk > 8);
+    const int kb = ok ? kbr : 0;
     int ch[8];
     #pragma unroll
-    for (int j = 0; j < 8; ++j) ch[j] = j < nk ? v.kid[kb + j] : NBH_NO_KID;
+    for (int j = 0; j < 8; ++j) ch[j] = (ok && j < nk) ? v.kid[kb + j] : NBH_NO_KID;
     /* Batcher's odd-even merge network for 8 keys (19 compare-exchanges,
      * static indices: the list stays in registers) */
 #define NBH_CE(a, b)                                                    \
@@ -1319,23 +1323,27 @@ __device__ __forceinline__ bool nbh_cell_children(const SumView& v, int p, int* 
     NBH_CE(2, 4); NBH_CE(3, 5);
     NBH_CE(1, 2); NBH_CE(3, 4); NBH_CE(5, 6);
 #undef NBH_CE
-    if (ch[0] != p + 1) return false;
-    for (int j = 0; j < nk; ++j)
-        if (ch[j] < 0 || ch[j] >= v.nn) return false;
-    int lv[8];
+    ok = ok && ch[0] == p + 1;
     #pragma unroll
-    for (int j = 0; j < 8; ++j) lv[j] = j < nk ? v.nlevel[ch[j]] : -1;
-    int nx[8];
+    for (int j = 0; j < 8; ++j)
+        if (j < nk) ok = ok && ch[j] >= 0 && ch[j] < v.nn;
+    int lv[8], nx[8];
     #pragma unroll
-    for (int j = 0; j < 8; ++j) nx[j] = j < nk ? (lv[j] < 0 ? ch[j] + 1 : v.nnext[ch[j]]) : 0;
+    for (int j = 0; j < 8; ++j) {
+        const bool use = ok && j < nk;
+        const int c = use ? ch[j] : 0;
+        lv[j] = use ? v.nlevel[c] : -1;
+        const int nn = use ? v.nnext[c] : 0;
+        nx[j] = use ? (lv[j] < 0 ? ch[j] + 1 : nn) : 0;
+    }
     /* the list must be the Next chain p+1 -> ... -> end */
     #pragma unroll
     for (int j = 0; j < 8; ++j)
-        if (j < nk && nx[j] != (j + 1 < nk ? ch[j + 1] : end)) return false;
+        if (j < nk) ok = ok && nx[j] == (j + 1 < nk ? ch[j + 1] : end);
     #pragma unroll
     for (int j = 0; j < 8; ++j)
         out[j] = j < nk ? (int) ((unsigned) ch[j] | (lv[j] >= 0 ? NBH_KID_CELL : 0u)) : NBH_NO_KID;
-    return true;
+    return ok;
 }
 
 /* The value half: the cell's record (centre of mass, mass, critical
@@ -1502,15 +1510,24 @@ __device__ bool nbh_sum_levels(const SumView& v, const int* list, const int* off
     const int total = off[64];
     bool ok = true;
     if (total <= NBH_SUM_CAP) {
-        for (int i = tid; i < total; i += NT) {
-            int ch[8];
-            if (nbh_cell_children(v, list[i], ch)) {
+        /* two cells per round, their loads overlapped */
+        auto keep = [&](int i, bool good, const int* ch) {
+            if (good) {
                 s_ch[2 * i] = make_int4(ch[0], ch[1], ch[2], ch[3]);
                 s_ch[2 * i + 1] = make_int4(ch[4], ch[5], ch[6], ch[7]);
             } else {
                 ok = false;                               /* marked: a list never starts empty */
                 s_ch[2 * i] = make_int4(NBH_NO_KID, NBH_NO_KID, NBH_NO_KID, NBH_NO_KID);
             }
+        };
+        for (int i = tid; i < total; i += 2 * NT) {
+            const int i1 = i + NT;
+            const bool has1 = i1 < total;
+            int c0[8], c1[8];
+            const bool g0 = nbh_cell_children(v, list[i], c0);
+            const bool g1 = nbh_cell_children(v, list[has1 ? i1 : i], c1);
+            keep(i, g0, c0);
+            if (has1) keep(i1, g1, c1);
         }
         __syncthreads();
         for (int l = 63; l >= 0; --l) {
