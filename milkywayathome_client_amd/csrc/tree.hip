@@ -1759,7 +1759,7 @@ __global__ __launch_bounds__(256) void k_group_table(const NodeW* __restrict__ n
                                                      const double* __restrict__ nquad,
                                                      const int* __restrict__ nparent,
                                                      const int* __restrict__ nsim,
-                                                     const int* __restrict__ kid, GroupTab g,
+                                                     const int* __restrict__ nrank, GroupTab g,
                                                      double rc_scale, int use_quad, int node_cap,
                                                      DevScalars* __restrict__ scal)
 {
@@ -1773,11 +1773,10 @@ __global__ __launch_bounds__(256) void k_group_table(const NodeW* __restrict__ n
             g.T[nsim[q]] = r;
             continue;
         }
-        const int kb = (int) g.kbase[par];
-        const int nk = min(__popc(g.nkid[par]), 8);
-        int rank = 0;
-        for (int j = 0; j < nk; ++j) rank += kid[kb + j] < q;
-        g.T[g.nsim + kb + rank] = r;
+        /* the rank among the siblings: the occupied octants below its own
+         * (the child list's order; k_fill_kids has checked the slot) */
+        const int o = nrank[q] & 7;
+        g.T[g.nsim + g.kbase[par] + __popc(g.nkid[par] & ((1u << o) - 1u))] = r;
     }
 }
 
@@ -1982,7 +1981,7 @@ void nbh_build(nbhip_state* st, int graph_safe)
     if (st->nodeg) {
         const GroupTab g{ st->nodeg, st->nkid, st->kbase, st->nsim };
         hipLaunchKernelGGL(k_group_table, dim3(nbh_blocks(st->node_cap, 256 * 4)), dim3(256), 0,
-                           s, st->node, st->nquad, st->nparent, st->nsim_node, st->kid, g,
+                           s, st->node, st->nquad, st->nparent, st->nsim_node, st->nrank, g,
                            st->open_scale, st->ctx.use_quad, st->node_cap, st->scal);
     }
     nbh_mark(st, 4);
